@@ -1,0 +1,153 @@
+#!/usr/bin/env python3
+"""Benchmark of the grid-node pair counting hot path on one MI355X.
+
+Workload (the north-star 2PCF shape): 2iso, data and random catalogs of
+N = 1,048,576 points each, uniform FP32 in a box of side 1000 (seeded), rmax
+80, 30 bins, no periodic boundaries.  One step = the hot path over one batch:
+the device grid build of both catalogs (counting sort into cell order) plus the
+DD, RR and DR histograms in the FP32 + FP64-edge-recheck form.  Inputs are
+resident in HBM before the timed region (catalog upload is not timed).
+
+value = useful pairs per second (sum of the three unweighted histograms divided
+by the wall time of the step), the reference's "pairs/s".
+Also reported: the same step on the plain FP64 path, the roofline of the pair
+kernel (8 FP32 flops per candidate-pair evaluation against the 157.3 TFLOP/s
+FP32 vector peak), and the CPU oracle timed on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+import desi_amd as da  # noqa: E402
+from desi_amd import synth  # noqa: E402
+
+PEAK_FP32 = 157.3e12  # MI355X FP32 vector (packed) peak, FLOP/s
+FLOPS_PER_PAIR = 8    # 3 sub + 3 mul + 2 add (FMA = 2 flops)
+N = 1 << 20
+BOX = 1000.0
+RMAX = 80.0
+BINS = 30
+
+
+def run_step(e, cd, cr, precision, rebuild=True):
+    tot_pairs = 0
+    evals = 0
+    kms = 0.0
+    rech = 0
+    for hist, cy in ((da.PCF_XX, -1), (da.PCF_XX, None), (da.PCF_XY, cr)):
+        cx = cr if cy is None else cd
+        job = da.make_job(da.PCF_2ISO, hist, bins=BINS, dmax=RMAX, cat_x=cx,
+                          cat_y=-1 if cy is None else cy, precision=precision,
+                          rebuild_grid=rebuild)
+        c, _ = e.histogram(job)
+        st = e.stats()
+        tot_pairs += int(c.sum())
+        evals += st["evaluations"]
+        kms += st["kernel_ms"]
+        rech += st["rechecks"]
+    return tot_pairs, evals, kms, rech
+
+
+def cpu_baseline(d):
+    """The oracle (a C++/OpenMP restatement, FP64 canonical mode) on a bounded
+    sample: the DD histogram of the data points inside the sub-box [0,500)^3."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    sel = np.all(d[:, :3] < 500.0, axis=1)
+    sample = np.ascontiguousarray(d[sel])
+    cores = os.cpu_count() or 1
+    cores = min(cores, 16)
+    t = time.perf_counter()
+    c, _ = O.pairs(sample, None, bins=BINS, dmax=RMAX, threads=cores)
+    dt = time.perf_counter() - t
+    return {"value": float(c.sum()) / dt, "unit": "pairs/s", "cores": cores, "kind": "port",
+            "sample": f"oracle FP64 DD of the {len(sample)} data points in [0,500)^3 "
+                      f"(rmax {RMAX}, {BINS} bins), {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this engine runs the pair counting on a single "
+              "device (multi-GPU sharding is a design note only).", file=sys.stderr)
+        sys.exit(2)
+
+    d = synth.uniform(N, BOX, seed=1)
+    r = synth.uniform(N, BOX, seed=2)
+    with da.Engine(0) as e:
+        cd, cr = e.upload(d), e.upload(r)
+        for _ in range(args.warmup):
+            run_step(e, cd, cr, da.PCF_PREC_MIXED)
+        t0 = time.perf_counter()
+        pairs = evals = rech = 0
+        kms = 0.0
+        for _ in range(args.steps):
+            p, ev, k, rc = run_step(e, cd, cr, da.PCF_PREC_MIXED)
+            pairs += p
+            evals += ev
+            kms += k
+            rech += rc
+        t1 = time.perf_counter()
+        ms_step = (t1 - t0) * 1e3 / args.steps
+        value = pairs / (t1 - t0)
+
+        # Plain FP64 path, same step.
+        run_step(e, cd, cr, da.PCF_PREC_FP64)
+        t2 = time.perf_counter()
+        p64 = 0
+        k64 = 0.0
+        for _ in range(args.fp64_steps):
+            p, _, k, _ = run_step(e, cd, cr, da.PCF_PREC_FP64)
+            p64 += p
+            k64 += k
+        t3 = time.perf_counter()
+        ms_step64 = (t3 - t2) * 1e3 / args.fp64_steps
+        if p64 // args.fp64_steps != pairs // args.steps:
+            raise SystemExit("plain FP64 path and mixed path disagree on the pair count")
+
+    achieved = evals * FLOPS_PER_PAIR / (kms * 1e-3)
+    out = {
+        "metric": "pairs/s (2iso DD+RR+DR, useful pairs)",
+        "value": value,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64-recheck",
+        "data": "synthetic (uniform, seeded)",
+        "config": {"workload": "2iso N=1048576 data + 1048576 random, box 1000, rmax 80, "
+                               "30 bins, DD+RR+DR, no BPC",
+                   "n_data": N, "n_random": N, "box": BOX, "rmax": RMAX, "bins": BINS},
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None},
+        "pair_kernel_ms_per_step": kms / args.steps,
+        "evaluations_per_step": evals // args.steps,
+        "rechecks_per_step": rech // args.steps,
+        "fp64_plain_ms_per_step": ms_step64,
+        "fp64_plain_kernel_ms_per_step": k64 / args.fp64_steps,
+        "speedup_vs_fp64_plain": ms_step64 / ms_step,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(d)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,210 @@
+"""desi_amd -- MI355X (gfx950) pair / triplet histogram engine for 3PCFGPU.
+
+Python mirror of the C ABI in include/pcf_abi.h (libpcf_hip.so), used by the
+tests, the benchmark and the smoke check.  The compute path is the HIP library;
+this module only marshals host arrays through ctypes and raises when the
+native library is missing or a call fails.  There is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpcf_hip.so")
+CLI_PATH = os.path.join(HERE, "bin", "PCF")
+
+# Histogram types (argv[1] of the reference, cuda/main.cu:59).
+PCF_2ISO, PCF_2ANI, PCF_3ISO, PCF_3ANI = 0, 1, 2, 3
+TYPES = {"2iso": PCF_2ISO, "2ani": PCF_2ANI, "3iso": PCF_3ISO, "3ani": PCF_3ANI}
+# Which histogram.
+PCF_XX, PCF_XY, PCF_XXX, PCF_XXY = 0, 1, 2, 3
+# Arithmetic.
+PCF_PREC_MIXED, PCF_PREC_FP64 = 0, 1
+
+ERRORS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "out of device memory",
+          -4: "coordinates out of range", -5: "unsupported", -6: "busy"}
+
+EXPORTED_SYMBOLS = (
+    "pcf_hist_size", "pcf_create", "pcf_destroy", "pcf_last_error", "pcf_create_error",
+    "pcf_catalog_upload", "pcf_catalog_upload_device", "pcf_catalog_release",
+    "pcf_catalog_size", "pcf_histogram", "pcf_last_stats", "pcf_analytic_rr",
+    "pcf_set_hbm_budget", "pcf_submit", "pcf_poll", "pcf_wait",
+)
+
+
+class PcfJob(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("hist", ctypes.c_int32), ("bpc", ctypes.c_int32),
+                ("bins", ctypes.c_int32), ("dmax", ctypes.c_float), ("box", ctypes.c_float),
+                ("precision", ctypes.c_int32), ("cat_x", ctypes.c_int32),
+                ("cat_y", ctypes.c_int32), ("partitions", ctypes.c_int32),
+                ("cells_per_rmax", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+
+
+class PcfStats(ctypes.Structure):
+    _fields_ = [("evaluations", ctypes.c_uint64), ("rechecks", ctypes.c_uint64),
+                ("work_items", ctypes.c_uint64), ("kernel_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double), ("cells_per_rmax", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 3)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class PcfError(RuntimeError):
+    def __init__(self, rc, msg):
+        super().__init__(f"{ERRORS.get(rc, rc)}: {msg}")
+        self.rc = rc
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libpcf_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `make -C desi_amd` "
+                           "(or __graft_entry__.build())")
+    L = ctypes.CDLL(path)
+    c_i32, c_i64, vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p
+    L.pcf_hist_size.argtypes = [c_i32, c_i32]
+    L.pcf_hist_size.restype = c_i64
+    L.pcf_create.argtypes = [ctypes.POINTER(vp), c_i32]
+    L.pcf_destroy.argtypes = [vp]
+    L.pcf_destroy.restype = None
+    L.pcf_last_error.argtypes = [vp]
+    L.pcf_last_error.restype = ctypes.c_char_p
+    L.pcf_create_error.restype = ctypes.c_char_p
+    L.pcf_catalog_upload.argtypes = [vp, vp, c_i64, ctypes.POINTER(c_i32)]
+    L.pcf_catalog_upload_device.argtypes = [vp, vp, c_i64, ctypes.POINTER(c_i32)]
+    L.pcf_catalog_release.argtypes = [vp, c_i32]
+    L.pcf_catalog_size.argtypes = [vp, c_i32]
+    L.pcf_catalog_size.restype = c_i64
+    L.pcf_histogram.argtypes = [vp, ctypes.POINTER(PcfJob), vp, vp]
+    L.pcf_last_stats.argtypes = [vp, ctypes.POINTER(PcfStats)]
+    L.pcf_analytic_rr.argtypes = [vp, c_i32, c_i64, ctypes.c_float, ctypes.c_float, vp]
+    L.pcf_set_hbm_budget.argtypes = [vp, c_i64, c_i32]
+    L.pcf_submit.argtypes = [vp, ctypes.POINTER(PcfJob), ctypes.POINTER(c_i64)]
+    L.pcf_poll.argtypes = [vp, c_i64, ctypes.POINTER(c_i32)]
+    L.pcf_wait.argtypes = [vp, c_i64, vp, vp, ctypes.POINTER(PcfStats)]
+    _lib = L
+    return L
+
+
+def hist_size(type_, bins):
+    return {PCF_2ISO: bins, PCF_2ANI: bins ** 2, PCF_3ISO: bins ** 3,
+            PCF_3ANI: bins ** 5}[type_]
+
+
+def make_job(type_, hist, *, bins, dmax, cat_x, cat_y=-1, bpc=False, box=0.0,
+             precision=PCF_PREC_MIXED, cells_per_rmax=0, partitions=35, rebuild_grid=False):
+    j = PcfJob()
+    j.type = int(type_)
+    j.hist = int(hist)
+    j.bpc = int(bool(bpc))
+    j.bins = int(bins)
+    j.dmax = float(dmax)
+    j.box = float(box)
+    j.precision = int(precision)
+    j.cat_x = int(cat_x)
+    j.cat_y = int(cat_y)
+    j.partitions = int(partitions)
+    j.cells_per_rmax = int(cells_per_rmax)
+    j.reserved[0] = int(bool(rebuild_grid))
+    return j
+
+
+class Engine:
+    """One pcf_ctx on one HIP device."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        self.ctx = ctypes.c_void_p()
+        rc = self.lib.pcf_create(ctypes.byref(self.ctx), int(device))
+        if rc != 0:
+            raise PcfError(rc, self.lib.pcf_create_error().decode())
+
+    def close(self):
+        if self.ctx:
+            self.lib.pcf_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise PcfError(rc, self.lib.pcf_last_error(self.ctx).decode())
+
+    def upload(self, xyzw):
+        a = np.ascontiguousarray(xyzw, dtype=np.float32)
+        if a.ndim != 2 or a.shape[1] != 4:
+            raise ValueError("catalog must be (n, 4) float32: x, y, z, w")
+        cid = ctypes.c_int32()
+        self._check(self.lib.pcf_catalog_upload(self.ctx, a.ctypes.data, len(a),
+                                                ctypes.byref(cid)))
+        return cid.value
+
+    def upload_device(self, dev_ptr, n):
+        cid = ctypes.c_int32()
+        self._check(self.lib.pcf_catalog_upload_device(self.ctx, ctypes.c_void_p(dev_ptr), n,
+                                                       ctypes.byref(cid)))
+        return cid.value
+
+    def release(self, cid):
+        self._check(self.lib.pcf_catalog_release(self.ctx, cid))
+
+    def histogram(self, job):
+        n = hist_size(job.type, job.bins)
+        c = np.zeros(n, np.uint64)
+        w = np.zeros(n, np.float64)
+        self._check(self.lib.pcf_histogram(self.ctx, ctypes.byref(job), c.ctypes.data,
+                                           w.ctypes.data))
+        return c, w
+
+    def stats(self):
+        s = PcfStats()
+        self._check(self.lib.pcf_last_stats(self.ctx, ctypes.byref(s)))
+        return s.as_dict()
+
+    def analytic_rr(self, bins, np_, dmax, box):
+        rr = np.zeros(bins, np.float64)
+        self._check(self.lib.pcf_analytic_rr(self.ctx, bins, np_, dmax, box, rr.ctypes.data))
+        return rr
+
+    # ---- asynchronous (serving) ------------------------------------------
+    def set_hbm_budget(self, nbytes, max_inflight=4):
+        self._check(self.lib.pcf_set_hbm_budget(self.ctx, int(nbytes), int(max_inflight)))
+
+    def submit(self, job):
+        t = ctypes.c_int64()
+        self._check(self.lib.pcf_submit(self.ctx, ctypes.byref(job), ctypes.byref(t)))
+        return t.value
+
+    def poll(self, ticket):
+        d = ctypes.c_int32()
+        self._check(self.lib.pcf_poll(self.ctx, ticket, ctypes.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket, type_, bins):
+        n = hist_size(type_, bins)
+        c = np.zeros(n, np.uint64)
+        w = np.zeros(n, np.float64)
+        s = PcfStats()
+        self._check(self.lib.pcf_wait(self.ctx, ticket, c.ctypes.data, w.ctypes.data,
+                                      ctypes.byref(s)))
+        return c, w, s.as_dict()
+
+
+def read_catalog(path, n=None):
+    """The reference's 4-column text catalog (readme.md:36-40)."""
+    a = np.loadtxt(path, dtype=np.float32, ndmin=2, max_rows=n)
+    if a.shape[1] != 4:
+        raise ValueError(f"{path}: expected 4 columns, got {a.shape[1]}")
+    return a

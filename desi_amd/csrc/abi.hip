@@ -1,0 +1,644 @@
+// abi.hip -- host side of libpcf_hip.so: contexts, resident catalogs, job
+// set-up (geometry, exact bin thresholds, FP32 bin table) and the C ABI
+// declared in include/pcf_abi.h.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pcf_abi.h"
+#include "pcf_internal.h"
+#include "pcf_ctx.h"
+
+using namespace pcf;
+
+static thread_local std::string g_create_error;
+
+// ---------------------------------------------------------------------------
+// Exact FP64 bin thresholds and the FP32 bulk table
+// ---------------------------------------------------------------------------
+namespace pcf {
+
+double ds_floored(int bins, float dmax) {
+  // ds = floor(bn/dmax*1e6)/1e6 in double (cuda/kernels/2iso_k.cuh:37).
+  return std::floor(((double)bins / (double)dmax) * 1000000.0) / 1000000.0;
+}
+
+static inline uint64_t dbits(double x) {
+  uint64_t u;
+  std::memcpy(&u, &x, 8);
+  return u;
+}
+static inline double dfrom(uint64_t u) {
+  double x;
+  std::memcpy(&x, &u, 8);
+  return x;
+}
+
+// T[k] = smallest double d2 >= 0 with (int)(sqrt(d2)*ds) >= k, by bisection
+// over the ordered bit patterns of non-negative doubles (host sqrt is IEEE
+// correctly rounded).  The map d2 -> (int)(sqrt(d2)*ds) is monotone, so
+// bin(d2) = #{k >= 1 : T[k] <= d2} reproduces it exactly without a device sqrt.
+void bin_thresholds(int bins, double ds, double* T) {
+  T[0] = 0.0;
+  for (int k = 1; k <= bins; ++k) {
+    auto f = [&](double x) { return (int)(std::sqrt(x) * ds) >= k; };
+    uint64_t lo = 0;  // f(0) false for k >= 1
+    double h = ((double)k / ds) * ((double)k / ds) * 4.0 + 1.0;
+    while (!f(h)) h *= 2;
+    uint64_t hi = dbits(h);
+    while (hi - lo > 1) {
+      uint64_t mid = lo + (hi - lo) / 2;
+      if (f(dfrom(mid))) hi = mid;
+      else lo = mid;
+    }
+    T[k] = dfrom(hi);
+  }
+}
+
+void make_rules(int type, int bpc, int auto_, int bins, float dmax, float box, BinRules* r) {
+  std::memset(r, 0, sizeof(*r));
+  const bool ani = (type == PCF_2ANI);
+  float dd_max = dmax * dmax;  // FP32 product as in the kernels
+  r->R = (double)dd_max;
+  r->L = (double)box;
+  r->bins = bins;
+  r->ani = ani ? 1 : 0;
+  if (!ani) {
+    r->ds = bpc ? (double)bins / (double)dmax : ds_floored(bins, dmax);
+    r->le = 0;
+    r->zero = auto_ ? 1 : 0;
+  } else {
+    r->ds = ds_floored(bins, dmax);
+    r->le = bpc ? 1 : 0;
+    r->zero = (!bpc || auto_) ? 2 : 0;
+  }
+  bin_thresholds(bins, r->ds, r->T);
+}
+
+// Exact class of a double d2 for one component: bin in [0,bins) or -1 (out).
+static int host_class(double d2, const BinRules& r) {
+  if (r.le ? !(d2 <= r.R) : !(d2 < r.R)) return -1;
+  if (d2 >= r.T[r.bins]) return -1;
+  int b = 0;
+  for (int k = 1; k < r.bins; ++k)
+    if (r.T[k] <= d2) b = k;
+  return b;
+}
+
+// Bound on |d2_fp32 - d2_exact| for the bulk arithmetic of the pair kernels
+// (per axis one FP32 subtraction after an optional image shift xi - k*L, then
+// dx*dx, fma, fma).  u = 2^-24.  A = max|coordinate| + L when periodic images
+// are applied, else 0 (a plain FP32 difference has relative error <= u).
+//   E(d2) = 5u d2 + 3.5u A sqrt(d2) + 6u^2 A^2      (doubled for margin)
+static double err_bound(double d2, double A) {
+  const double u = std::ldexp(1.0, -24);
+  return 2.0 * (5 * u * d2 + 3.5 * u * A * std::sqrt(d2) + 6 * u * u * A * A) + 1e-300;
+}
+
+int build_table(const BinRules& r, double A, int nt, std::vector<uint16_t>* table, float* inv_h_out) {
+  const int bins = r.bins;
+  const bool ani = r.ani != 0;
+  double U = std::min(r.R, r.T[bins]);
+  double vmax = U * (1.0 + 1e-3) + 4.0 * err_bound(2 * U + 1, A) + 1e-30;
+  float inv_h = (float)((double)(nt - 1) / vmax);
+  const double ih = (double)inv_h;
+  const double u = std::ldexp(1.0, -24);
+  table->assign(nt, 0);
+  const uint16_t out_code = ani ? kAniOut : (uint16_t)bins;
+  const uint16_t amb_code = ani ? kAniAmb : (uint16_t)(bins + 1);
+  for (int t = 0; t < nt; ++t) {
+    // FP32 values v with min(cvt_rz(fl(v*inv_h)), nt-1) == t.
+    double v_lo = (double)t * (1.0 - 4 * u) / ih;
+    double v_hi = (t == nt - 1) ? std::numeric_limits<double>::infinity()
+                                : (double)(t + 1) * (1.0 + 4 * u) / ih;
+    double d_lo = v_lo - 2.0 * err_bound(2 * v_lo + 1, A);
+    double d_hi = std::isinf(v_hi) ? v_hi : v_hi + 2.0 * err_bound(2 * v_hi + 1, A);
+    uint16_t code;
+    if (r.zero != 0 && d_lo <= 0.0) {
+      code = amb_code;  // exact zero is excluded, anything above is bin 0
+    } else {
+      int c_lo = host_class(std::max(d_lo, 0.0), r);
+      int c_hi = std::isinf(d_hi) ? -1 : host_class(d_hi, r);
+      if (c_lo == c_hi) code = (c_lo < 0) ? out_code : (uint16_t)c_lo;
+      else code = amb_code;
+    }
+    (*table)[t] = code;
+  }
+  if ((*table)[nt - 1] != out_code) return -1;
+  *inv_h_out = inv_h;
+  return 0;
+}
+
+}  // namespace pcf
+
+namespace {
+
+int finish_upload(pcf_ctx* c, Catalog* cat) {
+  const int nb = 512;
+  HIPCHK(c, c->stats_partials.ensure(sizeof(float) * nb * 10));
+  float* part = c->stats_partials.as<float>();
+  if (cat->n > 0) {
+    HIPCHK(c, launch_catalog_stats(cat->raw, cat->n, part, nb, c->stream));
+    std::vector<float> h(nb * 10);
+    HIPCHK(c, hipMemcpyAsync(h.data(), part, sizeof(float) * nb * 10, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int a = 0; a < 3; ++a) {
+      cat->lo[a] = INFINITY;
+      cat->hi[a] = -INFINITY;
+    }
+    cat->maxabs = 0;
+    cat->all_w1 = true;
+    cat->finite = true;
+    for (int b = 0; b < nb; ++b) {
+      const float* p = &h[b * 10];
+      for (int a = 0; a < 3; ++a) {
+        cat->lo[a] = std::min(cat->lo[a], p[a]);
+        cat->hi[a] = std::max(cat->hi[a], p[3 + a]);
+      }
+      cat->maxabs = std::max(cat->maxabs, p[6]);
+      if (p[7] < 1.0f) cat->all_w1 = false;
+      if (p[8] < 1.0f) cat->finite = false;
+    }
+  }
+  if (!cat->finite) return fail(c, PCF_ERR_RANGE, "catalog holds non-finite values");
+  return PCF_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Geometry and grids
+// ---------------------------------------------------------------------------
+namespace pcf {
+
+int choose_geometry(const Catalog* X, const Catalog* Y, float dmax, int bpc, float box,
+                    int cells_per_rmax, GridGeom* g, int K[3], float* reach_out, int* m_out) {
+  const double u = std::ldexp(1.0, -24);
+  double lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = X->n ? X->lo[a] : 0;
+    hi[a] = X->n ? X->hi[a] : 1;
+    if (Y && Y->n) {
+      lo[a] = std::min(lo[a], (double)Y->lo[a]);
+      hi[a] = std::max(hi[a], (double)Y->hi[a]);
+    }
+  }
+  int64_t ny = Y ? Y->n : X->n;
+  int m = cells_per_rmax;
+  if (m <= 0) {
+    // Largest m (cells per rmax) whose mean occupancy still fills most of a
+    // 64-lane tile.
+    double vol = 1.0;
+    for (int a = 0; a < 3; ++a) vol *= bpc ? (double)box : std::max(hi[a] - lo[a], (double)dmax);
+    double dens = (double)std::max<int64_t>(ny, 1) / vol;
+    m = 1;
+    for (int t = 4; t >= 1; --t) {
+      double cs = dmax / t;
+      if (dens * cs * cs * cs >= 32.0) { m = t; break; }
+    }
+  }
+  if (m > 8) m = 8;
+  double ext_max = 0;
+  for (int a = 0; a < 3; ++a) ext_max = std::max(ext_max, bpc ? (double)box : hi[a] - lo[a]);
+  double cs_target = (double)dmax / m;
+  g->bpc = bpc;
+  g->L = box;
+  for (int iter = 0; iter < 64; ++iter) {
+    int64_t ncell = 1;
+    for (int a = 0; a < 3; ++a) {
+      if (bpc) {
+        int P = (int)std::floor((double)box / cs_target);
+        if (P < 3) P = 3;
+        g->P[a] = P;
+        g->lo[a] = 0.f;
+        g->cs[a] = (float)((double)box / P);
+      } else {
+        double ext = hi[a] - lo[a];
+        int P = (int)std::floor(ext / cs_target) + 1;
+        if (P < 1) P = 1;
+        g->P[a] = P;
+        g->lo[a] = (float)lo[a];
+        g->cs[a] = (float)cs_target;
+      }
+      g->inv_cs[a] = 1.0f / g->cs[a];
+      ncell *= g->P[a];
+    }
+    if (ncell <= (int64_t)1 << 22) break;
+    cs_target *= 1.25;
+  }
+  // Slack for the FP32 cell assignment (a point may sit ~3u*(extent+cs) outside
+  // its nominal cell) and for the FP32 pruning arithmetic.
+  double delta = 4 * u * (ext_max + std::fabs(lo[0]) + std::fabs(lo[1]) + std::fabs(lo[2]) +
+                          2 * cs_target) + 1e-7;
+  double reach = (double)dmax * (1.0 + 1e-5) + 2 * delta;
+  for (int a = 0; a < 3; ++a) {
+    reach = std::max(reach, reach);
+    K[a] = (int)std::ceil(reach / g->cs[a]);
+    if (bpc && K[a] >= g->P[a]) return -1;
+  }
+  *reach_out = (float)(reach * (1.0 + 1e-6));
+  if (m_out) *m_out = m;
+  return 0;
+}
+
+DevGrid* get_grid(pcf_ctx* c, Catalog* cat, const GridGeom& g, bool force_rebuild, int* rc) {
+  for (auto& gr : cat->grids) {
+    if (gr->valid && gr->g.same(g)) {
+      if (force_rebuild) {
+        hipError_t e = build_grid(cat->raw, cat->n, g, gr.get(), c->scratch_cells,
+                                  c->scratch_perm, c->stream);
+        if (e != hipSuccess) {
+          *rc = hipfail(c, e, "build_grid");
+          return nullptr;
+        }
+      }
+      return gr.get();
+    }
+  }
+  // Keep a handful of geometries per catalog (DR/RR reuse, serving).
+  if (cat->grids.size() >= 4) {
+    DevGrid* old = cat->grids.front().get();
+    if (old->pts) (void)hipFree(old->pts);
+    if (old->pts64) (void)hipFree(old->pts64);
+    if (old->cell_start) (void)hipFree(old->cell_start);
+    cat->grids.erase(cat->grids.begin());
+  }
+  auto gr = std::make_unique<DevGrid>();
+  hipError_t e = build_grid(cat->raw, cat->n, g, gr.get(), c->scratch_cells, c->scratch_perm,
+                            c->stream);
+  if (e != hipSuccess) {
+    if (gr->pts) (void)hipFree(gr->pts);
+    if (gr->cell_start) (void)hipFree(gr->cell_start);
+    *rc = (e == hipErrorOutOfMemory) ? fail(c, PCF_ERR_NOMEM, "grid allocation")
+                                     : hipfail(c, e, "build_grid");
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  cat->grids.push_back(std::move(gr));
+  return cat->grids.back().get();
+}
+
+void free_catalog(Catalog* cat) {
+  if (cat->raw) (void)hipFree(cat->raw);
+  cat->raw = nullptr;
+  for (auto& gr : cat->grids) {
+    if (gr->pts) (void)hipFree(gr->pts);
+    if (gr->pts64) (void)hipFree(gr->pts64);
+    if (gr->cell_start) (void)hipFree(gr->cell_start);
+  }
+  cat->grids.clear();
+}
+
+}  // namespace pcf
+
+// ---------------------------------------------------------------------------
+// Pair histograms (2iso / 2ani)
+// ---------------------------------------------------------------------------
+namespace {
+
+int lds_copies(int nrows, int bytes_per_row_copy, int budget) {
+  int c = 64;
+  while (c > 1 && nrows * c * bytes_per_row_copy > budget) c >>= 1;
+  return c;
+}
+
+int ilog2(int v) {
+  int s = 0;
+  while ((1 << s) < v) ++s;
+  return s;
+}
+
+int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights) {
+  const bool auto_ = job->hist == PCF_XX;
+  Catalog* X = get_cat(c, job->cat_x);
+  Catalog* Y = auto_ ? X : get_cat(c, job->cat_y);
+  if (!X || !Y) return fail(c, PCF_ERR_ARG, "unknown catalog id");
+  const int bins = job->bins;
+  const bool ani = job->type == PCF_2ANI;
+  if (bins <= 0 || bins > kMaxBinsIso || (ani && bins > 120))
+    return fail(c, PCF_ERR_ARG, "bins out of range");
+  if (!(job->dmax > 0) || !std::isfinite(job->dmax)) return fail(c, PCF_ERR_ARG, "dmax must be > 0");
+  const int bpc = job->bpc ? 1 : 0;
+  const float L = job->box;
+  if (bpc) {
+    if (!(L > 0)) return fail(c, PCF_ERR_ARG, "box side required for periodic boundaries");
+    if (!(2.0 * job->dmax < L))
+      return fail(c, PCF_ERR_UNSUPPORTED, "periodic boundaries need 2*dmax < box side");
+    for (Catalog* cc : {X, Y})
+      for (int a = 0; a < 3; ++a)
+        if (cc->n && (cc->lo[a] < 0.f || !(cc->hi[a] < L)))
+          return fail(c, PCF_ERR_RANGE, "coordinates outside [0, box) with periodic boundaries");
+  }
+  const int nflat = ani ? bins * bins : bins;
+  const bool weighted = !(X->all_w1 && Y->all_w1);
+  const bool plain = job->precision == PCF_PREC_FP64;
+
+  GridGeom g;
+  int K[3];
+  float reach;
+  int m = 0;
+  if (choose_geometry(X, auto_ ? nullptr : Y, job->dmax, bpc, L, job->cells_per_rmax, &g, K,
+                      &reach, &m) != 0)
+    return fail(c, PCF_ERR_UNSUPPORTED, "geometry");
+  const bool rebuild = job->reserved[0] != 0;
+  int rc = PCF_OK;
+  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  DevGrid* gx = get_grid(c, X, g, rebuild, &rc);
+  if (!gx) return rc;
+  DevGrid* gy = auto_ ? gx : get_grid(c, Y, g, rebuild, &rc);
+  if (!gy) return rc;
+  if (plain) {
+    HIPCHK(c, build_grid64(gx, c->stream));
+    if (!auto_) HIPCHK(c, build_grid64(gy, c->stream));
+  }
+
+  // Rules, thresholds and the FP32 table.
+  static thread_local BinRules rules;
+  make_rules(job->type, bpc, auto_, bins, job->dmax, L, &rules);
+  double A = bpc ? (double)std::max(X->maxabs, Y->maxabs) + (double)L : 0.0;
+  const int nt = 8192;
+  static thread_local std::vector<uint16_t> table;
+  float inv_h = 0.f;
+  if (!plain && build_table(rules, A, nt, &table, &inv_h) != 0)
+    return fail(c, PCF_ERR_UNSUPPORTED, "bin table");
+  HIPCHK(c, c->rules_buf.ensure(sizeof(BinRules)));
+  HIPCHK(c, c->table_buf.ensure(sizeof(uint16_t) * nt));
+  HIPCHK(c, hipMemcpyAsync(c->rules_buf.p, &rules, sizeof(BinRules), hipMemcpyHostToDevice,
+                           c->stream));
+  if (!plain)
+    HIPCHK(c, hipMemcpyAsync(c->table_buf.p, table.data(), sizeof(uint16_t) * nt,
+                             hipMemcpyHostToDevice, c->stream));
+
+  // Work list.
+  WorkListArgs wa;
+  wa.x_cell_start = gx->cell_start;
+  wa.y_cell_start = gy->cell_start;
+  wa.g = g;
+  for (int a = 0; a < 3; ++a) wa.K[a] = K[a];
+  wa.reach = reach;
+  wa.auto_ = auto_ ? 1 : 0;
+  int nitems = 0;
+  HIPCHK(c, build_pair_worklist(wa, c->wl_counts, c->wl_items, &nitems, c->stream));
+
+  // Histogram + counters.
+  const size_t hist_bytes = sizeof(unsigned long long) * nflat;
+  const size_t w_bytes = sizeof(double) * nflat;
+  HIPCHK(c, c->hist_buf.ensure(hist_bytes + w_bytes + 64));
+  unsigned long long* d_hist = c->hist_buf.as<unsigned long long>();
+  double* d_w = reinterpret_cast<double*>(c->hist_buf.as<char>() + hist_bytes);
+  unsigned long long* d_stats =
+      reinterpret_cast<unsigned long long*>(c->hist_buf.as<char>() + hist_bytes + w_bytes);
+  int* d_counter = reinterpret_cast<int*>(d_stats + 4);
+  HIPCHK(c, launch_zero(c->hist_buf.p, hist_bytes + w_bytes + 64, c->stream));
+
+  PairLaunch pl{};
+  pl.xp = gx->pts;
+  pl.yp = gy->pts;
+  pl.xp64 = gx->pts64;
+  pl.yp64 = gy->pts64;
+  pl.items = c->wl_items.as<PairItem>();
+  pl.nitems = nitems;
+  pl.counter = d_counter;
+  pl.table = c->table_buf.as<uint16_t>();
+  pl.nt = nt;
+  pl.inv_h = inv_h;
+  pl.L = L;
+  pl.bins = bins;
+  pl.nflat = nflat;
+  const int lds_budget = 48 * 1024;
+  const int rows = nflat + 2;
+  const int per = 4 + (weighted ? 8 : 0);
+  pl.cshift = ilog2(lds_copies(rows, per, lds_budget));
+  pl.hist = d_hist;
+  pl.whist = d_w;
+  pl.stats = d_stats;
+  pl.rules = c->rules_buf.as<BinRules>();
+  pl.ani = ani ? 1 : 0;
+  pl.weighted = weighted ? 1 : 0;
+  pl.plain64 = plain ? 1 : 0;
+  const size_t lds_bytes = (plain ? 0 : (size_t)nt * 2) + (size_t)rows * (1 << pl.cshift) * per;
+  int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / (lds_bytes + 1024)));
+  pl.nblocks = std::max(1, c->ncu * per_cu);
+  if (nitems > 0) {
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(c, launch_pairs(pl, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+  }
+  std::vector<unsigned long long> hc(nflat + 8);
+  std::vector<double> hw(nflat);
+  HIPCHK(c, hipMemcpyAsync(hc.data(), d_hist, hist_bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(hw.data(), d_w, w_bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(hc.data() + nflat, d_stats, 4 * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const uint64_t mult = auto_ ? 2 : 1;
+  for (int b = 0; b < nflat; ++b) {
+    if (counts) counts[b] = (uint64_t)hc[b] * mult;
+    if (weights) weights[b] = weighted ? hw[b] * (double)mult : (double)((uint64_t)hc[b] * mult);
+  }
+  pcf_stats st{};
+  st.evaluations = hc[nflat];
+  st.rechecks = hc[nflat + 1];
+  st.work_items = (uint64_t)nitems;
+  float ms = 0.f;
+  if (nitems > 0) {
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));
+    st.kernel_ms = ms;
+  }
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[3]));
+  st.total_ms = ms;
+  st.cells_per_rmax = m;
+  c->last = st;
+  return PCF_OK;
+}
+
+__global__ void k_analytic_rr(double alpha, int bins, double* rr) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < bins) {
+    long long k = 3LL * i * i + 3LL * i + 1;  // 64-bit: no overflow for large bin counts
+    rr[i] = alpha * (double)k;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int64_t pcf_hist_size(int32_t type, int32_t bins) {
+  int64_t b = bins;
+  switch (type) {
+    case PCF_2ISO: return b;
+    case PCF_2ANI: return b * b;
+    case PCF_3ISO: return b * b * b;
+    case PCF_3ANI: return b * b * b * b * b;
+  }
+  return -1;
+}
+
+const char* pcf_create_error(void) { return g_create_error.c_str(); }
+
+int pcf_create(pcf_ctx** out, int32_t device) {
+  if (!out) return PCF_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0) {
+    g_create_error = std::string("no HIP device: ") + hipGetErrorString(e);
+    (void)hipGetLastError();
+    return PCF_ERR_HIP;
+  }
+  if (device < 0 || device >= ndev) {
+    g_create_error = "device index out of range";
+    return PCF_ERR_ARG;
+  }
+  auto c = new pcf_ctx();
+  c->device = device;
+  if ((e = hipSetDevice(device)) != hipSuccess) goto fail;
+  hipDeviceProp_t prop;
+  if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) goto fail;
+  c->ncu = prop.multiProcessorCount;
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) goto fail;
+  for (auto& ev : c->ev)
+    if ((e = hipEventCreate(&ev)) != hipSuccess) goto fail;
+  *out = c;
+  return PCF_OK;
+fail:
+  g_create_error = std::string("HIP init: ") + hipGetErrorString(e);
+  (void)hipGetLastError();
+  delete c;
+  return PCF_ERR_HIP;
+}
+
+void pcf_destroy(pcf_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  destroy_engine(c->engine);
+  c->engine = nullptr;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& cat : c->cats)
+    if (cat) free_catalog(cat.get());
+  for (DevBuf* b : {&c->scratch_cells, &c->scratch_perm, &c->wl_counts, &c->wl_items,
+                    &c->table_buf, &c->rules_buf, &c->hist_buf, &c->misc_buf, &c->stats_partials})
+    b->release();
+  for (auto& ev : c->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* pcf_last_error(const pcf_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+static int upload_common(pcf_ctx* c, const void* src, int64_t n, hipMemcpyKind kind,
+                         pcf_catalog* id) {
+  if (!c || !id || n < 0 || (n > 0 && !src)) return fail(c, PCF_ERR_ARG, "bad upload arguments");
+  if (n > (int64_t)INT32_MAX - 64) return fail(c, PCF_ERR_ARG, "catalog too large");
+  HIPCHK(c, hipSetDevice(c->device));
+  auto cat = std::make_unique<Catalog>();
+  cat->n = n;
+  HIPCHK(c, hipMalloc(&cat->raw, sizeof(float4) * (n > 0 ? n : 1)));
+  if (n > 0) {
+    hipError_t e = hipMemcpyAsync(cat->raw, src, sizeof(float4) * n, kind, c->stream);
+    if (e != hipSuccess) {
+      (void)hipFree(cat->raw);
+      return hipfail(c, e, "catalog copy");
+    }
+  }
+  int rc = finish_upload(c, cat.get());
+  if (rc != PCF_OK) {
+    free_catalog(cat.get());
+    return rc;
+  }
+  for (size_t i = 0; i < c->cats.size(); ++i)
+    if (!c->cats[i]) {
+      c->cats[i] = std::move(cat);
+      *id = (pcf_catalog)i;
+      return PCF_OK;
+    }
+  c->cats.push_back(std::move(cat));
+  *id = (pcf_catalog)(c->cats.size() - 1);
+  return PCF_OK;
+}
+
+int pcf_catalog_upload(pcf_ctx* c, const float* xyzw, int64_t n, pcf_catalog* id) {
+  return upload_common(c, xyzw, n, hipMemcpyHostToDevice, id);
+}
+
+int pcf_catalog_upload_device(pcf_ctx* c, const void* d_xyzw, int64_t n, pcf_catalog* id) {
+  return upload_common(c, d_xyzw, n, hipMemcpyDeviceToDevice, id);
+}
+
+int pcf_catalog_release(pcf_ctx* c, pcf_catalog id) {
+  if (!c) return PCF_ERR_ARG;
+  Catalog* cat = get_cat(c, id);
+  if (!cat) return fail(c, PCF_ERR_ARG, "unknown catalog id");
+  (void)hipStreamSynchronize(c->stream);
+  if (c->engine) engine_drain(c->engine);
+  free_catalog(cat);
+  c->cats[id].reset();
+  return PCF_OK;
+}
+
+int64_t pcf_catalog_size(pcf_ctx* c, pcf_catalog id) {
+  if (!c) return -1;
+  Catalog* cat = get_cat(c, id);
+  return cat ? cat->n : -1;
+}
+
+int pcf_histogram(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights) {
+  if (!c || !job) return fail(c, PCF_ERR_ARG, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  c->err.clear();
+  switch (job->type) {
+    case PCF_2ISO:
+    case PCF_2ANI:
+      if (job->hist != PCF_XX && job->hist != PCF_XY)
+        return fail(c, PCF_ERR_ARG, "pair types take PCF_XX or PCF_XY");
+      return run_pairs(c, job, counts, weights);
+    case PCF_3ISO:
+    case PCF_3ANI:
+      if (job->hist != PCF_XXX && job->hist != PCF_XXY)
+        return fail(c, PCF_ERR_ARG, "triplet types take PCF_XXX or PCF_XXY");
+      return run_triplets(c, job, counts, weights);
+  }
+  return fail(c, PCF_ERR_ARG, "unknown histogram type");
+}
+
+int pcf_last_stats(const pcf_ctx* c, pcf_stats* out) {
+  if (!c || !out) return PCF_ERR_ARG;
+  *out = c->last;
+  return PCF_OK;
+}
+
+int pcf_analytic_rr(pcf_ctx* c, int32_t bins, int64_t np, float dmax, float box, double* rr) {
+  if (!c || !rr || bins <= 0 || np < 0 || !(dmax > 0) || !(box > 0))
+    return fail(c, PCF_ERR_ARG, "bad analytic RR arguments");
+  HIPCHK(c, hipSetDevice(c->device));
+  // Host constants of cuda/pcf2isoBPC.cuh:199-202 with FP32 dr and beta as the
+  // reference forms them, np*np in 64-bit.
+  float drf = dmax / (float)bins;
+  double dr = drf;
+  float np2 = (float)((double)np * (double)np);
+  float L3 = box * box * box;
+  float betaf = np2 / L3;
+  double beta = betaf;
+  double alpha = 8 * dr * dr * dr * std::acos(0.0) * beta / 3;
+  HIPCHK(c, c->misc_buf.ensure(sizeof(double) * bins));
+  hipLaunchKernelGGL(k_analytic_rr, dim3((bins + 255) / 256), dim3(256), 0, c->stream, alpha,
+                     (int)bins, c->misc_buf.as<double>());
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(rr, c->misc_buf.p, sizeof(double) * bins, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return PCF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,210 @@
+// grid.hip -- on-device cell grid ("create_grid" of the reference).
+//
+// The reference classifies points into nodes on the host, one point at a time,
+// re-allocating the node's array on every insert (cuda/create_grid.cuh:81-138),
+// then linearises the non-empty nodes and copies them to the device
+// (cuda/main.cu:215-329).  Here the grid is a counting sort on the device:
+//   1. k_cell_index  : cell of every point + per-cell counts (global atomics)
+//   2. k_scan_exclusive: cell_start[] = exclusive prefix sum of the counts
+//   3. k_scatter     : points copied to their cell's range, SoA-friendly float4
+// The resulting layout is cell-ordered (x-major, z fastest) so that every z-range
+// of a (cx,cy) column is one contiguous, coalesced point range.
+#include "pcf_internal.h"
+
+namespace pcf {
+
+namespace {
+
+constexpr int kStatsBlock = 256;
+
+// Per-block reduction of the catalog: min/max per axis, max |coordinate|,
+// whether every weight is exactly 1 and whether every value is finite.
+// partials[b*10 + 0..2] = min, [3..5] = max, [6] = max|c|, [7] = all_w1,
+// [8] = finite, [9] unused.
+__global__ __launch_bounds__(kStatsBlock) void k_catalog_stats(const float4* __restrict__ raw,
+                                                               int64_t n,
+                                                               float* __restrict__ partials) {
+  __shared__ float red[10][kStatsBlock];
+  float mn[3] = {INFINITY, INFINITY, INFINITY};
+  float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  float ma = 0.f, w1 = 1.f, fin = 1.f;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    float4 p = raw[i];
+    float v[3] = {p.x, p.y, p.z};
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = fminf(mn[a], v[a]);
+      mx[a] = fmaxf(mx[a], v[a]);
+      ma = fmaxf(ma, fabsf(v[a]));
+    }
+    if (p.w != 1.0f) w1 = 0.f;
+    if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(p.w))) fin = 0.f;
+  }
+  const int t = threadIdx.x;
+  for (int a = 0; a < 3; ++a) {
+    red[a][t] = mn[a];
+    red[3 + a][t] = mx[a];
+  }
+  red[6][t] = ma;
+  red[7][t] = w1;
+  red[8][t] = fin;
+  __syncthreads();
+  for (int s = kStatsBlock / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      for (int a = 0; a < 3; ++a) {
+        red[a][t] = fminf(red[a][t], red[a][t + s]);
+        red[3 + a][t] = fmaxf(red[3 + a][t], red[3 + a][t + s]);
+      }
+      red[6][t] = fmaxf(red[6][t], red[6][t + s]);
+      red[7][t] = fminf(red[7][t], red[7][t + s]);
+      red[8][t] = fminf(red[8][t], red[8][t + s]);
+    }
+    __syncthreads();
+  }
+  if (t < 9) partials[(int64_t)blockIdx.x * 10 + t] = red[t][0];
+}
+
+__device__ __forceinline__ int cell_axis(float v, float lo, float inv_cs, int P) {
+  int c = (int)floorf((v - lo) * inv_cs);
+  c = c < 0 ? 0 : c;
+  return c >= P ? P - 1 : c;
+}
+
+__global__ void k_cell_index(const float4* __restrict__ raw, int64_t n, GridGeom g,
+                             int* __restrict__ cell_of, int* __restrict__ counts) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 p = raw[i];
+  int cx = cell_axis(p.x, g.lo[0], g.inv_cs[0], g.P[0]);
+  int cy = cell_axis(p.y, g.lo[1], g.inv_cs[1], g.P[1]);
+  int cz = cell_axis(p.z, g.lo[2], g.inv_cs[2], g.P[2]);
+  int c = (cx * g.P[1] + cy) * g.P[2] + cz;
+  cell_of[i] = c;
+  atomicAdd(&counts[c], 1);
+}
+
+// Exclusive scan of counts[0..m) into out[0..m] (out[m] = total), one
+// workgroup: each thread reduces a contiguous chunk, the chunk sums are scanned
+// in LDS, then each thread writes its chunk's prefix.  out may alias counts.
+constexpr int kScanBlock = 1024;
+__global__ __launch_bounds__(kScanBlock) void k_scan_exclusive(const int* counts, int64_t m,
+                                                              int* out) {
+  __shared__ long long part[kScanBlock];
+  const int t = threadIdx.x;
+  const int64_t chunk = (m + kScanBlock - 1) / kScanBlock;
+  const int64_t b = t * chunk;
+  const int64_t e = b + chunk < m ? b + chunk : m;
+  long long s = 0;
+  for (int64_t i = b; i < e; ++i) s += counts[i];
+  part[t] = s;
+  __syncthreads();
+  // Hillis-Steele inclusive scan of the chunk sums.
+  for (int off = 1; off < kScanBlock; off <<= 1) {
+    long long v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  long long run = part[t] - s;  // exclusive prefix of this chunk
+  for (int64_t i = b; i < e; ++i) {
+    int c = counts[i];
+    out[i] = (int)run;
+    run += c;
+  }
+  if (t == kScanBlock - 1) out[m] = (int)part[t];
+}
+
+__global__ void k_scatter(const float4* __restrict__ raw, int64_t n, const int* __restrict__ cell_of,
+                          const int* __restrict__ cell_start, int* __restrict__ cursor,
+                          float4* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c = cell_of[i];
+  int pos = cell_start[c] + atomicAdd(&cursor[c], 1);
+  out[pos] = raw[i];
+}
+
+__global__ void k_to_double(const float4* __restrict__ in, int64_t n, double4* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 p = in[i];
+  out[i] = make_double4(p.x, p.y, p.z, p.w);
+}
+
+__global__ void k_zero_words(uint32_t* p, int64_t nwords) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nwords) p[i] = 0u;
+}
+
+}  // namespace
+
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
+  int64_t words = (int64_t)((bytes + 3) / 4);
+  if (words == 0) return hipSuccess;
+  int blocks = (int)((words + 255) / 256);
+  hipLaunchKernelGGL(k_zero_words, dim3(blocks), dim3(256), 0, s, (uint32_t*)p, words);
+  return hipGetLastError();
+}
+
+hipError_t launch_catalog_stats(const float4* raw, int64_t n, float* partials, int nblocks,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(k_catalog_stats, dim3(nblocks), dim3(kStatsBlock), 0, s, raw, n, partials);
+  return hipGetLastError();
+}
+
+hipError_t build_grid(const float4* raw, int64_t n, const GridGeom& g, DevGrid* out,
+                      DevBuf& scratch_cells, DevBuf& scratch_perm, hipStream_t s) {
+  hipError_t e;
+  const int64_t nc = g.ncell();
+  if (!out->pts || out->n != n) {
+    if (out->pts) (void)hipFree(out->pts);
+    out->pts = nullptr;
+    if ((e = hipMalloc(&out->pts, sizeof(float4) * (n > 0 ? n : 1))) != hipSuccess) return e;
+  }
+  if (!out->cell_start || !out->g.same(g)) {
+    if (out->cell_start) (void)hipFree(out->cell_start);
+    out->cell_start = nullptr;
+    if ((e = hipMalloc(&out->cell_start, sizeof(int) * (nc + 1))) != hipSuccess) return e;
+  }
+  if (out->pts64) {
+    (void)hipFree(out->pts64);
+    out->pts64 = nullptr;
+  }
+  out->g = g;
+  out->n = n;
+  if ((e = scratch_cells.ensure(sizeof(int) * (nc + 1))) != hipSuccess) return e;
+  if ((e = scratch_perm.ensure(sizeof(int) * (n > 0 ? n : 1))) != hipSuccess) return e;
+  int* counts = scratch_cells.as<int>();
+  int* cell_of = scratch_perm.as<int>();
+  if ((e = launch_zero(counts, sizeof(int) * (nc + 1), s)) != hipSuccess) return e;
+  const int blocks = (int)((n + 255) / 256);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_cell_index, dim3(blocks), dim3(256), 0, s, raw, n, g, cell_of, counts);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(kScanBlock), 0, s, counts, nc,
+                     out->cell_start);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_zero(counts, sizeof(int) * nc, s)) != hipSuccess) return e;
+  if (n > 0) {
+    hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, raw, n, cell_of,
+                       out->cell_start, counts, out->pts);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  out->valid = true;
+  return hipSuccess;
+}
+
+hipError_t build_grid64(DevGrid* grid, hipStream_t s) {
+  if (grid->pts64) return hipSuccess;
+  hipError_t e = hipMalloc(&grid->pts64, sizeof(double4) * (grid->n > 0 ? grid->n : 1));
+  if (e != hipSuccess) return e;
+  if (grid->n > 0) {
+    hipLaunchKernelGGL(k_to_double, dim3((unsigned)((grid->n + 255) / 256)), dim3(256), 0, s,
+                       grid->pts, grid->n, grid->pts64);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
+}  // namespace pcf

@@ -1,0 +1,519 @@
+// pairs.hip -- flattened cell-pair work list and the gfx950 pair-histogram
+// kernels (2iso, 2ani; periodic or not).
+//
+// Reference behaviour restated (cuda/kernels/2iso_k.cuh:8-120,
+// 2isoBPC_k.cuh:8-557, 2ani_k.cuh:7-135, 2aniBPC_k.cuh:8-682): every ordered
+// pair of points whose separation passes the range test adds w_i*w_j to bin
+// (int)(sqrt(d2)*ds).  The reference runs one thread per ordered node pair with
+// a global FP64 atomicAdd per counted pair and evaluates periodic images with
+// seven branchy copies of the point loop.  Here:
+//
+//   * the work list holds (i-tile of <= 64 points, contiguous j-range of one
+//     neighbour column segment, periodic image) items, built on the device;
+//     periodic boundaries are one image offset per item, not per point;
+//   * auto histograms (DD, RR) visit each unordered pair once (j > i in cell
+//     order) and double the result at the end;
+//   * each lane holds one i point in VGPRs; the j points stream through scalar
+//     loads (uniform across the wave), so no LDS is spent on point staging;
+//   * the squared distance is formed in FP32 (3 sub + mul + 2 fma) and binned
+//     through a d2 -> bin table in LDS built on the host from exact FP64 bin
+//     thresholds; table cells that a bin edge or the range limit could cross
+//     within the FP32 error bound are marked ambiguous, and the lanes that hit
+//     them re-evaluate their pair in FP64 exactly as the oracle does;
+//   * counts go to a per-lane-column LDS histogram (no bank conflicts among the
+//     lanes of a wave), merged into the global uint64 / FP64 histogram once per
+//     workgroup of a persistent grid that pulls items from an atomic counter.
+#include "pcf_internal.h"
+
+namespace pcf {
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Work list
+// ---------------------------------------------------------------------------
+template <bool FILL>
+__device__ int enumerate_cell(const WorkListArgs& a, int cell, PairItem* out) {
+  const int ib = a.x_cell_start[cell];
+  const int ie = a.x_cell_start[cell + 1];
+  if (ie <= ib) return 0;
+  const GridGeom& g = a.g;
+  const int Pz = g.P[2], Py = g.P[1], Px = g.P[0];
+  const int cz = cell % Pz;
+  const int cy = (cell / Pz) % Py;
+  const int cx = cell / (Pz * Py);
+  const float reach2 = a.reach * a.reach;
+  int count = 0;
+  for (int i0 = ib; i0 < ie; i0 += kTile) {
+    const int ni = min(kTile, ie - i0);
+    for (int dx = -a.K[0]; dx <= a.K[0]; ++dx) {
+      int ccx = cx + dx, kx = 0;
+      if (ccx < 0) {
+        if (!g.bpc) continue;
+        ccx += Px;
+        kx = -1;
+      } else if (ccx >= Px) {
+        if (!g.bpc) continue;
+        ccx -= Px;
+        kx = 1;
+      }
+      const float gx = (float)max(0, abs(dx) - 1) * g.cs[0];
+      for (int dy = -a.K[1]; dy <= a.K[1]; ++dy) {
+        int ccy = cy + dy, ky = 0;
+        if (ccy < 0) {
+          if (!g.bpc) continue;
+          ccy += Py;
+          ky = -1;
+        } else if (ccy >= Py) {
+          if (!g.bpc) continue;
+          ccy -= Py;
+          ky = 1;
+        }
+        const float gy = (float)max(0, abs(dy) - 1) * g.cs[1];
+        const float rho2 = gx * gx + gy * gy;
+        if (rho2 >= reach2) continue;
+        const float hz = sqrtf(reach2 - rho2);
+        const int Kz = min(a.K[2], 1 + (int)floorf(hz * g.inv_cs[2]));
+        const int col = (ccx * Py + ccy) * Pz;
+        int zlo = cz - Kz, zhi = cz + Kz;
+        // Up to three z segments: wrapped below (kz=-1), inside, wrapped above.
+        int seg_lo[3], seg_hi[3], seg_k[3], nseg = 0;
+        if (g.bpc) {
+          if (zlo < 0) { seg_lo[nseg] = zlo + Pz; seg_hi[nseg] = Pz - 1; seg_k[nseg++] = -1; }
+          seg_lo[nseg] = max(zlo, 0); seg_hi[nseg] = min(zhi, Pz - 1); seg_k[nseg++] = 0;
+          if (zhi > Pz - 1) { seg_lo[nseg] = 0; seg_hi[nseg] = zhi - Pz; seg_k[nseg++] = 1; }
+        } else {
+          seg_lo[0] = max(zlo, 0); seg_hi[0] = min(zhi, Pz - 1); seg_k[0] = 0; nseg = 1;
+        }
+        for (int s = 0; s < nseg; ++s) {
+          if (seg_hi[s] < seg_lo[s]) continue;
+          int j0 = a.y_cell_start[col + seg_lo[s]];
+          int j1 = a.y_cell_start[col + seg_hi[s] + 1];
+          if (j1 <= j0) continue;
+          int flag = 0;
+          if (a.auto_) {
+            // Count each unordered pair once: j > i in cell order.
+            if (j1 <= i0 + 1) continue;
+            if (j0 < i0 + ni) {
+              j0 = max(j0, i0 + 1);
+              flag = kCodeMask;
+            }
+          }
+          if (FILL) {
+            PairItem it;
+            it.i0 = i0;
+            it.ni = ni;
+            it.j0 = j0;
+            it.nj = j1 - j0;
+            it.code = ((kx + 1) * 9 + (ky + 1) * 3 + (seg_k[s] + 1)) | flag;
+            out[count] = it;
+          }
+          ++count;
+        }
+      }
+    }
+  }
+  return count;
+}
+
+__global__ void k_worklist_count(WorkListArgs a, int* __restrict__ counts) {
+  const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= a.g.ncell()) return;
+  counts[cell] = enumerate_cell<false>(a, (int)cell, nullptr);
+}
+
+__global__ void k_worklist_fill(WorkListArgs a, const int* __restrict__ offsets,
+                                PairItem* __restrict__ items) {
+  const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= a.g.ncell()) return;
+  enumerate_cell<true>(a, (int)cell, items + offsets[cell]);
+}
+
+// Same single-workgroup exclusive scan as the grid build (kept local so the
+// translation units stay independent).
+constexpr int kScanBlock = 1024;
+__global__ __launch_bounds__(kScanBlock) void k_scan_items(const int* counts, int64_t m, int* out) {
+  __shared__ long long part[kScanBlock];
+  const int t = threadIdx.x;
+  const int64_t chunk = (m + kScanBlock - 1) / kScanBlock;
+  const int64_t b = t * chunk;
+  const int64_t e = b + chunk < m ? b + chunk : m;
+  long long s = 0;
+  for (int64_t i = b; i < e; ++i) s += counts[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < kScanBlock; off <<= 1) {
+    long long v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  long long run = part[t] - s;
+  for (int64_t i = b; i < e; ++i) {
+    int c = counts[i];
+    out[i] = (int)run;
+    run += c;
+  }
+  if (t == kScanBlock - 1) out[m] = (int)part[t];
+}
+
+// ---------------------------------------------------------------------------
+// Exact FP64 re-evaluation (the oracle's arithmetic, no contraction).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool axis_sq(float ai, float aj, int k, double L, double* out) {
+  double d = __dsub_rn((double)aj, (double)ai);
+  if (k == 0) {
+    *out = __dmul_rn(d, d);
+    return true;
+  }
+  // Only the image that brings the points closer is a reference term
+  // (size_box - |delta|, cuda/kernels/2isoBPC_k.cuh:87).
+  if ((k > 0 && d >= 0.0) || (k < 0 && d < 0.0)) return false;
+  double t = __dsub_rn(L, fabs(d));
+  *out = __dmul_rn(t, t);
+  return true;
+}
+
+__device__ __forceinline__ int class_of(double d2, const BinRules* __restrict__ r) {
+  if (r->le ? !(d2 <= r->R) : !(d2 < r->R)) return -1;
+  const int bins = r->bins;
+  if (d2 >= r->T[bins]) return -1;
+  int lo = 0, hi = bins - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (r->T[mid] <= d2) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __noinline__ int recheck_pair(float4 pi, float4 pj, int img,
+                                         const BinRules* __restrict__ r) {
+  const int kx = img / 9 - 1, ky = (img / 3) % 3 - 1, kz = img % 3 - 1;
+  double tx, ty, tz;
+  if (!axis_sq(pi.x, pj.x, kx, r->L, &tx)) return -1;
+  if (!axis_sq(pi.y, pj.y, ky, r->L, &ty)) return -1;
+  if (!axis_sq(pi.z, pj.z, kz, r->L, &tz)) return -1;
+  if (!r->ani) {
+    double d2 = __dadd_rn(__dadd_rn(tx, ty), tz);
+    if (r->zero == 1 && img == kImgZero && !(d2 > 0.0)) return -1;
+    return class_of(d2, r);
+  }
+  double do2 = __dadd_rn(tx, ty);
+  if (r->zero == 2 && !(tz > 0.0 && do2 > 0.0)) return -1;
+  int bz = class_of(tz, r);
+  if (bz < 0) return -1;
+  int bo = class_of(do2, r);
+  if (bo < 0) return -1;
+  return bz * r->bins + bo;
+}
+
+// ---------------------------------------------------------------------------
+// Counting kernels
+// ---------------------------------------------------------------------------
+struct KArgs {
+  int nitems;
+  int nt;
+  float inv_h;
+  float L;
+  int bins;
+  int nflat;
+  int cshift;
+};
+
+__device__ __forceinline__ int wave_fetch(int* counter, int lane) {
+  int it = 0;
+  if (lane == 0) it = atomicAdd(counter, 1);
+  return __builtin_amdgcn_readfirstlane(it);
+}
+
+// FP32 bulk + FP64 edge re-evaluation.
+template <int ANI, int WEIGHTED>
+__global__ __launch_bounds__(kBlock) void k_pairs_mixed(
+    const float4* __restrict__ xp, const float4* __restrict__ yp,
+    const PairItem* __restrict__ items, int* __restrict__ counter,
+    const uint16_t* __restrict__ table, const BinRules* __restrict__ rules,
+    unsigned long long* __restrict__ hist, double* __restrict__ whist,
+    unsigned long long* __restrict__ stats, KArgs k) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tab_bytes = (k.nt * 2 + 15) & ~15;
+  const int nrows = k.nflat + 2;
+  const int ncopy = 1 << k.cshift;
+  uint16_t* tab = reinterpret_cast<uint16_t*>(smem);
+  uint32_t* hc = reinterpret_cast<uint32_t*>(smem + tab_bytes);
+  double* hw = reinterpret_cast<double*>(smem + tab_bytes + ((nrows * ncopy * 4 + 15) & ~15));
+  for (int t = threadIdx.x; t < k.nt; t += kBlock) tab[t] = table[t];
+  for (int t = threadIdx.x; t < nrows * ncopy; t += kBlock) {
+    hc[t] = 0u;
+    if (WEIGHTED) hw[t] = 0.0;
+  }
+  __syncthreads();
+
+  const int lane = threadIdx.x & (kWave - 1);
+  const int copy = lane & (ncopy - 1);
+  const unsigned ntm1 = (unsigned)(k.nt - 1);
+  const unsigned out_row = (unsigned)k.nflat;
+  const unsigned amb_row = (unsigned)k.nflat + 1;
+  const float inv_h = k.inv_h;
+  unsigned long long evals = 0;
+  unsigned long long nre = 0;
+
+  for (;;) {
+    const int it = wave_fetch(counter, lane);
+    if (it >= k.nitems) break;
+    const int i0 = __builtin_amdgcn_readfirstlane(items[it].i0);
+    const int ni = __builtin_amdgcn_readfirstlane(items[it].ni);
+    const int j0 = __builtin_amdgcn_readfirstlane(items[it].j0);
+    const int nj = __builtin_amdgcn_readfirstlane(items[it].nj);
+    const int code = __builtin_amdgcn_readfirstlane(items[it].code);
+    const int img = code & 31;
+    const bool masked = (code & kCodeMask) != 0;
+    evals += (unsigned long long)ni * (unsigned long long)nj;
+
+    float4 pi;
+    if (lane < ni) pi = xp[i0 + lane];
+    else pi = make_float4(1e18f, 1e18f, 1e18f, 0.f);
+    // Image offset applied once to the i point: xi' = xi - k*L.
+    const float sx = (float)(img / 9 - 1) * k.L;
+    const float sy = (float)((img / 3) % 3 - 1) * k.L;
+    const float sz = (float)(img % 3 - 1) * k.L;
+    const float xs = pi.x - sx, ys = pi.y - sy, zs = pi.z - sz;
+    const int my_i = i0 + lane;
+    const double wi = (double)pi.w;
+
+    for (int j = j0; j < j0 + nj; ++j) {
+      const float4 q = yp[j];
+      const float dx = q.x - xs;
+      const float dy = q.y - ys;
+      const float dz = q.z - zs;
+      unsigned row;
+      if (!ANI) {
+        const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+        const unsigned t = min(__float2uint_rz(d2 * inv_h), ntm1);
+        row = tab[t];
+      } else {
+        const float dz2 = dz * dz;
+        const float do2 = __builtin_fmaf(dy, dy, dx * dx);
+        const unsigned tz = min(__float2uint_rz(dz2 * inv_h), ntm1);
+        const unsigned to = min(__float2uint_rz(do2 * inv_h), ntm1);
+        const unsigned cz = tab[tz];
+        const unsigned co = tab[to];
+        const unsigned m = max(cz, co);
+        row = (m < (unsigned)k.bins) ? cz * (unsigned)k.bins + co
+                                     : (m == kAniAmb ? amb_row : out_row);
+      }
+      if (masked && j <= my_i) row = out_row;
+      const bool amb = row == amb_row;
+      if (__builtin_expect(__ballot(amb) != 0, 0)) {
+        if (amb) {
+          const int b = recheck_pair(pi, q, img, rules);
+          ++nre;
+          if (b >= 0) {
+            atomicAdd(&hc[((unsigned)b << k.cshift) + copy], 1u);
+            if (WEIGHTED) atomicAdd(&hw[((unsigned)b << k.cshift) + copy], wi * (double)q.w);
+          }
+        }
+      }
+      atomicAdd(&hc[(row << k.cshift) + copy], 1u);
+      if (WEIGHTED) atomicAdd(&hw[(row << k.cshift) + copy], wi * (double)q.w);
+    }
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < k.nflat; r += kBlock) {
+    unsigned long long c = 0;
+    double w = 0.0;
+    for (int cpy = 0; cpy < ncopy; ++cpy) {
+      c += hc[(r << k.cshift) + cpy];
+      if (WEIGHTED) w += hw[(r << k.cshift) + cpy];
+    }
+    if (c) {
+      atomicAdd(&hist[r], c);
+      if (WEIGHTED) atomicAdd(&whist[r], w);
+    }
+  }
+  if (lane == 0) atomicAdd(&stats[0], evals);
+  if (nre) atomicAdd(&stats[1], nre);
+}
+
+// Plain FP64 path: every pair formed in FP64, the range test and
+// bin = (int)(sqrt(d2)*ds) exactly as the reference kernels write them.
+__device__ __forceinline__ bool axis_sq64(double ai, double aj, int k, double L, double* out) {
+  double d = __dsub_rn(aj, ai);
+  if (k == 0) {
+    *out = __dmul_rn(d, d);
+    return true;
+  }
+  if ((k > 0 && d >= 0.0) || (k < 0 && d < 0.0)) return false;
+  double t = __dsub_rn(L, fabs(d));
+  *out = __dmul_rn(t, t);
+  return true;
+}
+
+template <int ANI, int WEIGHTED>
+__global__ __launch_bounds__(kBlock) void k_pairs_fp64(
+    const double4* __restrict__ xp, const double4* __restrict__ yp,
+    const PairItem* __restrict__ items, int* __restrict__ counter,
+    const BinRules* __restrict__ rules, unsigned long long* __restrict__ hist,
+    double* __restrict__ whist, unsigned long long* __restrict__ stats, KArgs k) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int nrows = k.nflat + 1;
+  const int ncopy = 1 << k.cshift;
+  uint32_t* hc = reinterpret_cast<uint32_t*>(smem);
+  double* hw = reinterpret_cast<double*>(smem + ((nrows * ncopy * 4 + 15) & ~15));
+  for (int t = threadIdx.x; t < nrows * ncopy; t += kBlock) {
+    hc[t] = 0u;
+    if (WEIGHTED) hw[t] = 0.0;
+  }
+  __syncthreads();
+  const double R = rules->R;
+  const double ds = rules->ds;
+  const double L = rules->L;
+  const int le = rules->le;
+  const int zero = rules->zero;
+  const int bins = k.bins;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int copy = lane & (ncopy - 1);
+  unsigned long long evals = 0;
+  for (;;) {
+    const int it = wave_fetch(counter, lane);
+    if (it >= k.nitems) break;
+    const int i0 = __builtin_amdgcn_readfirstlane(items[it].i0);
+    const int ni = __builtin_amdgcn_readfirstlane(items[it].ni);
+    const int j0 = __builtin_amdgcn_readfirstlane(items[it].j0);
+    const int nj = __builtin_amdgcn_readfirstlane(items[it].nj);
+    const int code = __builtin_amdgcn_readfirstlane(items[it].code);
+    const int img = code & 31;
+    const int kx = img / 9 - 1, ky = (img / 3) % 3 - 1, kz = img % 3 - 1;
+    const bool masked = (code & kCodeMask) != 0;
+    evals += (unsigned long long)ni * (unsigned long long)nj;
+    const bool active = lane < ni;
+    double4 pi = active ? xp[i0 + lane] : make_double4(0, 0, 0, 0);
+    const int my_i = i0 + lane;
+    for (int j = j0; j < j0 + nj; ++j) {
+      const double4 q = yp[j];
+      double tx, ty, tz;
+      bool ok = active && !(masked && j <= my_i);
+      ok = ok && axis_sq64(pi.x, q.x, kx, L, &tx);
+      ok = ok && axis_sq64(pi.y, q.y, ky, L, &ty);
+      ok = ok && axis_sq64(pi.z, q.z, kz, L, &tz);
+      int row = -1;
+      if (ok) {
+        if (!ANI) {
+          const double d2 = __dadd_rn(__dadd_rn(tx, ty), tz);
+          const bool in = le ? (d2 <= R) : (d2 < R);
+          if (in && !(zero == 1 && img == kImgZero && !(d2 > 0.0))) {
+            const int b = (int)(sqrt(d2) * ds);
+            if (b <= bins - 1) row = b;
+          }
+        } else {
+          const double do2 = __dadd_rn(tx, ty);
+          const bool in = le ? (tz <= R && do2 <= R) : (tz < R && do2 < R);
+          if (in && !(zero == 2 && !(tz > 0.0 && do2 > 0.0))) {
+            const int bz = (int)(sqrt(tz) * ds);
+            const int bo = (int)(sqrt(do2) * ds);
+            if (bz <= bins - 1 && bo <= bins - 1) row = bz * bins + bo;
+          }
+        }
+      }
+      const unsigned r = row >= 0 ? (unsigned)row : (unsigned)k.nflat;
+      atomicAdd(&hc[(r << k.cshift) + copy], 1u);
+      if (WEIGHTED) atomicAdd(&hw[(r << k.cshift) + copy], pi.w * q.w);
+    }
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < k.nflat; r += kBlock) {
+    unsigned long long c = 0;
+    double w = 0.0;
+    for (int cpy = 0; cpy < ncopy; ++cpy) {
+      c += hc[(r << k.cshift) + cpy];
+      if (WEIGHTED) w += hw[(r << k.cshift) + cpy];
+    }
+    if (c) {
+      atomicAdd(&hist[r], c);
+      if (WEIGHTED) atomicAdd(&whist[r], w);
+    }
+  }
+  if (lane == 0) atomicAdd(&stats[0], evals);
+}
+
+template <class K>
+hipError_t set_lds(K kern, size_t bytes) {
+  if (bytes <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+}  // namespace
+
+hipError_t build_pair_worklist(const WorkListArgs& a, DevBuf& counts, DevBuf& items, int* nitems,
+                               hipStream_t s) {
+  hipError_t e;
+  const int64_t nc = a.g.ncell();
+  if ((e = counts.ensure(sizeof(int) * (2 * nc + 2))) != hipSuccess) return e;
+  int* cnt = counts.as<int>();
+  int* off = cnt + nc + 1;
+  const int blocks = (int)((nc + 127) / 128);
+  hipLaunchKernelGGL(k_worklist_count, dim3(blocks), dim3(128), 0, s, a, cnt);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_scan_items, dim3(1), dim3(kScanBlock), 0, s, cnt, nc, off);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  int total = 0;
+  if ((e = hipMemcpyAsync(&total, off + nc, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess)
+    return e;
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  *nitems = total;
+  if (total == 0) return hipSuccess;
+  if ((e = items.ensure(sizeof(PairItem) * (size_t)total)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_worklist_fill, dim3(blocks), dim3(128), 0, s, a, off,
+                     items.as<PairItem>());
+  return hipGetLastError();
+}
+
+hipError_t launch_pairs(const PairLaunch& pl, hipStream_t s) {
+  KArgs k;
+  k.nitems = pl.nitems;
+  k.nt = pl.nt;
+  k.inv_h = pl.inv_h;
+  k.L = pl.L;
+  k.bins = pl.bins;
+  k.nflat = pl.nflat;
+  k.cshift = pl.cshift;
+  const int ncopy = 1 << pl.cshift;
+  dim3 grid(pl.nblocks), block(kBlock);
+  hipError_t e;
+  if (!pl.plain64) {
+    const size_t tab_bytes = (size_t)((pl.nt * 2 + 15) & ~15);
+    const size_t hc_bytes = (size_t)(((pl.nflat + 2) * ncopy * 4 + 15) & ~15);
+    const size_t hw_bytes = pl.weighted ? (size_t)(pl.nflat + 2) * ncopy * 8 : 0;
+    const size_t lds = tab_bytes + hc_bytes + hw_bytes;
+#define PCF_LAUNCH_MIXED(A, W)                                                                 \
+  do {                                                                                         \
+    if ((e = set_lds(k_pairs_mixed<A, W>, lds)) != hipSuccess) return e;                       \
+    hipLaunchKernelGGL((k_pairs_mixed<A, W>), grid, block, lds, s, pl.xp, pl.yp, pl.items,     \
+                       pl.counter, pl.table, pl.rules, pl.hist, pl.whist, pl.stats, k);        \
+  } while (0)
+    if (!pl.ani && !pl.weighted) PCF_LAUNCH_MIXED(0, 0);
+    else if (!pl.ani && pl.weighted) PCF_LAUNCH_MIXED(0, 1);
+    else if (pl.ani && !pl.weighted) PCF_LAUNCH_MIXED(1, 0);
+    else PCF_LAUNCH_MIXED(1, 1);
+#undef PCF_LAUNCH_MIXED
+  } else {
+    const size_t hc_bytes = (size_t)(((pl.nflat + 1) * ncopy * 4 + 15) & ~15);
+    const size_t hw_bytes = pl.weighted ? (size_t)(pl.nflat + 1) * ncopy * 8 : 0;
+    const size_t lds = hc_bytes + hw_bytes;
+#define PCF_LAUNCH_64(A, W)                                                                    \
+  do {                                                                                         \
+    if ((e = set_lds(k_pairs_fp64<A, W>, lds)) != hipSuccess) return e;                        \
+    hipLaunchKernelGGL((k_pairs_fp64<A, W>), grid, block, lds, s, pl.xp64, pl.yp64, pl.items,  \
+                       pl.counter, pl.rules, pl.hist, pl.whist, pl.stats, k);                  \
+  } while (0)
+    if (!pl.ani && !pl.weighted) PCF_LAUNCH_64(0, 0);
+    else if (!pl.ani && pl.weighted) PCF_LAUNCH_64(0, 1);
+    else if (pl.ani && !pl.weighted) PCF_LAUNCH_64(1, 0);
+    else PCF_LAUNCH_64(1, 1);
+#undef PCF_LAUNCH_64
+  }
+  return hipGetLastError();
+}
+
+}  // namespace pcf

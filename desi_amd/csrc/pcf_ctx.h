@@ -1,0 +1,73 @@
+// pcf_ctx.h -- the context object behind the C ABI and the host helpers
+// shared by the pair, triplet and serving translation units.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/pcf_abi.h"
+#include "pcf_internal.h"
+
+namespace pcf {
+struct JobEngine;
+void destroy_engine(JobEngine* e);
+void engine_drain(JobEngine* e);
+}  // namespace pcf
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+struct pcf_ctx {
+  int device = 0;
+  int ncu = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::string err;
+  std::vector<std::unique_ptr<pcf::Catalog>> cats;
+  pcf::DevBuf scratch_cells, scratch_perm, wl_counts, wl_items, table_buf, rules_buf, hist_buf,
+      misc_buf, stats_partials;
+  pcf_stats last{};
+  pcf::JobEngine* engine = nullptr;  // asynchronous serving engine (pcf_jobs.hip)
+};
+
+
+namespace pcf {
+
+inline int fail(pcf_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+inline int hipfail(pcf_ctx* c, hipError_t e, const char* what) {
+  return fail(c, PCF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(ctx, call)                                        \
+  do {                                                           \
+    hipError_t e_ = (call);                                      \
+    if (e_ != hipSuccess) {                                      \
+      (void)hipGetLastError();                                   \
+      if (e_ == hipErrorOutOfMemory) return fail(ctx, PCF_ERR_NOMEM, #call); \
+      return hipfail(ctx, e_, #call);                            \
+    }                                                            \
+  } while (0)
+
+inline Catalog* get_cat(pcf_ctx* c, pcf_catalog id) {
+  if (id < 0 || id >= (pcf_catalog)c->cats.size()) return nullptr;
+  return c->cats[id].get();
+}
+
+
+// Defined in abi.hip.
+double ds_floored(int bins, float dmax);
+void bin_thresholds(int bins, double ds, double* T);
+int build_table(const BinRules& r, double A, int nt, std::vector<uint16_t>* table, float* inv_h);
+int choose_geometry(const Catalog* X, const Catalog* Y, float dmax, int bpc, float box,
+                    int cells_per_rmax, GridGeom* g, int K[3], float* reach_out, int* m_out);
+DevGrid* get_grid(pcf_ctx* c, Catalog* cat, const GridGeom& g, bool force_rebuild, int* rc);
+void free_catalog(Catalog* cat);
+
+// Defined in triplets.hip.
+int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights);
+
+}  // namespace pcf

@@ -1,0 +1,157 @@
+// pcf_internal.h -- types shared by the gfx950 kernels and the host side of
+// libpcf_hip.so.  Not part of the public ABI (include/pcf_abi.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace pcf {
+
+constexpr int kWave = 64;        // CDNA wavefront
+constexpr int kTile = 64;        // points of one i-tile (one per lane)
+constexpr int kBlock = 256;      // threads per workgroup of the counting kernels
+constexpr int kMaxBinsIso = 1024;
+
+// Cell grid of one job.  Cells are ordered x-major, z-fastest
+// (cell = (cx*Py + cy)*Pz + cz), the order the reference linearises its nodes
+// in (cuda/main.cu:274-317), so the cells of one (cx,cy) column are contiguous
+// in memory and a z-range of a column is one contiguous point range.
+struct GridGeom {
+  int P[3];
+  float lo[3];
+  float cs[3];
+  float inv_cs[3];
+  int bpc;
+  float L;
+  __host__ __device__ int64_t ncell() const { return (int64_t)P[0] * P[1] * P[2]; }
+  __host__ bool same(const GridGeom& o) const {
+    for (int a = 0; a < 3; ++a)
+      if (P[a] != o.P[a] || lo[a] != o.lo[a] || cs[a] != o.cs[a]) return false;
+    return bpc == o.bpc && L == o.L;
+  }
+};
+
+// One work item of the flattened pair work list: the points [i0, i0+ni) of
+// one i-tile (ni <= 64, one per lane) against the contiguous j-range
+// [j0, j0+nj) of one neighbour column segment, seen through periodic image
+// `img` = (kx+1)*9 + (ky+1)*3 + (kz+1).  Bit 5 of `code`: the range overlaps
+// the tile in an auto histogram, so only j > i is counted.  Bits 8..: slot of
+// a batched launch.
+struct PairItem {
+  int i0, ni, j0, nj, code;
+};
+
+constexpr int kImgZero = 13;  // kx = ky = kz = 0
+constexpr int kCodeMask = 32;
+
+// Exact re-evaluation rules, uploaded once per job (device copy in Job).
+struct BinRules {
+  double R;           // range limit on d^2 ((double)(float)(dmax*dmax))
+  double ds;          // bin scale
+  double L;           // box side
+  int le;             // range test uses <=
+  int zero;           // 0 none, 1 iso regular d2>0, 2 ani both >0
+  int bins;           // bins per dimension
+  int ani;
+  double T[kMaxBinsIso + 1];  // T[k] = min d2 with (int)(sqrt(d2)*ds) >= k, k=1..bins
+};
+
+// Table codes for the FP32 bulk path.
+constexpr uint16_t kAniAmb = 0x4000;
+constexpr uint16_t kAniOut = 0x8000;
+
+struct DevGrid {
+  GridGeom g;
+  float4* pts = nullptr;     // cell-ordered points
+  double4* pts64 = nullptr;  // FP64 copy for the plain path (lazily built)
+  int* cell_start = nullptr; // ncell+1
+  int64_t n = 0;
+  bool valid = false;
+};
+
+struct Catalog {
+  float4* raw = nullptr;
+  int64_t n = 0;
+  float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  float maxabs = 0;
+  bool all_w1 = true;
+  bool finite = true;
+  std::vector<std::unique_ptr<DevGrid>> grids;  // resident grids by geometry
+};
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t need) {
+    if (need <= bytes) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    size_t b = need + need / 4 + 256;
+    hipError_t e = hipMalloc(&p, b);
+    if (e == hipSuccess) bytes = b;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// ---- device entry points (defined in grid.hip / pairs.hip) -------------
+hipError_t launch_catalog_stats(const float4* raw, int64_t n, float* partials, int nblocks,
+                                hipStream_t s);
+hipError_t build_grid(const float4* raw, int64_t n, const GridGeom& g, DevGrid* out,
+                      DevBuf& scratch_cells, DevBuf& scratch_perm, hipStream_t s);
+hipError_t build_grid64(DevGrid* grid, hipStream_t s);
+
+struct PairLaunch {
+  const float4* xp;
+  const float4* yp;
+  const double4* xp64;
+  const double4* yp64;
+  const PairItem* items;
+  int nitems;
+  int* counter;
+  const uint16_t* table;
+  int nt;
+  float inv_h;
+  float L;
+  int bins;
+  int nflat;      // flat histogram bins (bins or bins^2)
+  int cshift;     // log2 of histogram copies per workgroup
+  unsigned long long* hist;
+  double* whist;
+  unsigned long long* stats;  // [0] evaluations, [1] rechecks
+  const BinRules* rules;      // device copy
+  int ani;
+  int weighted;
+  int plain64;
+  int nblocks;
+};
+hipError_t launch_pairs(const PairLaunch& pl, hipStream_t s);
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+
+struct WorkListArgs {
+  const int* x_cell_start;  // i side (tiles)
+  const int* y_cell_start;  // j side
+  GridGeom g;
+  int K[3];                 // reach in cells per axis
+  float reach;              // rmax plus slack
+  int auto_;
+};
+// Two-pass (count, scan, fill) work list on the device.  Returns the item
+// count through *nitems (a synchronous read of one integer).
+hipError_t build_pair_worklist(const WorkListArgs& a, DevBuf& counts, DevBuf& items,
+                               int* nitems, hipStream_t s);
+
+}  // namespace pcf

@@ -1,0 +1,159 @@
+/* pcf_abi.h -- C ABI of libpcf_hip.so, the MI355X (gfx950) pair/triplet
+ * histogram engine behind the PCF command line.
+ *
+ * The reference (jaarpa/DESI, "3PCFGPU") has no library API: its boundary is
+ * the PCF.out command line plus the host functions that main() calls after it
+ * has built the node grid,
+ *
+ *   pcf_2iso, pcf_2iso_BPC, pcf_2iso_BPCanalytic   cuda/pcf2iso.cuh:11, cuda/pcf2isoBPC.cuh:11,166
+ *   pcf_2ani, pcf_2aniBPC                          cuda/pcf2ani.cuh:5,  cuda/pcf2aniBPC.cuh:5
+ *   pcf_3iso, pcf_3isoBPC, pcf_3isoBPC_analytic    cuda/pcf3iso.cuh:10, cuda/pcf3isoBPC.cuh:10,177
+ *   pcf_3ani, pcf_3aniBPC                          cuda/pcf3ani.cuh:10, cuda/pcf3aniBPC.cuh:10
+ *
+ * each of which allocates device histograms, launches the XX/XY (XXX/XXY)
+ * kernels, copies the histograms back and writes the files.  The grid they
+ * receive is built on the host by make_nodos (cuda/create_grid.cuh:104-138)
+ * and copied to the device in cuda/main.cu:320-329.
+ *
+ * This ABI cuts at the same place but splits "compute" from "write files":
+ *   - pcf_catalog_upload replaces open_files' output hand-over plus the
+ *     host grid build and H2D copy (cuda/main.cu:148-329); the grid is built on
+ *     the device, per job geometry, and stays resident;
+ *   - pcf_histogram computes ONE histogram (XX, XY, XXX or XXY) of one of the
+ *     reference's types into caller-owned host arrays; the drivers in the PCF
+ *     command line compose these into the reference's DD/RR/DR and
+ *     DDD/DDR/DRR/RRR outputs and write the same files;
+ *   - pcf_analytic_rr is RR2iso_BPC_analytic (cuda/kernels/2isoBPC_k.cuh:559)
+ *     with the host constants of cuda/pcf2isoBPC.cuh:199-202;
+ *   - pcf_submit / pcf_wait are the asynchronous form used by the serving loop.
+ *
+ * Conventions: plain C types, host pointers for results, int status codes
+ * (PCF_OK == 0), never exit().  A context is not thread-safe; use one per
+ * host thread or lock externally.  Results are deterministic for counts
+ * (uint64, bit-exact to the FP64 oracle); weighted sums are FP64 with
+ * a summation order that may vary between runs (relative error ~1e-15).
+ */
+#ifndef PCF_ABI_H
+#define PCF_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define PCF_OK 0
+#define PCF_ERR_ARG -1      /* invalid argument (sizes, ids, dmax, bins ...)   */
+#define PCF_ERR_HIP -2      /* HIP runtime error (message in pcf_last_error)    */
+#define PCF_ERR_NOMEM -3    /* device allocation failed or over the HBM budget  */
+#define PCF_ERR_RANGE -4    /* coordinates outside the periodic box, non-finite */
+#define PCF_ERR_UNSUPPORTED -5 /* combination the engine does not implement    */
+#define PCF_ERR_BUSY -6     /* serving queue full (retry after pcf_wait)        */
+
+/* ---- histogram types (argv[1] of the reference, cuda/main.cu:59) ------ */
+#define PCF_2ISO 0
+#define PCF_2ANI 1
+#define PCF_3ISO 2
+#define PCF_3ANI 3
+
+/* ---- which histogram ------------------------------------------------- */
+#define PCF_XX 0   /* DD or RR      (XX2iso, cuda/kernels/2iso_k.cuh:8)   */
+#define PCF_XY 1   /* DR            (XY2iso, cuda/kernels/2iso_k.cuh:63)  */
+#define PCF_XXX 2  /* DDD or RRR    (XXX3iso, cuda/kernels/3iso_k.cuh:8)  */
+#define PCF_XXY 3  /* DDR or DRR    (XXY3iso, cuda/kernels/3iso_k.cuh:94) */
+
+/* ---- arithmetic ------------------------------------------------------ */
+#define PCF_PREC_MIXED 0 /* FP32 distances, FP64 re-evaluation near bin edges */
+#define PCF_PREC_FP64 1  /* plain FP64 distances and sqrt for every pair      */
+
+typedef struct pcf_ctx pcf_ctx;
+typedef int32_t pcf_catalog;
+
+/* One histogram request.  Field meanings follow the reference flags:
+ * bins = -b, dmax = -d, bpc = -bpc, box = -s (or the inferred box side),
+ * partitions = -p (accepted for drop-in compatibility; the engine picks its
+ * own cell size, which does not change any result). */
+typedef struct pcf_job {
+  int32_t type;        /* PCF_2ISO .. PCF_3ANI                            */
+  int32_t hist;        /* PCF_XX .. PCF_XXY                               */
+  int32_t bpc;         /* periodic box                                    */
+  int32_t bins;        /* bins per histogram dimension                    */
+  float dmax;          /* maximum separation                              */
+  float box;           /* box side (required when bpc)                    */
+  int32_t precision;   /* PCF_PREC_MIXED or PCF_PREC_FP64                 */
+  pcf_catalog cat_x;   /* X: the auto catalog, or D of a cross histogram  */
+  pcf_catalog cat_y;   /* Y: R of a cross histogram; -1 for XX / XXX      */
+  int32_t partitions;  /* -p; informational                               */
+  int32_t cells_per_rmax; /* 0 = autotuned cell size, else rmax/cell edge  */
+  int32_t reserved[5];
+} pcf_job;
+
+/* Per-call statistics of the last completed histogram. */
+typedef struct pcf_stats {
+  uint64_t evaluations;  /* candidate pairs (or triplets) evaluated       */
+  uint64_t rechecks;     /* evaluations re-done in FP64 near a bin edge   */
+  uint64_t work_items;   /* entries of the flattened work list            */
+  double kernel_ms;      /* device time of the counting kernel(s)         */
+  double total_ms;       /* device time of grid build + counting + merge  */
+  int32_t cells_per_rmax;
+  int32_t reserved[3];
+} pcf_stats;
+
+/* Sizes of a histogram: bins (2iso), bins^2 (2ani), bins^3 (3iso),
+ * bins^5 (3ani). */
+int64_t pcf_hist_size(int32_t type, int32_t bins);
+
+/* Context on HIP device `device`.  Replaces the CUDA context set-up implied by
+ * cuda/main.cu (no explicit call there). */
+int pcf_create(pcf_ctx** out, int32_t device);
+void pcf_destroy(pcf_ctx* ctx);
+/* Message of the last failing call on this context ("" if none). */
+const char* pcf_last_error(const pcf_ctx* ctx);
+/* Thread-independent message for a failed pcf_create (ctx unavailable). */
+const char* pcf_create_error(void);
+
+/* Upload n points, 4 floats each (x, y, z, w; the 4-column catalog format,
+ * readme.md:36-40), from host memory into a device-resident catalog.
+ * Replaces the hand-over of open_files (cuda/create_grid.cuh:35) to the grid
+ * build and the copy to the device (cuda/main.cu:320-329). */
+int pcf_catalog_upload(pcf_ctx* ctx, const float* xyzw, int64_t n, pcf_catalog* id);
+/* Same, from device memory already in HBM (copied device-to-device). */
+int pcf_catalog_upload_device(pcf_ctx* ctx, const void* d_xyzw, int64_t n, pcf_catalog* id);
+int pcf_catalog_release(pcf_ctx* ctx, pcf_catalog id);
+/* Number of points of a catalog, or -1. */
+int64_t pcf_catalog_size(pcf_ctx* ctx, pcf_catalog id);
+
+/* Compute one histogram.  counts[pcf_hist_size] receives the number of
+ * ordered pairs (triplets) per bin, weights[...] the sum of the products of
+ * the FP32 weights formed in FP64.  Either output may be NULL.  The layout is
+ * the reference GPU layout (cuda/kernels/ *_k.cuh files).  Blocks until done.
+ * Replaces the kernel launches + D2H copies of one histogram in the
+ * reference drivers (e.g. cuda/pcf2iso.cuh:95-124). */
+int pcf_histogram(pcf_ctx* ctx, const pcf_job* job, uint64_t* counts, double* weights);
+
+/* Statistics of the last pcf_histogram / pcf_wait on this context. */
+int pcf_last_stats(const pcf_ctx* ctx, pcf_stats* out);
+
+/* Analytic RR of the periodic 2iso case with np points in a box of side box
+ * (cuda/pcf2isoBPC.cuh:199-202 + cuda/kernels/2isoBPC_k.cuh:559-575),
+ * computed on the device in FP64 with 64-bit integer arithmetic. */
+int pcf_analytic_rr(pcf_ctx* ctx, int32_t bins, int64_t np, float dmax, float box, double* rr);
+
+/* ---- asynchronous form (serving) ------------------------------------- *
+ * pcf_submit enqueues a histogram job; the engine admits it when enough HBM
+ * (the budget set by pcf_set_hbm_budget) and a stream slot are free, runs
+ * admitted jobs concurrently on separate streams, and keeps the gridded
+ * catalogs resident for reuse.  pcf_wait blocks until the job has finished
+ * and copies its results out (counts/weights may be NULL). */
+typedef int64_t pcf_ticket;
+int pcf_set_hbm_budget(pcf_ctx* ctx, int64_t bytes, int32_t max_inflight);
+int pcf_submit(pcf_ctx* ctx, const pcf_job* job, pcf_ticket* ticket);
+int pcf_poll(pcf_ctx* ctx, pcf_ticket ticket, int32_t* done);
+int pcf_wait(pcf_ctx* ctx, pcf_ticket ticket, uint64_t* counts, double* weights,
+             pcf_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PCF_ABI_H */

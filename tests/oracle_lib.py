@@ -1,0 +1,117 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle_pcf.so).
+
+Test infrastructure: imported only by tests/, the smoke check and the CPU
+baseline leg of bench.py.  The oracle is the checker, never the product.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_LIB_PATH = os.path.join(ROOT, "oracle", "liboracle_pcf.so")
+_lib = None
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = ctypes.CDLL(_LIB_PATH)
+        L.pcf_oracle_pairs.argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32, _i32, ctypes.c_float,
+                                       _i32, ctypes.c_float, _i32, _u64p, _f64p]
+        L.pcf_oracle_pairs_brute.argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32, _i32,
+                                             ctypes.c_float, _i32, ctypes.c_float, _u64p, _f64p]
+        L.pcf_oracle_cpp32_2iso.argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32, ctypes.c_float,
+                                            _i32, _f64p]
+        L.pcf_oracle_analytic_rr.argtypes = [_i32, _i64, ctypes.c_float, ctypes.c_float, _f64p]
+        L.pcf_oracle_analytic_rr.restype = None
+        for name in ("pcf_oracle_triplets", "pcf_oracle_triplets_brute"):
+            if hasattr(L, name):
+                getattr(L, name).argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32, _i32,
+                                             ctypes.c_float, _i32, ctypes.c_float, _i32,
+                                             _u64p, _f64p]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _cat(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    assert a.ndim == 2 and a.shape[1] == 4
+    return a
+
+
+def pairs(a, b=None, *, ani=False, bpc=False, box=0.0, bins=30, dmax=80.0, threads=0,
+          brute=False):
+    """Ordered-pair histogram (counts uint64, weights float64) in the FP64
+    canonical mode.  b=None means the auto histogram (XX)."""
+    a = _cat(a)
+    auto = b is None
+    b = a if auto else _cat(b)
+    n = bins * bins if ani else bins
+    c = np.zeros(n, np.uint64)
+    w = np.zeros(n, np.float64)
+    if brute:
+        rc = lib().pcf_oracle_pairs_brute(_p(a, _f32p), len(a), _p(b, _f32p), len(b), int(auto),
+                                          int(ani), int(bpc), box, bins, dmax,
+                                          _p(c, _u64p), _p(w, _f64p))
+    else:
+        rc = lib().pcf_oracle_pairs(_p(a, _f32p), len(a), _p(b, _f32p), len(b), int(auto),
+                                    int(ani), int(bpc), box, bins, dmax, threads,
+                                    _p(c, _u64p), _p(w, _f64p))
+    if rc != 0:
+        raise ValueError(f"oracle pairs rc={rc}")
+    return c, w
+
+
+def cpp32_2iso(a, b=None, *, bins=20, dmax=150.0, threads=0):
+    a = _cat(a)
+    auto = b is None
+    b = a if auto else _cat(b)
+    h = np.zeros(bins, np.float64)
+    rc = lib().pcf_oracle_cpp32_2iso(_p(a, _f32p), len(a), _p(b, _f32p), len(b), int(auto), bins,
+                                     dmax, threads, _p(h, _f64p))
+    if rc != 0:
+        raise ValueError(f"oracle cpp32 rc={rc}")
+    return h
+
+
+def analytic_rr(bins, np_, dmax, box):
+    h = np.zeros(bins, np.float64)
+    lib().pcf_oracle_analytic_rr(bins, np_, dmax, box, _p(h, _f64p))
+    return h
+
+
+def triplets(a, b=None, *, ani=False, bpc=False, box=0.0, bins=30, dmax=80.0, threads=0,
+             brute=False, ddr=True):
+    """Ordered-triplet histogram.  b=None: XXX over a.  Otherwise XXY with the
+    pair (1,2) from a and point 3 from b."""
+    a = _cat(a)
+    auto = b is None
+    b = a if auto else _cat(b)
+    n = bins ** 5 if ani else bins ** 3
+    c = np.zeros(n, np.uint64)
+    w = np.zeros(n, np.float64)
+    L = lib()
+    if brute:
+        rc = L.pcf_oracle_triplets_brute(_p(a, _f32p), len(a), _p(b, _f32p), len(b), int(auto),
+                                         int(ani), int(bpc), box, bins, dmax, 1,
+                                         _p(c, _u64p), _p(w, _f64p))
+    else:
+        rc = L.pcf_oracle_triplets(_p(a, _f32p), len(a), _p(b, _f32p), len(b), int(auto),
+                                   int(ani), int(bpc), box, bins, dmax, threads,
+                                   _p(c, _u64p), _p(w, _f64p))
+    if rc != 0:
+        raise ValueError(f"oracle triplets rc={rc}")
+    return c, w
