@@ -1,0 +1,187 @@
+"""CPU checks that pin the oracle (oracle/pcf_oracle.cpp) before it is trusted.
+
+* cpp32 mode vs the outputs of the reference's own CPU programs, compiled
+  unmodified and run on the reference's sample catalogs (tests/golden/ref_out,
+  written by tests/golden/make_ref_golden.sh): bit-exact.
+* FP64 mode vs independent integer arithmetic on dyadic lattices, where every
+  squared distance is an exact integer and the bin is an integer square root.
+* cell-grid enumeration vs brute force over all ordered pairs and all 8
+  periodic projections.
+* periodic mode vs minimum-image distances and translation invariance.
+* analytic RR vs the shell-volume closed form.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from conftest import GOLDEN, catalog
+from desi_amd import synth
+
+
+def _ref(prog, name):
+    return np.loadtxt(os.path.join(GOLDEN, "ref_out", prog, name))
+
+
+# ---------------------------------------------------------------------------
+# cpp32 mode against the compiled reference programs
+# ---------------------------------------------------------------------------
+def test_cpp32_2iso_matches_compiled_reference():
+    # cpp/2PCFisotropic/normal/main.cpp:25-27: N=32768, bn=20, dmax=150.
+    d = catalog("data.dat")
+    r = catalog("rand0.dat")
+    dd = O.cpp32_2iso(d, bins=20, dmax=150.0)
+    rr = O.cpp32_2iso(r, bins=20, dmax=150.0)
+    dr = O.cpp32_2iso(d, r, bins=20, dmax=150.0)
+    assert np.array_equal(dd, _ref("2iso_normal", "DDiso_mesh_3D_ref.dat"))
+    assert np.array_equal(rr, _ref("2iso_normal", "RRiso_mesh_3D_ref.dat"))
+    assert np.array_equal(dr, _ref("2iso_normal", "DRiso_mesh_3D_ref.dat"))
+    assert dd.sum() == 442528646 and dr.sum() == 441325784
+
+
+def test_cpp32_planted_error_is_caught():
+    # A dropped term in the distance must fail the bit-exact comparison: shift
+    # one coordinate of the data by one FP32 ulp-scale amount and check the
+    # histogram moves (the comparison above is therefore sensitive).
+    d = catalog("data.dat").copy()
+    d[:, 2] = np.nextafter(d[:, 2] * np.float32(1.0001), np.float32(1e9))
+    dd = O.cpp32_2iso(d, bins=20, dmax=150.0)
+    assert not np.array_equal(dd, _ref("2iso_normal", "DDiso_mesh_3D_ref.dat"))
+
+
+# ---------------------------------------------------------------------------
+# FP64 mode against exact integer arithmetic
+# ---------------------------------------------------------------------------
+def _isqrt_bins_iso(a, b, bins, dmax, auto, box=None):
+    """Reference GPU rules on an integer lattice (coordinates are integers,
+    ds = bins/dmax = 1): count ordered pairs with 0 < d2 < dmax^2 (XX) or
+    d2 < dmax^2 (XY), bin = isqrt(d2)."""
+    ai = a[:, :3].astype(np.int64)
+    bi = b[:, :3].astype(np.int64)
+    h = np.zeros(bins, np.uint64)
+    for i in range(len(ai)):
+        dlt = np.abs(bi - ai[i])
+        if box is not None:
+            dlt = np.minimum(dlt, int(box) - dlt)
+        d2 = (dlt * dlt).sum(axis=1)
+        sel = d2 < dmax * dmax
+        if auto:
+            sel &= d2 > 0
+        k = np.array([math.isqrt(int(v)) for v in d2[sel]], np.int64)
+        k = k[k < bins]
+        h += np.bincount(k, minlength=bins).astype(np.uint64)
+    return h
+
+
+def _isqrt_bins_ani(a, b, bins, dmax, zero_rule, box=None, le=False):
+    ai = a[:, :3].astype(np.int64)
+    bi = b[:, :3].astype(np.int64)
+    h = np.zeros(bins * bins, np.uint64)
+    R = dmax * dmax
+    for i in range(len(ai)):
+        dlt = np.abs(bi - ai[i])
+        if box is not None:
+            dlt = np.minimum(dlt, int(box) - dlt)
+        z2 = dlt[:, 2] ** 2
+        o2 = dlt[:, 0] ** 2 + dlt[:, 1] ** 2
+        sel = (z2 <= R) & (o2 <= R) if le else (z2 < R) & (o2 < R)
+        if zero_rule:
+            sel &= (z2 > 0) & (o2 > 0)
+        bz = np.array([math.isqrt(int(v)) for v in z2[sel]], np.int64)
+        bo = np.array([math.isqrt(int(v)) for v in o2[sel]], np.int64)
+        ok = (bz < bins) & (bo < bins)
+        h += np.bincount(bz[ok] * bins + bo[ok], minlength=bins * bins).astype(np.uint64)
+    return h
+
+
+@pytest.mark.parametrize("auto", [True, False])
+def test_fp64_iso_integer_lattice(auto):
+    a = synth.lattice(600, 32.0, 1.0, seed=1)
+    b = a if auto else synth.lattice(500, 32.0, 1.0, seed=2)
+    c, _ = O.pairs(a, None if auto else b, bins=10, dmax=10.0)
+    assert np.array_equal(c, _isqrt_bins_iso(a, b, 10, 10, auto))
+
+
+@pytest.mark.parametrize("auto", [True, False])
+def test_fp64_ani_integer_lattice(auto):
+    a = synth.lattice(600, 32.0, 1.0, seed=3)
+    b = a if auto else synth.lattice(500, 32.0, 1.0, seed=4)
+    c, _ = O.pairs(a, None if auto else b, ani=True, bins=10, dmax=10.0)
+    assert np.array_equal(c, _isqrt_bins_ani(a, b, 10, 10, zero_rule=True))
+
+
+@pytest.mark.parametrize("auto", [True, False])
+def test_fp64_bpc_minimum_image_lattice(auto):
+    # 2*dmax < box: the reference's projections equal the minimum image.
+    a = synth.lattice(500, 32.0, 1.0, seed=5)
+    b = a if auto else synth.lattice(400, 32.0, 1.0, seed=6)
+    c, _ = O.pairs(a, None if auto else b, bpc=True, box=32.0, bins=10, dmax=10.0)
+    assert np.array_equal(c, _isqrt_bins_iso(a, b, 10, 10, auto, box=32))
+    c, _ = O.pairs(a, None if auto else b, ani=True, bpc=True, box=32.0, bins=10, dmax=10.0)
+    # periodic 2ani: <= range test; zero components excluded only in XX
+    assert np.array_equal(c, _isqrt_bins_ani(a, b, 10, 10, zero_rule=auto, box=32, le=True))
+
+
+def test_fp64_bpc_translation_invariance():
+    a = synth.lattice(800, 64.0, 0.5, seed=7)
+    shifted = a.copy()
+    shifted[:, :3] = np.mod(a[:, :3] + np.array([13.5, 40.0, 63.5], np.float32), 64.0)
+    for ani in (False, True):
+        c1, _ = O.pairs(a, bpc=True, ani=ani, box=64.0, bins=12, dmax=12.0)
+        c2, _ = O.pairs(shifted, bpc=True, ani=ani, box=64.0, bins=12, dmax=12.0)
+        assert np.array_equal(c1, c2)
+
+
+# ---------------------------------------------------------------------------
+# Enumeration: cell grid vs brute force
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("ani", [False, True])
+@pytest.mark.parametrize("bpc", [False, True])
+def test_grid_equals_brute(ani, bpc):
+    a = synth.uniform(600, 100.0, seed=21, weights="uniform")
+    b = synth.uniform(450, 100.0, seed=22, weights="uniform")
+    for bb in (None, b):
+        for dmax in (17.0, 40.0, 70.0):  # 70 > box/2: several images per pair
+            c1, w1 = O.pairs(a, bb, ani=ani, bpc=bpc, box=100.0, bins=9, dmax=dmax)
+            c2, w2 = O.pairs(a, bb, ani=ani, bpc=bpc, box=100.0, bins=9, dmax=dmax, brute=True)
+            assert np.array_equal(c1, c2)
+            assert np.allclose(w1, w2, rtol=1e-13, atol=0)
+
+
+def test_weights_are_products():
+    a = synth.uniform(300, 50.0, seed=31, weights="uniform")
+    b = synth.uniform(200, 50.0, seed=32, weights="uniform")
+    c, w = O.pairs(a, b, bins=5, dmax=20.0, brute=True)
+    # Sum of w_i*w_j over pairs in range, computed independently in float64.
+    d2 = ((a[:, None, :3].astype(np.float64) - b[None, :, :3]) ** 2).sum(-1)
+    sel = d2 < np.float64(np.float32(400.0))
+    ww = (a[:, None, 3].astype(np.float64) * b[None, :, 3])[sel].sum()
+    assert c.sum() == sel.sum()
+    assert abs(w.sum() - ww) <= 1e-12 * ww
+
+
+def test_real_catalog_fp64_vs_cpp32_close():
+    # FP64 and the reference's FP32 arithmetic differ only by pairs within a
+    # rounding of a bin edge or the range limit (Appendix of the survey):
+    # a handful out of ~4e8.
+    d = catalog("data.dat")
+    c, _ = O.pairs(d, bins=20, dmax=150.0)
+    ref = _ref("2iso_normal", "DDiso_mesh_3D_ref.dat")
+    diff = np.abs(c.astype(np.float64) - ref)
+    assert diff.sum() < 200 and diff.sum() > 0
+
+
+# ---------------------------------------------------------------------------
+# Analytic RR
+# ---------------------------------------------------------------------------
+def test_analytic_rr_closed_form():
+    for bins, n, dmax, box in ((30, 32768, 80.0, 250.0), (20, 1 << 20, 150.0, 1000.0),
+                               (200, 50000, 60.0, 500.0)):
+        rr = O.analytic_rr(bins, n, dmax, box)
+        dr = dmax / bins
+        shell = np.array([(i + 1) ** 3 - i ** 3 for i in range(bins)], np.float64)
+        closed = 4 * np.pi / 3 * n * n / box ** 3 * dr ** 3 * shell
+        assert np.allclose(rr, closed, rtol=2e-6, atol=0)
+        assert abs(rr.sum() - 4 * np.pi / 3 * n * n * dmax ** 3 / box ** 3) < 2e-6 * rr.sum()
