@@ -382,6 +382,7 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   for (int a = 0; a < 3; ++a) wa.K[a] = K[a];
   wa.reach = reach;
   wa.auto_ = auto_ ? 1 : 0;
+  wa.cylinder = ani ? 1 : 0;
   int nitems = 0;
   HIPCHK(c, build_pair_worklist(wa, c->wl_counts, c->wl_items, &nitems, c->stream));
 

@@ -72,7 +72,9 @@ __device__ int enumerate_cell(const WorkListArgs& a, int cell, PairItem* out) {
         const float gy = (float)max(0, abs(dy) - 1) * g.cs[1];
         const float rho2 = gx * gx + gy * gy;
         if (rho2 >= reach2) continue;
-        const float hz = sqrtf(reach2 - rho2);
+        // Sphere: the z reach shrinks with the lateral gap.  Cylinder (2ani):
+        // the z and perpendicular ranges are independent.
+        const float hz = a.cylinder ? a.reach : sqrtf(reach2 - rho2);
         const int Kz = min(a.K[2], 1 + (int)floorf(hz * g.inv_cs[2]));
         const int col = (ccx * Py + ccy) * Pz;
         int zlo = cz - Kz, zhi = cz + Kz;

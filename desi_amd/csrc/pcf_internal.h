@@ -148,6 +148,7 @@ struct WorkListArgs {
   int K[3];                 // reach in cells per axis
   float reach;              // rmax plus slack
   int auto_;
+  int cylinder;             // 2ani: |dz| < rmax and d_perp < rmax (not a sphere)
 };
 // Two-pass (count, scan, fill) work list on the device.  Returns the item
 // count through *nitems (a synchronous read of one integer).
