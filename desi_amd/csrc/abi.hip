@@ -99,17 +99,28 @@ static double err_bound(double d2, double A) {
   return 2.0 * (5 * u * d2 + 3.5 * u * A * std::sqrt(d2) + 6 * u * u * A * A) + 1e-300;
 }
 
-int build_table(const BinRules& r, double A, int nt, std::vector<uint16_t>* table, float* inv_h_out) {
+// The table scale inv_h is a power of four, so the bulk path can pre-scale
+// coordinates by sigma = sqrt(inv_h) (a power of two, exact in FP32) and
+// convert d2 straight to a table index.  *nt_out <= nt_max entries cover
+// [0, Vmax) where everything beyond Vmax is certainly out of range.
+int build_table(const BinRules& r, double A, int nt_max, std::vector<uint8_t>* table,
+                float* inv_h_out, int* nt_out) {
   const int bins = r.bins;
   const bool ani = r.ani != 0;
   double U = std::min(r.R, r.T[bins]);
   double vmax = U * (1.0 + 1e-3) + 4.0 * err_bound(2 * U + 1, A) + 1e-30;
-  float inv_h = (float)((double)(nt - 1) / vmax);
-  const double ih = (double)inv_h;
+  int e2 = (int)std::floor(std::log2((double)(nt_max - 2) / vmax) / 2.0);
+  if (e2 > 60) e2 = 60;
+  if (e2 < -60) e2 = -60;
+  double ih = std::ldexp(1.0, 2 * e2);
+  while (vmax * ih > (double)(nt_max - 2)) ih *= 0.25;
+  const int nt = (int)std::ceil(vmax * ih) + 2;
+  const float inv_h = (float)ih;
   const double u = std::ldexp(1.0, -24);
   table->assign(nt, 0);
-  const uint16_t out_code = ani ? kAniOut : (uint16_t)bins;
-  const uint16_t amb_code = ani ? kAniAmb : (uint16_t)(bins + 1);
+  if (bins > (ani ? 200 : 253)) return -1;
+  const uint8_t out_code = ani ? kAniOut8 : (uint8_t)bins;
+  const uint8_t amb_code = ani ? kAniAmb8 : (uint8_t)(bins + 1);
   for (int t = 0; t < nt; ++t) {
     // FP32 values v with min(cvt_rz(fl(v*inv_h)), nt-1) == t.
     double v_lo = (double)t * (1.0 - 4 * u) / ih;
@@ -117,19 +128,20 @@ int build_table(const BinRules& r, double A, int nt, std::vector<uint16_t>* tabl
                                 : (double)(t + 1) * (1.0 + 4 * u) / ih;
     double d_lo = v_lo - 2.0 * err_bound(2 * v_lo + 1, A);
     double d_hi = std::isinf(v_hi) ? v_hi : v_hi + 2.0 * err_bound(2 * v_hi + 1, A);
-    uint16_t code;
+    uint8_t code;
     if (r.zero != 0 && d_lo <= 0.0) {
       code = amb_code;  // exact zero is excluded, anything above is bin 0
     } else {
       int c_lo = host_class(std::max(d_lo, 0.0), r);
       int c_hi = std::isinf(d_hi) ? -1 : host_class(d_hi, r);
-      if (c_lo == c_hi) code = (c_lo < 0) ? out_code : (uint16_t)c_lo;
+      if (c_lo == c_hi) code = (c_lo < 0) ? out_code : (uint8_t)c_lo;
       else code = amb_code;
     }
     (*table)[t] = code;
   }
   if ((*table)[nt - 1] != out_code) return -1;
   *inv_h_out = inv_h;
+  *nt_out = nt;
   return 0;
 }
 
@@ -205,7 +217,14 @@ int choose_geometry(const Catalog* X, const Catalog* Y, float dmax, int bpc, flo
   if (m > 8) m = 8;
   double ext_max = 0;
   for (int a = 0; a < 3; ++a) ext_max = std::max(ext_max, bpc ? (double)box : hi[a] - lo[a]);
-  double cs_target = (double)dmax / m;
+  // Slack: a point may sit ~3u*(|coordinate| + extent) outside its nominal
+  // cell after the FP32 cell assignment (per axis, both points of a pair), and
+  // the FP32 pruning arithmetic rounds at ~1e-7 relative.
+  const double delta = 4 * u * (2 * ext_max + std::fabs(lo[0]) + std::fabs(lo[1]) +
+                                std::fabs(lo[2]) + 2 * (double)dmax) + 1e-7;
+  double reach = (double)dmax * (1.0 + 1e-5) + 4 * delta;
+  // Cell edge a hair above reach/m, so the neighbour reach is exactly m cells.
+  double cs_target = reach / m * (1.0 + 1e-6);
   g->bpc = bpc;
   g->L = box;
   for (int iter = 0; iter < 64; ++iter) {
@@ -231,13 +250,7 @@ int choose_geometry(const Catalog* X, const Catalog* Y, float dmax, int bpc, flo
     if (ncell <= (int64_t)1 << 22) break;
     cs_target *= 1.25;
   }
-  // Slack for the FP32 cell assignment (a point may sit ~3u*(extent+cs) outside
-  // its nominal cell) and for the FP32 pruning arithmetic.
-  double delta = 4 * u * (ext_max + std::fabs(lo[0]) + std::fabs(lo[1]) + std::fabs(lo[2]) +
-                          2 * cs_target) + 1e-7;
-  double reach = (double)dmax * (1.0 + 1e-5) + 2 * delta;
   for (int a = 0; a < 3; ++a) {
-    reach = std::max(reach, reach);
     K[a] = (int)std::ceil(reach / g->cs[a]);
     if (bpc && K[a] >= g->P[a]) return -1;
   }
@@ -361,17 +374,19 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   static thread_local BinRules rules;
   make_rules(job->type, bpc, auto_, bins, job->dmax, L, &rules);
   double A = bpc ? (double)std::max(X->maxabs, Y->maxabs) + (double)L : 0.0;
-  const int nt = 8192;
-  static thread_local std::vector<uint16_t> table;
-  float inv_h = 0.f;
-  if (!plain && build_table(rules, A, nt, &table, &inv_h) != 0)
+  const int nt_max = kMixedTableMax;
+  static thread_local std::vector<uint8_t> table;
+  float inv_h = 1.f;
+  int nt = 2;
+  if (!plain && build_table(rules, A, nt_max, &table, &inv_h, &nt) != 0)
     return fail(c, PCF_ERR_UNSUPPORTED, "bin table");
+  const float sigma = std::sqrt(inv_h);  // exact power of two
   HIPCHK(c, c->rules_buf.ensure(sizeof(BinRules)));
-  HIPCHK(c, c->table_buf.ensure(sizeof(uint16_t) * nt));
+  HIPCHK(c, c->table_buf.ensure(nt));
   HIPCHK(c, hipMemcpyAsync(c->rules_buf.p, &rules, sizeof(BinRules), hipMemcpyHostToDevice,
                            c->stream));
   if (!plain)
-    HIPCHK(c, hipMemcpyAsync(c->table_buf.p, table.data(), sizeof(uint16_t) * nt,
+    HIPCHK(c, hipMemcpyAsync(c->table_buf.p, table.data(), nt,
                              hipMemcpyHostToDevice, c->stream));
 
   // Work list.
@@ -400,15 +415,28 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   PairLaunch pl{};
   pl.xp = gx->pts;
   pl.yp = gy->pts;
+  if (!plain) {
+    // Scaled copies for the bulk path (x, y, z times sigma, exact).
+    HIPCHK(c, c->scaled_x.ensure(sizeof(float4) * (size_t)std::max<int64_t>(gx->n, 1)));
+    HIPCHK(c, launch_scale_points(gx->pts, gx->n, sigma, c->scaled_x.as<float4>(), c->stream));
+    pl.xs = c->scaled_x.as<float4>();
+    if (auto_) {
+      pl.ys = pl.xs;
+    } else {
+      HIPCHK(c, c->scaled_y.ensure(sizeof(float4) * (size_t)std::max<int64_t>(gy->n, 1)));
+      HIPCHK(c, launch_scale_points(gy->pts, gy->n, sigma, c->scaled_y.as<float4>(), c->stream));
+      pl.ys = c->scaled_y.as<float4>();
+    }
+  }
   pl.xp64 = gx->pts64;
   pl.yp64 = gy->pts64;
   pl.items = c->wl_items.as<PairItem>();
   pl.nitems = nitems;
   pl.counter = d_counter;
-  pl.table = c->table_buf.as<uint16_t>();
+  pl.table = c->table_buf.as<uint8_t>();
   pl.nt = nt;
   pl.inv_h = inv_h;
-  pl.L = L;
+  pl.L = L * sigma;
   pl.bins = bins;
   pl.nflat = nflat;
   const int lds_budget = 48 * 1024;
@@ -422,8 +450,15 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   pl.ani = ani ? 1 : 0;
   pl.weighted = weighted ? 1 : 0;
   pl.plain64 = plain ? 1 : 0;
-  const size_t lds_bytes = (plain ? 0 : (size_t)nt * 2) + (size_t)rows * (1 << pl.cshift) * per;
-  int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / (lds_bytes + 1024)));
+  pl.R = rules.R;
+  pl.Lx = (double)L;
+  pl.le = rules.le;
+  pl.zero = rules.zero;
+  const size_t lds_bytes = (plain ? 0 : (size_t)kMixedTableMax) +
+                           (size_t)rows * (1 << pl.cshift) * per + (size_t)(bins + 2) * 8;
+  const int threads = plain ? kBlock : kMixedBlock;
+  int per_cu = (int)std::max<size_t>(1, (160 * 1024) / (lds_bytes + 512));
+  per_cu = std::min(per_cu, 2048 / threads);
   pl.nblocks = std::max(1, c->ncu * per_cu);
   if (nitems > 0) {
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
@@ -528,7 +563,8 @@ void pcf_destroy(pcf_ctx* c) {
   for (auto& cat : c->cats)
     if (cat) free_catalog(cat.get());
   for (DevBuf* b : {&c->scratch_cells, &c->scratch_perm, &c->wl_counts, &c->wl_items,
-                    &c->table_buf, &c->rules_buf, &c->hist_buf, &c->misc_buf, &c->stats_partials})
+                    &c->table_buf, &c->rules_buf, &c->hist_buf, &c->misc_buf, &c->stats_partials,
+                    &c->scaled_x, &c->scaled_y})
     b->release();
   for (auto& ev : c->ev)
     if (ev) (void)hipEventDestroy(ev);

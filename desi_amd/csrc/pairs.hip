@@ -32,6 +32,10 @@ namespace {
 // ---------------------------------------------------------------------------
 // Work list
 // ---------------------------------------------------------------------------
+// Tiles are 64 consecutive points of one (cx,cy) column (they may span a few
+// z-cells), so every lane of a wave holds a point; a tile's j-range in a
+// neighbour column covers the z-cells within reach of any of its cells.  The
+// thread of cell c emits the tiles whose first point lies in c.
 template <bool FILL>
 __device__ int enumerate_cell(const WorkListArgs& a, int cell, PairItem* out) {
   const int ib = a.x_cell_start[cell];
@@ -42,10 +46,18 @@ __device__ int enumerate_cell(const WorkListArgs& a, int cell, PairItem* out) {
   const int cz = cell % Pz;
   const int cy = (cell / Pz) % Py;
   const int cx = cell / (Pz * Py);
+  const int colbase = cell - cz;
+  const int col_start = a.x_cell_start[colbase];
+  const int col_end = a.x_cell_start[colbase + Pz];
   const float reach2 = a.reach * a.reach;
   int count = 0;
-  for (int i0 = ib; i0 < ie; i0 += kTile) {
-    const int ni = min(kTile, ie - i0);
+  // First tile boundary at or after ib (tiles start at col_start + 64k).
+  int i0 = col_start + ((ib - col_start + kTile - 1) / kTile) * kTile;
+  for (; i0 < ie; i0 += kTile) {
+    const int ni = min(kTile, col_end - i0);
+    // z-cell of the tile's last point.
+    int zb = cz;
+    while (zb + 1 < Pz && a.x_cell_start[colbase + zb + 1] <= i0 + ni - 1) ++zb;
     for (int dx = -a.K[0]; dx <= a.K[0]; ++dx) {
       int ccx = cx + dx, kx = 0;
       if (ccx < 0) {
@@ -75,10 +87,11 @@ __device__ int enumerate_cell(const WorkListArgs& a, int cell, PairItem* out) {
         // Sphere: the z reach shrinks with the lateral gap.  Cylinder (2ani):
         // the z and perpendicular ranges are independent.
         const float hz = a.cylinder ? a.reach : sqrtf(reach2 - rho2);
-        const int Kz = min(a.K[2], 1 + (int)floorf(hz * g.inv_cs[2]));
+        const int Kz = min(a.K[2], (int)ceilf(hz * g.inv_cs[2]));
         const int col = (ccx * Py + ccy) * Pz;
-        int zlo = cz - Kz, zhi = cz + Kz;
+        const int zlo = cz - Kz, zhi = zb + Kz;
         // Up to three z segments: wrapped below (kz=-1), inside, wrapped above.
+        // zlo > -Pz and zhi < 2*Pz because Kz < Pz.
         int seg_lo[3], seg_hi[3], seg_k[3], nseg = 0;
         if (g.bpc) {
           if (zlo < 0) { seg_lo[nseg] = zlo + Pz; seg_hi[nseg] = Pz - 1; seg_k[nseg++] = -1; }
@@ -176,40 +189,6 @@ __device__ __forceinline__ bool axis_sq(float ai, float aj, int k, double L, dou
   return true;
 }
 
-__device__ __forceinline__ int class_of(double d2, const BinRules* __restrict__ r) {
-  if (r->le ? !(d2 <= r->R) : !(d2 < r->R)) return -1;
-  const int bins = r->bins;
-  if (d2 >= r->T[bins]) return -1;
-  int lo = 0, hi = bins - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (r->T[mid] <= d2) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
-__device__ __noinline__ int recheck_pair(float4 pi, float4 pj, int img,
-                                         const BinRules* __restrict__ r) {
-  const int kx = img / 9 - 1, ky = (img / 3) % 3 - 1, kz = img % 3 - 1;
-  double tx, ty, tz;
-  if (!axis_sq(pi.x, pj.x, kx, r->L, &tx)) return -1;
-  if (!axis_sq(pi.y, pj.y, ky, r->L, &ty)) return -1;
-  if (!axis_sq(pi.z, pj.z, kz, r->L, &tz)) return -1;
-  if (!r->ani) {
-    double d2 = __dadd_rn(__dadd_rn(tx, ty), tz);
-    if (r->zero == 1 && img == kImgZero && !(d2 > 0.0)) return -1;
-    return class_of(d2, r);
-  }
-  double do2 = __dadd_rn(tx, ty);
-  if (r->zero == 2 && !(tz > 0.0 && do2 > 0.0)) return -1;
-  int bz = class_of(tz, r);
-  if (bz < 0) return -1;
-  int bo = class_of(do2, r);
-  if (bo < 0) return -1;
-  return bz * r->bins + bo;
-}
-
 // ---------------------------------------------------------------------------
 // Counting kernels
 // ---------------------------------------------------------------------------
@@ -217,10 +196,14 @@ struct KArgs {
   int nitems;
   int nt;
   float inv_h;
-  float L;
+  float L;      // box side in the bulk path's units (scaled by sigma)
   int bins;
   int nflat;
   int cshift;
+  double R;     // exact rules (mirror BinRules) for the in-kernel FP64 path
+  double Lx;    // box side, unscaled
+  int le;
+  int zero;
 };
 
 __device__ __forceinline__ int wave_fetch(int* counter, int lane) {
@@ -230,33 +213,110 @@ __device__ __forceinline__ int wave_fetch(int* counter, int lane) {
 }
 
 // FP32 bulk + FP64 edge re-evaluation.
-template <int ANI, int WEIGHTED>
-__global__ __launch_bounds__(kBlock) void k_pairs_mixed(
+//
+// Table codes (u8): iso -> the histogram row (bin, nflat = out, nflat+1 =
+// ambiguous); ani -> per component bin, kAniAmb8 / kAniOut8.  For both the
+// ambiguous row is the largest row index, so one max over four pairs tells
+// whether any of them needs the exact path.
+template <int ANI>
+__device__ __forceinline__ unsigned pair_row(const float4 q, float xs, float ys, float zs,
+                                             unsigned ntm1, const uint8_t* __restrict__ tab,
+                                             unsigned bins, unsigned out_row, unsigned amb_row) {
+  const float dx = q.x - xs;
+  const float dy = q.y - ys;
+  const float dz = q.z - zs;
+  if (!ANI) {
+    const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    const unsigned t = min(__float2uint_rz(d2), ntm1);
+    return tab[t];
+  } else {
+    const float dz2 = dz * dz;
+    const float do2 = __builtin_fmaf(dy, dy, dx * dx);
+    const unsigned tz = min(__float2uint_rz(dz2), ntm1);
+    const unsigned to = min(__float2uint_rz(do2), ntm1);
+    const unsigned cz = tab[tz];
+    const unsigned co = tab[to];
+    const unsigned m = max(cz, co);
+    const unsigned r = cz * bins + co;
+    return m < bins ? r : (m == kAniOut8 ? out_row : amb_row);
+  }
+}
+
+// Bin of one exact squared separation against thresholds held in LDS.
+__device__ __forceinline__ int class_lds(double d2, const double* T, int bins, double R, int le) {
+  if (le ? !(d2 <= R) : !(d2 < R)) return -1;
+  if (d2 >= T[bins]) return -1;
+  int lo = 0, hi = bins - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (T[mid] <= d2) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// The oracle's arithmetic for one (pair, image): FP64 from the unscaled FP32
+// inputs, no contraction.  Returns the histogram row (out_row if excluded).
+template <int ANI>
+__device__ __forceinline__ unsigned exact_row(const float4 pi, const float4 pj, int img,
+                                              const double* T, int bins, double R, int le,
+                                              int zero, double L, unsigned out_row) {
+  const int kx = img / 9 - 1, ky = (img / 3) % 3 - 1, kz = img % 3 - 1;
+  double tx, ty, tz;
+  if (!axis_sq(pi.x, pj.x, kx, L, &tx)) return out_row;
+  if (!axis_sq(pi.y, pj.y, ky, L, &ty)) return out_row;
+  if (!axis_sq(pi.z, pj.z, kz, L, &tz)) return out_row;
+  if (!ANI) {
+    const double d2 = __dadd_rn(__dadd_rn(tx, ty), tz);
+    if (zero == 1 && img == kImgZero && !(d2 > 0.0)) return out_row;
+    const int b = class_lds(d2, T, bins, R, le);
+    return b < 0 ? out_row : (unsigned)b;
+  }
+  const double do2 = __dadd_rn(tx, ty);
+  if (zero == 2 && !(tz > 0.0 && do2 > 0.0)) return out_row;
+  const int bz = class_lds(tz, T, bins, R, le);
+  if (bz < 0) return out_row;
+  const int bo = class_lds(do2, T, bins, R, le);
+  if (bo < 0) return out_row;
+  return (unsigned)(bz * bins + bo);
+}
+
+constexpr int kBlockMixed = kMixedBlock;
+constexpr int kMaxTable = kMixedTableMax;
+
+// xp/yp: points scaled by sigma (bulk arithmetic); xp_orig/yp_orig: the same
+// points unscaled (exact path, weights).
+template <int ANI, int WEIGHTED, int CS>
+__global__ __launch_bounds__(kBlockMixed, 4) void k_pairs_mixed(
     const float4* __restrict__ xp, const float4* __restrict__ yp,
+    const float4* __restrict__ xp_orig, const float4* __restrict__ yp_orig,
     const PairItem* __restrict__ items, int* __restrict__ counter,
-    const uint16_t* __restrict__ table, const BinRules* __restrict__ rules,
+    const uint8_t* __restrict__ table, const BinRules* __restrict__ rules,
     unsigned long long* __restrict__ hist, double* __restrict__ whist,
     unsigned long long* __restrict__ stats, KArgs k) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int tab_bytes = (k.nt * 2 + 15) & ~15;
+  __shared__ uint8_t tab[kMaxTable];
   const int nrows = k.nflat + 2;
-  const int ncopy = 1 << k.cshift;
-  uint16_t* tab = reinterpret_cast<uint16_t*>(smem);
-  uint32_t* hc = reinterpret_cast<uint32_t*>(smem + tab_bytes);
-  double* hw = reinterpret_cast<double*>(smem + tab_bytes + ((nrows * ncopy * 4 + 15) & ~15));
-  for (int t = threadIdx.x; t < k.nt; t += kBlock) tab[t] = table[t];
-  for (int t = threadIdx.x; t < nrows * ncopy; t += kBlock) {
+  constexpr int ncopy = 1 << CS;
+  uint32_t* hc = reinterpret_cast<uint32_t*>(smem);
+  const int hc_bytes = (nrows * ncopy * 4 + 15) & ~15;
+  double* sT = reinterpret_cast<double*>(smem + hc_bytes);
+  double* hw = sT + ((k.bins + 2) & ~1);
+  for (int t = threadIdx.x; t < k.nt; t += kBlockMixed) tab[t] = table[t];
+  for (int t = threadIdx.x; t < nrows * ncopy; t += kBlockMixed) {
     hc[t] = 0u;
     if (WEIGHTED) hw[t] = 0.0;
   }
+  for (int t = threadIdx.x; t <= k.bins; t += kBlockMixed) sT[t] = rules->T[t];
   __syncthreads();
 
   const int lane = threadIdx.x & (kWave - 1);
-  const int copy = lane & (ncopy - 1);
+  uint32_t* const hc_lane = hc + (lane & (ncopy - 1));
+  double* const hw_lane = hw + (lane & (ncopy - 1));
   const unsigned ntm1 = (unsigned)(k.nt - 1);
   const unsigned out_row = (unsigned)k.nflat;
   const unsigned amb_row = (unsigned)k.nflat + 1;
-  const float inv_h = k.inv_h;
+  const unsigned bins = (unsigned)k.bins;
   unsigned long long evals = 0;
   unsigned long long nre = 0;
 
@@ -272,61 +332,81 @@ __global__ __launch_bounds__(kBlock) void k_pairs_mixed(
     const bool masked = (code & kCodeMask) != 0;
     evals += (unsigned long long)ni * (unsigned long long)nj;
 
-    float4 pi;
-    if (lane < ni) pi = xp[i0 + lane];
-    else pi = make_float4(1e18f, 1e18f, 1e18f, 0.f);
-    // Image offset applied once to the i point: xi' = xi - k*L.
-    const float sx = (float)(img / 9 - 1) * k.L;
-    const float sy = (float)((img / 3) % 3 - 1) * k.L;
-    const float sz = (float)(img % 3 - 1) * k.L;
-    const float xs = pi.x - sx, ys = pi.y - sy, zs = pi.z - sz;
+    float4 ps, pi;
+    if (lane < ni) {
+      ps = xp[i0 + lane];
+      pi = xp_orig[i0 + lane];
+    } else {
+      ps = make_float4(1e18f, 1e18f, 1e18f, 0.f);
+      pi = ps;
+    }
+    // Image offset applied once to the (scaled) i point: xi' = xi - k*L.
+    const float xs = ps.x - (float)(img / 9 - 1) * k.L;
+    const float ys = ps.y - (float)((img / 3) % 3 - 1) * k.L;
+    const float zs = ps.z - (float)(img % 3 - 1) * k.L;
     const int my_i = i0 + lane;
     const double wi = (double)pi.w;
 
-    for (int j = j0; j < j0 + nj; ++j) {
+    auto deposit = [&](unsigned row, float wj) {
+      atomicAdd(hc_lane + (row << CS), 1u);
+      if (WEIGHTED) atomicAdd(hw_lane + (row << CS), wi * (double)wj);
+    };
+    auto fix = [&](unsigned& r, int j) {
+      // Lanes whose table cell straddles an edge: exact FP64 bin.
+      const unsigned long long amb = __ballot(r == amb_row);
+      if (amb) {
+        nre += __popcll(amb);
+        if (r == amb_row)
+          r = exact_row<ANI>(pi, yp_orig[j], img, sT, k.bins, k.R, k.le, k.zero, k.Lx, out_row);
+      }
+    };
+    auto quad = [&](int j, const float4 q0, const float4 q1, const float4 q2, const float4 q3) {
+      unsigned r0 = pair_row<ANI>(q0, xs, ys, zs, ntm1, tab, bins, out_row, amb_row);
+      unsigned r1 = pair_row<ANI>(q1, xs, ys, zs, ntm1, tab, bins, out_row, amb_row);
+      unsigned r2 = pair_row<ANI>(q2, xs, ys, zs, ntm1, tab, bins, out_row, amb_row);
+      unsigned r3 = pair_row<ANI>(q3, xs, ys, zs, ntm1, tab, bins, out_row, amb_row);
+      if (masked) {
+        r0 = (j <= my_i) ? out_row : r0;
+        r1 = (j + 1 <= my_i) ? out_row : r1;
+        r2 = (j + 2 <= my_i) ? out_row : r2;
+        r3 = (j + 3 <= my_i) ? out_row : r3;
+      }
+      const unsigned m = max(max(r0, r1), max(r2, r3));
+      if (__builtin_expect(__ballot(m == amb_row) != 0, 0)) {
+        fix(r0, j);
+        fix(r1, j + 1);
+        fix(r2, j + 2);
+        fix(r3, j + 3);
+      }
+      deposit(r0, q0.w);
+      deposit(r1, q1.w);
+      deposit(r2, q2.w);
+      deposit(r3, q3.w);
+    };
+
+    int j = j0;
+    const int jend = j0 + nj;
+    for (; j + 8 <= jend; j += 8) {
+      const float4 q0 = yp[j], q1 = yp[j + 1], q2 = yp[j + 2], q3 = yp[j + 3];
+      const float4 q4 = yp[j + 4], q5 = yp[j + 5], q6 = yp[j + 6], q7 = yp[j + 7];
+      quad(j, q0, q1, q2, q3);
+      quad(j + 4, q4, q5, q6, q7);
+    }
+    for (; j < jend; ++j) {
       const float4 q = yp[j];
-      const float dx = q.x - xs;
-      const float dy = q.y - ys;
-      const float dz = q.z - zs;
-      unsigned row;
-      if (!ANI) {
-        const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-        const unsigned t = min(__float2uint_rz(d2 * inv_h), ntm1);
-        row = tab[t];
-      } else {
-        const float dz2 = dz * dz;
-        const float do2 = __builtin_fmaf(dy, dy, dx * dx);
-        const unsigned tz = min(__float2uint_rz(dz2 * inv_h), ntm1);
-        const unsigned to = min(__float2uint_rz(do2 * inv_h), ntm1);
-        const unsigned cz = tab[tz];
-        const unsigned co = tab[to];
-        const unsigned m = max(cz, co);
-        row = (m < (unsigned)k.bins) ? cz * (unsigned)k.bins + co
-                                     : (m == kAniAmb ? amb_row : out_row);
-      }
-      if (masked && j <= my_i) row = out_row;
-      const bool amb = row == amb_row;
-      if (__builtin_expect(__ballot(amb) != 0, 0)) {
-        if (amb) {
-          const int b = recheck_pair(pi, q, img, rules);
-          ++nre;
-          if (b >= 0) {
-            atomicAdd(&hc[((unsigned)b << k.cshift) + copy], 1u);
-            if (WEIGHTED) atomicAdd(&hw[((unsigned)b << k.cshift) + copy], wi * (double)q.w);
-          }
-        }
-      }
-      atomicAdd(&hc[(row << k.cshift) + copy], 1u);
-      if (WEIGHTED) atomicAdd(&hw[(row << k.cshift) + copy], wi * (double)q.w);
+      unsigned r = pair_row<ANI>(q, xs, ys, zs, ntm1, tab, bins, out_row, amb_row);
+      if (masked) r = (j <= my_i) ? out_row : r;
+      fix(r, j);
+      deposit(r, q.w);
     }
   }
   __syncthreads();
-  for (int r = threadIdx.x; r < k.nflat; r += kBlock) {
+  for (int r = threadIdx.x; r < k.nflat; r += kBlockMixed) {
     unsigned long long c = 0;
     double w = 0.0;
     for (int cpy = 0; cpy < ncopy; ++cpy) {
-      c += hc[(r << k.cshift) + cpy];
-      if (WEIGHTED) w += hw[(r << k.cshift) + cpy];
+      c += hc[(r << CS) + cpy];
+      if (WEIGHTED) w += hw[(r << CS) + cpy];
     }
     if (c) {
       atomicAdd(&hist[r], c);
@@ -334,7 +414,17 @@ __global__ __launch_bounds__(kBlock) void k_pairs_mixed(
     }
   }
   if (lane == 0) atomicAdd(&stats[0], evals);
-  if (nre) atomicAdd(&stats[1], nre);
+  if (lane == 0 && nre) atomicAdd(&stats[1], nre);
+}
+
+// Scaled copy of a point array: x, y, z multiplied by sigma (a power of two,
+// so the scaling is exact), w kept.
+__global__ void k_scale_points(const float4* __restrict__ in, int64_t n, float sigma,
+                               float4* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = in[i];
+  out[i] = make_float4(p.x * sigma, p.y * sigma, p.z * sigma, p.w);
 }
 
 // Plain FP64 path: every pair formed in FP64, the range test and
@@ -447,6 +537,14 @@ hipError_t set_lds(K kern, size_t bytes) {
 
 }  // namespace
 
+hipError_t launch_scale_points(const float4* in, int64_t n, float sigma, float4* out,
+                               hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scale_points, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, n,
+                     sigma, out);
+  return hipGetLastError();
+}
+
 hipError_t build_pair_worklist(const WorkListArgs& a, DevBuf& counts, DevBuf& items, int* nitems,
                                hipStream_t s) {
   hipError_t e;
@@ -480,24 +578,42 @@ hipError_t launch_pairs(const PairLaunch& pl, hipStream_t s) {
   k.bins = pl.bins;
   k.nflat = pl.nflat;
   k.cshift = pl.cshift;
+  k.R = pl.R;
+  k.Lx = pl.Lx;
+  k.le = pl.le;
+  k.zero = pl.zero;
   const int ncopy = 1 << pl.cshift;
   dim3 grid(pl.nblocks), block(kBlock);
   hipError_t e;
   if (!pl.plain64) {
-    const size_t tab_bytes = (size_t)((pl.nt * 2 + 15) & ~15);
+    if (pl.nt > kMaxTable) return hipErrorInvalidValue;
     const size_t hc_bytes = (size_t)(((pl.nflat + 2) * ncopy * 4 + 15) & ~15);
+    const size_t t_bytes = (size_t)((pl.bins + 2) & ~1) * 8;
     const size_t hw_bytes = pl.weighted ? (size_t)(pl.nflat + 2) * ncopy * 8 : 0;
-    const size_t lds = tab_bytes + hc_bytes + hw_bytes;
-#define PCF_LAUNCH_MIXED(A, W)                                                                 \
+    const size_t lds = hc_bytes + t_bytes + hw_bytes;
+    const dim3 mblock(kBlockMixed);
+#define PCF_LAUNCH_MIXED(A, W, C)                                                              \
   do {                                                                                         \
-    if ((e = set_lds(k_pairs_mixed<A, W>, lds)) != hipSuccess) return e;                       \
-    hipLaunchKernelGGL((k_pairs_mixed<A, W>), grid, block, lds, s, pl.xp, pl.yp, pl.items,     \
-                       pl.counter, pl.table, pl.rules, pl.hist, pl.whist, pl.stats, k);        \
+    if ((e = set_lds(k_pairs_mixed<A, W, C>, lds)) != hipSuccess) return e;                    \
+    hipLaunchKernelGGL((k_pairs_mixed<A, W, C>), grid, mblock, lds, s, pl.xs, pl.ys, pl.xp,    \
+                       pl.yp, pl.items, pl.counter, pl.table, pl.rules, pl.hist, pl.whist,     \
+                       pl.stats, k);                                                           \
   } while (0)
-    if (!pl.ani && !pl.weighted) PCF_LAUNCH_MIXED(0, 0);
-    else if (!pl.ani && pl.weighted) PCF_LAUNCH_MIXED(0, 1);
-    else if (pl.ani && !pl.weighted) PCF_LAUNCH_MIXED(1, 0);
-    else PCF_LAUNCH_MIXED(1, 1);
+#define PCF_LAUNCH_CS(A, W)                                   \
+  switch (pl.cshift) {                                        \
+    case 6: PCF_LAUNCH_MIXED(A, W, 6); break;                 \
+    case 5: PCF_LAUNCH_MIXED(A, W, 5); break;                 \
+    case 4: PCF_LAUNCH_MIXED(A, W, 4); break;                 \
+    case 3: PCF_LAUNCH_MIXED(A, W, 3); break;                 \
+    case 2: PCF_LAUNCH_MIXED(A, W, 2); break;                 \
+    case 1: PCF_LAUNCH_MIXED(A, W, 1); break;                 \
+    default: PCF_LAUNCH_MIXED(A, W, 0); break;                \
+  }
+    if (!pl.ani && !pl.weighted) { PCF_LAUNCH_CS(0, 0) }
+    else if (!pl.ani && pl.weighted) { PCF_LAUNCH_CS(0, 1) }
+    else if (pl.ani && !pl.weighted) { PCF_LAUNCH_CS(1, 0) }
+    else { PCF_LAUNCH_CS(1, 1) }
+#undef PCF_LAUNCH_CS
 #undef PCF_LAUNCH_MIXED
   } else {
     const size_t hc_bytes = (size_t)(((pl.nflat + 1) * ncopy * 4 + 15) & ~15);

@@ -25,7 +25,7 @@ struct pcf_ctx {
   std::string err;
   std::vector<std::unique_ptr<pcf::Catalog>> cats;
   pcf::DevBuf scratch_cells, scratch_perm, wl_counts, wl_items, table_buf, rules_buf, hist_buf,
-      misc_buf, stats_partials;
+      misc_buf, stats_partials, scaled_x, scaled_y;
   pcf_stats last{};
   pcf::JobEngine* engine = nullptr;  // asynchronous serving engine (pcf_jobs.hip)
 };
@@ -61,7 +61,8 @@ inline Catalog* get_cat(pcf_ctx* c, pcf_catalog id) {
 // Defined in abi.hip.
 double ds_floored(int bins, float dmax);
 void bin_thresholds(int bins, double ds, double* T);
-int build_table(const BinRules& r, double A, int nt, std::vector<uint16_t>* table, float* inv_h);
+int build_table(const BinRules& r, double A, int nt_max, std::vector<uint8_t>* table, float* inv_h,
+                int* nt_out);
 int choose_geometry(const Catalog* X, const Catalog* Y, float dmax, int bpc, float box,
                     int cells_per_rmax, GridGeom* g, int K[3], float* reach_out, int* m_out);
 DevGrid* get_grid(pcf_ctx* c, Catalog* cat, const GridGeom& g, bool force_rebuild, int* rc);

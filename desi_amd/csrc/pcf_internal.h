@@ -61,9 +61,10 @@ struct BinRules {
   double T[kMaxBinsIso + 1];  // T[k] = min d2 with (int)(sqrt(d2)*ds) >= k, k=1..bins
 };
 
-// Table codes for the FP32 bulk path.
-constexpr uint16_t kAniAmb = 0x4000;
-constexpr uint16_t kAniOut = 0x8000;
+// Table codes of the FP32 bulk path for 2ani (per component; iso tables hold
+// the histogram row directly).
+constexpr uint8_t kAniAmb8 = 254;
+constexpr uint8_t kAniOut8 = 255;
 
 struct DevGrid {
   GridGeom g;
@@ -115,17 +116,19 @@ hipError_t build_grid(const float4* raw, int64_t n, const GridGeom& g, DevGrid* 
 hipError_t build_grid64(DevGrid* grid, hipStream_t s);
 
 struct PairLaunch {
-  const float4* xp;
-  const float4* yp;
+  const float4* xp;   // i side, original coordinates
+  const float4* yp;   // j side, original coordinates
+  const float4* xs;   // i side scaled by sigma (bulk FP32 path)
+  const float4* ys;   // j side scaled by sigma
   const double4* xp64;
   const double4* yp64;
   const PairItem* items;
   int nitems;
   int* counter;
-  const uint16_t* table;
+  const uint8_t* table;
   int nt;
   float inv_h;
-  float L;
+  float L;        // box side in the bulk path's (scaled) units
   int bins;
   int nflat;      // flat histogram bins (bins or bins^2)
   int cshift;     // log2 of histogram copies per workgroup
@@ -137,8 +140,16 @@ struct PairLaunch {
   int weighted;
   int plain64;
   int nblocks;
+  double R;     // exact range limit on d^2
+  double Lx;    // box side, unscaled
+  int le;
+  int zero;
 };
+constexpr int kMixedBlock = 512;       // threads per workgroup of the mixed kernel
+constexpr int kMixedTableMax = 28672;  // static LDS bytes of its d2 -> bin table
 hipError_t launch_pairs(const PairLaunch& pl, hipStream_t s);
+hipError_t launch_scale_points(const float4* in, int64_t n, float sigma, float4* out,
+                               hipStream_t s);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
 
 struct WorkListArgs {
