@@ -22,6 +22,7 @@
 // where that applies.
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -426,6 +427,314 @@ void pcf_oracle_analytic_rr(int bins, int64_t np, float dmax, float box, double*
     int64_t k = 3 * (int64_t)i * i + 3 * (int64_t)i + 1;
     rr[i] = alpha * (double)k;
   }
+}
+
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Triplets (3iso, 3ani)
+// ---------------------------------------------------------------------------
+namespace {
+
+struct TripRule {
+  int ani, bpc, auto_, bins;
+  double R, ds, L, ds_th;
+};
+
+// Periodic per-axis term of the reference (cuda/kernels/3isoBPC_k.cuh:56-75):
+// |delta| - L*f, where f says whether the axis is projected.  Squared by the
+// caller.  For the 2-3 side f = |f2 - f3|.
+inline double proj(double d, double L, int f) { return std::fabs(d) - L * (double)f; }
+
+// One ordered triplet (i, j, k) seen through projection flags f2 (node 2) and
+// f3 (node 3), each a 3-bit set of axes.  Node 1 is never projected
+// (cuda/kernels/3isoBPC_k.cuh:56).  Non-periodic: f2 = f3 = 0 and the terms
+// are the plain differences (cuda/kernels/3iso_k.cuh:57-70).  Returns the flat
+// bin or -1.  Iso layout [b(d12)][b(d23)][b(d31)] (3iso_k.cuh:71-78); ani
+// layout [b(d12)][b(d31)][b(d23)][t][p] (3ani_k.cuh:75-92).
+inline int64_t triplet_bin(const Pt& p1, const Pt& p2, const Pt& p3, int f2, int f3,
+                           const TripRule& r) {
+  const double L = r.L;
+  double t12[3], t23[3], t31[3];
+  const float a1[3] = {p1.x, p1.y, p1.z}, a2[3] = {p2.x, p2.y, p2.z}, a3[3] = {p3.x, p3.y, p3.z};
+  for (int a = 0; a < 3; ++a) {
+    const double d12 = (double)a2[a] - (double)a1[a];
+    const double d23 = (double)a3[a] - (double)a2[a];
+    const double d31 = (double)a3[a] - (double)a1[a];
+    if (r.bpc) {
+      const int g2 = (f2 >> a) & 1, g3 = (f3 >> a) & 1;
+      t12[a] = proj(d12, L, g2);
+      t23[a] = proj(d23, L, g2 != g3);
+      t31[a] = proj(d31, L, g3);
+    } else {
+      t12[a] = d12;
+      t23[a] = d23;
+      t31[a] = d31;
+    }
+  }
+  const double s12 = t12[0] * t12[0] + t12[1] * t12[1] + t12[2] * t12[2];
+  if (!(s12 < r.R && s12 > 0)) return -1;
+  const double s23 = t23[0] * t23[0] + t23[1] * t23[1] + t23[2] * t23[2];
+  const double s31 = t31[0] * t31[0] + t31[1] * t31[1] + t31[2] * t31[2];
+  if (r.auto_) {
+    // XXX: all three sides > 0 (3iso_k.cuh:67,70)
+    if (!(s23 < r.R && s23 > 0 && s31 < r.R && s31 > 0)) return -1;
+  } else {
+    // XXY: only d12 > 0 (3iso_k.cuh:146,153,155)
+    if (!(s23 < r.R && s31 < r.R)) return -1;
+  }
+  const double d12 = std::sqrt(s12), d23 = std::sqrt(s23), d31 = std::sqrt(s31);
+  const int64_t bn = r.bins;
+  const int a = (int)(d12 * r.ds);
+  if (a > bn - 1) return -1;
+  const int b23 = (int)(d23 * r.ds);
+  if (b23 > bn - 1) return -1;
+  const int b31 = (int)(d31 * r.ds);
+  if (b31 > bn - 1) return -1;
+  if (!r.ani) return ((int64_t)a * bn + b23) * bn + b31;
+  // cos of the 1-2 and 1-3 directions to the line of sight z
+  // (3ani_k.cuh:75-76; periodic: signed by the projected z term,
+  // 3aniBPC_k.cuh:57,73).
+  const double cth12 = 1 + t12[2] / d12;
+  const double cth31 = 1 + t31[2] / d31;
+  const int t = (int)(cth12 * r.ds_th);
+  if (t > bn - 1) return -1;
+  const int p = (int)(cth31 * r.ds_th);
+  if (p > bn - 1) return -1;
+  return (((((int64_t)a * bn + b31) * bn + b23) * bn + t) * bn) + p;
+}
+
+// The projection of node 2 (or 3) relative to node 1 that the image k
+// corresponds to, or -1 if k is the far image (the reference projects an axis
+// only toward the close image: |delta| - L).  For delta == 0 the two images
+// are one term, kept for k = -1.
+inline int flags_of_image(const Pt& p1, const Pt& p2, const int k[3]) {
+  int f = 0;
+  const float a1[3] = {p1.x, p1.y, p1.z}, a2[3] = {p2.x, p2.y, p2.z};
+  for (int a = 0; a < 3; ++a) {
+    if (k[a] == 0) continue;
+    const double d = (double)a2[a] - (double)a1[a];
+    if ((k[a] > 0 && d >= 0.0) || (k[a] < 0 && d < 0.0)) return -1;
+    f |= 1 << a;
+  }
+  return f;
+}
+
+TripRule make_trip_rule(int ani, int bpc, int auto_, int bins, float dmax, float box) {
+  TripRule r;
+  r.ani = ani;
+  r.bpc = bpc;
+  r.auto_ = auto_;
+  r.bins = bins;
+  float dd_max = dmax * dmax;
+  r.R = (double)dd_max;
+  r.ds = ds_floored(bins, dmax);  // 3iso_k.cuh:44, 3isoBPC_k.cuh:218, 3ani_k.cuh:45
+  r.L = (double)box;
+  float ds_th = (float)bins / 2;  // 3ani_k.cuh:42
+  r.ds_th = (double)ds_th;
+  return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Ordered-triplet histogram in the FP64 canonical mode (GPU layout).
+//   auto_ = 1: XXX over a (DDD/RRR).  auto_ = 0: XXY with points 1, 2 from a
+//   and point 3 from b (DDR with a = D, b = R; DRR with a = R, b = D,
+//   cuda/pcf3iso.cuh:111-114).
+// Periodic: every image combination of nodes 2 and 3 relative to node 1 that
+// lies within range (the reference's 64 projection combinations,
+// cuda/kernels/3isoBPC_k.cuh:191-938).  Weighted sums use w1*w2*w3 formed in
+// double per triplet (the reference multiplies cumulatively across the k loop,
+// cuda/kernels/3iso_k.cuh:79 -- not reproduced).
+// Pinned by: brute force over all 64 projection combinations
+// (pcf_oracle_triplets_brute), the cpp32 comparison with the compiled
+// reference CPU program, the reference's results/*.example, permutation
+// symmetry, and the 3iso marginal of 3ani.
+int pcf_oracle_triplets(const float* a_, int64_t na, const float* b_, int64_t nb, int auto_,
+                        int ani, int bpc, float box, int bins, float dmax, int nthreads,
+                        uint64_t* counts, double* wsum) {
+  if (bins <= 0 || !(dmax > 0)) return -1;
+  if (bpc && !(2.0 * dmax < box)) return -2;
+  const Pt* a = reinterpret_cast<const Pt*>(a_);
+  const Pt* b = auto_ ? a : reinterpret_cast<const Pt*>(b_);
+  if (auto_) nb = na;
+  TripRule r = make_trip_rule(ani, bpc, auto_, bins, dmax, box);
+  const int64_t bn = bins;
+  const int64_t nout = ani ? bn * bn * bn * bn * bn : bn * bn * bn;
+  std::memset(counts, 0, sizeof(uint64_t) * nout);
+  std::memset(wsum, 0, sizeof(double) * nout);
+  if (na == 0 || nb == 0) return 0;
+  Grid g;
+  grid_geometry(a, na, auto_ ? nullptr : b, nb, (double)dmax * 1.0000001 + 1e-6, bpc,
+                (double)box, &g);
+  Grid gb = g;
+  g.build(a, na);
+  gb.build(b, nb);
+  const int nt = nthreads_or(nthreads);
+  const int64_t ncell = (int64_t)g.P[0] * g.P[1] * g.P[2];
+  // Thread-private histograms are large for 3ani; use atomics into shared ones.
+  std::vector<uint64_t> C(nout, 0);
+  std::vector<double> W(nout, 0.0);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+  for (int64_t ca = 0; ca < ncell; ++ca) {
+    int cx = (int)(ca / ((int64_t)g.P[1] * g.P[2]));
+    int cy = (int)((ca / g.P[2]) % g.P[1]);
+    int cz = (int)(ca % g.P[2]);
+    std::vector<std::pair<int64_t, std::array<int, 3>>> nbrs;
+    for_neighbours(g, bpc, cx, cy, cz, [&](int64_t cb, int kx, int ky, int kz) {
+      nbrs.push_back({cb, {kx, ky, kz}});
+    });
+    for (int64_t i = g.start[ca]; i < g.start[ca + 1]; ++i) {
+      const Pt& p1 = g.pts[i];
+      for (auto& n2 : nbrs) {
+        for (int64_t j = g.start[n2.first]; j < g.start[n2.first + 1]; ++j) {
+          const Pt& p2 = g.pts[j];
+          const int f2 = flags_of_image(p1, p2, n2.second.data());
+          if (f2 < 0) continue;
+          for (auto& n3 : nbrs) {
+            for (int64_t k = gb.start[n3.first]; k < gb.start[n3.first + 1]; ++k) {
+              const Pt& p3 = gb.pts[k];
+              const int f3 = flags_of_image(p1, p3, n3.second.data());
+              if (f3 < 0) continue;
+              const int64_t bin = triplet_bin(p1, p2, p3, f2, f3, r);
+              if (bin < 0) continue;
+              const double w = (double)p1.w * (double)p2.w * (double)p3.w;
+#pragma omp atomic
+              C[bin] += 1;
+#pragma omp atomic
+              W[bin] += w;
+            }
+          }
+        }
+      }
+    }
+  }
+  for (int64_t k = 0; k < nout; ++k) {
+    counts[k] = C[k];
+    wsum[k] = W[k];
+  }
+  return 0;
+}
+
+// Literal restatement for tiny inputs: every ordered triplet and, periodic,
+// every one of the 64 (f2, f3) projection combinations of
+// cuda/kernels/3isoBPC_k.cuh:191-938 with the per-axis terms of count123
+// (3isoBPC_k.cuh:56-75).  O(n^3 * 64).  Pins pcf_oracle_triplets.
+int pcf_oracle_triplets_brute(const float* a_, int64_t na, const float* b_, int64_t nb,
+                              int auto_, int ani, int bpc, float box, int bins, float dmax,
+                              int nthreads, uint64_t* counts, double* wsum) {
+  (void)nthreads;
+  const Pt* a = reinterpret_cast<const Pt*>(a_);
+  const Pt* b = auto_ ? a : reinterpret_cast<const Pt*>(b_);
+  if (auto_) nb = na;
+  TripRule r = make_trip_rule(ani, bpc, auto_, bins, dmax, box);
+  const int64_t bn = bins;
+  const int64_t nout = ani ? bn * bn * bn * bn * bn : bn * bn * bn;
+  std::memset(counts, 0, sizeof(uint64_t) * nout);
+  std::memset(wsum, 0, sizeof(double) * nout);
+  const int ncomb = bpc ? 8 : 1;
+  for (int64_t i = 0; i < na; ++i)
+    for (int64_t j = 0; j < na; ++j)
+      for (int64_t k = 0; k < nb; ++k)
+        for (int f2 = 0; f2 < ncomb; ++f2)
+          for (int f3 = 0; f3 < ncomb; ++f3) {
+            const int64_t bin = triplet_bin(a[i], a[j], b[k], f2, f3, r);
+            if (bin < 0) continue;
+            counts[bin] += 1;
+            wsum[bin] += (double)a[i].w * (double)a[j].w * (double)b[k].w;
+          }
+  return 0;
+}
+
+// cpp32 mode of cpp/3PCFisotropic/normal (3PCFiso.h): unique triangles with
+// FP32 differences and squares (no contraction) converted to double, only
+// d < dd_max tested (no d > 0 test, 3PCFiso.h:534-550), deposit at
+// [(int)(d12 ds)][(int)(d13 ds)][(int)(d23 ds)]; XXX deposits w1*w2*w3
+// (FP32 product), XXY deposits w1*w2*w3/3 (FP32, 3PCFiso.h:949,1011) for each
+// unordered X pair and Y point; then symmetrize (3PCFiso.h:705-727): the sum
+// of the six permutations, rounded to FP32, stored in all six.
+// Output: the program's histogram (double), bins^3 values.
+int pcf_oracle_cpp32_3iso(const float* a_, int64_t na, const float* b_, int64_t nb, int auto_,
+                          int bins, float dmax, int nthreads, double* hist) {
+  const Pt* a = reinterpret_cast<const Pt*>(a_);
+  const Pt* b = auto_ ? a : reinterpret_cast<const Pt*>(b_);
+  if (auto_) nb = na;
+  const float dd_max = dmax * dmax;
+  const double ds = ds_floored(bins, dmax);
+  const int64_t bn = bins;
+  std::vector<double> U(bn * bn * bn, 0.0);
+  Grid g;
+  grid_geometry(a, na, auto_ ? nullptr : b, nb, (double)dmax * 1.0000001 + 1e-6, 0, 0.0, &g);
+  Grid gb = g;
+  g.build(a, na);
+  gb.build(b, nb);
+  const int nt = nthreads_or(nthreads);
+  const int64_t ncell = (int64_t)g.P[0] * g.P[1] * g.P[2];
+  auto fdist = [](const Pt& p, const Pt& q) {
+    float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+    float d = dx * dx + dy * dy + dz * dz;
+    return (double)d;
+  };
+  int err = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+  for (int64_t ca = 0; ca < ncell; ++ca) {
+    int cx = (int)(ca / ((int64_t)g.P[1] * g.P[2]));
+    int cy = (int)((ca / g.P[2]) % g.P[1]);
+    int cz = (int)(ca % g.P[2]);
+    std::vector<int64_t> nb_cells;
+    for_neighbours(g, 0, cx, cy, cz, [&](int64_t cb, int, int, int) { nb_cells.push_back(cb); });
+    std::vector<double> Uloc;
+    std::vector<std::pair<int64_t, double>> dep;
+    for (int64_t ii = g.start[ca]; ii < g.start[ca + 1]; ++ii) {
+      // point i is the one with the smallest sorted index of its triangle
+      const Pt& p1 = g.pts[ii];
+      for (int64_t cj : nb_cells)
+        for (int64_t jj = g.start[cj]; jj < g.start[cj + 1]; ++jj) {
+          if (auto_ && jj <= ii) continue;
+          const Pt& p2 = g.pts[jj];
+          double d12 = fdist(p1, p2);
+          if (!(d12 < dd_max)) continue;
+          if (!auto_ && jj <= ii) continue;  // unordered X pair
+          for (int64_t ck : nb_cells)
+            for (int64_t kk = gb.start[ck]; kk < gb.start[ck + 1]; ++kk) {
+              if (auto_ && kk <= jj) continue;
+              const Pt& p3 = gb.pts[kk];
+              double d13 = fdist(p1, p3);
+              if (!(d13 < dd_max)) continue;
+              double d23 = fdist(p2, p3);
+              if (!(d23 < dd_max)) continue;
+              int x = (int)(std::sqrt(d12) * ds), y = (int)(std::sqrt(d13) * ds),
+                  z = (int)(std::sqrt(d23) * ds);
+              if (x >= bins || y >= bins || z >= bins) { err = 1; continue; }
+              float w = p1.w * p2.w * p3.w;
+              if (!auto_) w = w / 3;
+              dep.push_back({(x * bn + y) * bn + z, (double)w});
+            }
+        }
+    }
+    if (!dep.empty()) {
+#pragma omp critical
+      for (auto& d : dep) U[d.first] += d.second;
+    }
+  }
+  // symmetrize in FP32 (3PCFiso.h:705-727)
+  auto at = [&](int64_t i, int64_t j, int64_t k) -> double& { return U[(i * bn + j) * bn + k]; };
+  for (int64_t i = 0; i < bn; i++)
+    for (int64_t j = i; j < bn; j++)
+      for (int64_t k = j; k < bn; k++) {
+        float elem = at(i, j, k) + at(k, i, j) + at(j, k, i) + at(j, i, k) + at(k, j, i) + at(i, k, j);
+        at(i, j, k) = elem;
+        at(k, i, j) = elem;
+        at(j, k, i) = elem;
+        at(j, i, k) = elem;
+        at(k, j, i) = elem;
+        at(i, k, j) = elem;
+      }
+  std::memcpy(hist, U.data(), sizeof(double) * bn * bn * bn);
+  return err ? -3 : 0;
 }
 
 }  // extern "C"

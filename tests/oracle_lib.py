@@ -33,6 +33,8 @@ def lib():
                                             _i32, _f64p]
         L.pcf_oracle_analytic_rr.argtypes = [_i32, _i64, ctypes.c_float, ctypes.c_float, _f64p]
         L.pcf_oracle_analytic_rr.restype = None
+        L.pcf_oracle_cpp32_3iso.argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32, ctypes.c_float,
+                                            _i32, _f64p]
         for name in ("pcf_oracle_triplets", "pcf_oracle_triplets_brute"):
             if hasattr(L, name):
                 getattr(L, name).argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32, _i32,
@@ -84,6 +86,19 @@ def cpp32_2iso(a, b=None, *, bins=20, dmax=150.0, threads=0):
                                      dmax, threads, _p(h, _f64p))
     if rc != 0:
         raise ValueError(f"oracle cpp32 rc={rc}")
+    return h
+
+
+def cpp32_3iso(a, b=None, *, bins=30, dmax=50.0, threads=0):
+    """The CPU program's 3iso histogram (after its FP32 symmetrisation)."""
+    a = _cat(a)
+    auto = b is None
+    b = a if auto else _cat(b)
+    h = np.zeros(bins ** 3, np.float64)
+    rc = lib().pcf_oracle_cpp32_3iso(_p(a, _f32p), len(a), _p(b, _f32p), len(b), int(auto),
+                                     bins, dmax, threads, _p(h, _f64p))
+    if rc != 0:
+        raise ValueError(f"oracle cpp32 3iso rc={rc}")
     return h
 
 
