@@ -398,6 +398,7 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   wa.reach = reach;
   wa.auto_ = auto_ ? 1 : 0;
   wa.cylinder = ani ? 1 : 0;
+  wa.map = nullptr;
   int nitems = 0;
   HIPCHK(c, build_pair_worklist(wa, c->wl_counts, c->wl_items, &nitems, c->stream));
 
@@ -564,7 +565,8 @@ void pcf_destroy(pcf_ctx* c) {
     if (cat) free_catalog(cat.get());
   for (DevBuf* b : {&c->scratch_cells, &c->scratch_perm, &c->wl_counts, &c->wl_items,
                     &c->table_buf, &c->rules_buf, &c->hist_buf, &c->misc_buf, &c->stats_partials,
-                    &c->scaled_x, &c->scaled_y})
+                    &c->scaled_x, &c->scaled_y, &c->trip_idx, &c->trip_pairs, &c->trip_hist,
+                    &c->wl_items2})
     b->release();
   for (auto& ev : c->ev)
     if (ev) (void)hipEventDestroy(ev);

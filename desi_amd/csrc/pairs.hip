@@ -24,112 +24,12 @@
 //     lanes of a wave), merged into the global uint64 / FP64 histogram once per
 //     workgroup of a persistent grid that pulls items from an atomic counter.
 #include "pcf_internal.h"
+#include "worklist.h"
+#include "exact.h"
 
 namespace pcf {
 
 namespace {
-
-// ---------------------------------------------------------------------------
-// Work list
-// ---------------------------------------------------------------------------
-// Tiles are 64 consecutive points of one (cx,cy) column (they may span a few
-// z-cells), so every lane of a wave holds a point; a tile's j-range in a
-// neighbour column covers the z-cells within reach of any of its cells.  The
-// thread of cell c emits the tiles whose first point lies in c.
-template <bool FILL>
-__device__ int enumerate_cell(const WorkListArgs& a, int cell, PairItem* out) {
-  const int ib = a.x_cell_start[cell];
-  const int ie = a.x_cell_start[cell + 1];
-  if (ie <= ib) return 0;
-  const GridGeom& g = a.g;
-  const int Pz = g.P[2], Py = g.P[1], Px = g.P[0];
-  const int cz = cell % Pz;
-  const int cy = (cell / Pz) % Py;
-  const int cx = cell / (Pz * Py);
-  const int colbase = cell - cz;
-  const int col_start = a.x_cell_start[colbase];
-  const int col_end = a.x_cell_start[colbase + Pz];
-  const float reach2 = a.reach * a.reach;
-  int count = 0;
-  // First tile boundary at or after ib (tiles start at col_start + 64k).
-  int i0 = col_start + ((ib - col_start + kTile - 1) / kTile) * kTile;
-  for (; i0 < ie; i0 += kTile) {
-    const int ni = min(kTile, col_end - i0);
-    // z-cell of the tile's last point.
-    int zb = cz;
-    while (zb + 1 < Pz && a.x_cell_start[colbase + zb + 1] <= i0 + ni - 1) ++zb;
-    for (int dx = -a.K[0]; dx <= a.K[0]; ++dx) {
-      int ccx = cx + dx, kx = 0;
-      if (ccx < 0) {
-        if (!g.bpc) continue;
-        ccx += Px;
-        kx = -1;
-      } else if (ccx >= Px) {
-        if (!g.bpc) continue;
-        ccx -= Px;
-        kx = 1;
-      }
-      const float gx = (float)max(0, abs(dx) - 1) * g.cs[0];
-      for (int dy = -a.K[1]; dy <= a.K[1]; ++dy) {
-        int ccy = cy + dy, ky = 0;
-        if (ccy < 0) {
-          if (!g.bpc) continue;
-          ccy += Py;
-          ky = -1;
-        } else if (ccy >= Py) {
-          if (!g.bpc) continue;
-          ccy -= Py;
-          ky = 1;
-        }
-        const float gy = (float)max(0, abs(dy) - 1) * g.cs[1];
-        const float rho2 = gx * gx + gy * gy;
-        if (rho2 >= reach2) continue;
-        // Sphere: the z reach shrinks with the lateral gap.  Cylinder (2ani):
-        // the z and perpendicular ranges are independent.
-        const float hz = a.cylinder ? a.reach : sqrtf(reach2 - rho2);
-        const int Kz = min(a.K[2], (int)ceilf(hz * g.inv_cs[2]));
-        const int col = (ccx * Py + ccy) * Pz;
-        const int zlo = cz - Kz, zhi = zb + Kz;
-        // Up to three z segments: wrapped below (kz=-1), inside, wrapped above.
-        // zlo > -Pz and zhi < 2*Pz because Kz < Pz.
-        int seg_lo[3], seg_hi[3], seg_k[3], nseg = 0;
-        if (g.bpc) {
-          if (zlo < 0) { seg_lo[nseg] = zlo + Pz; seg_hi[nseg] = Pz - 1; seg_k[nseg++] = -1; }
-          seg_lo[nseg] = max(zlo, 0); seg_hi[nseg] = min(zhi, Pz - 1); seg_k[nseg++] = 0;
-          if (zhi > Pz - 1) { seg_lo[nseg] = 0; seg_hi[nseg] = zhi - Pz; seg_k[nseg++] = 1; }
-        } else {
-          seg_lo[0] = max(zlo, 0); seg_hi[0] = min(zhi, Pz - 1); seg_k[0] = 0; nseg = 1;
-        }
-        for (int s = 0; s < nseg; ++s) {
-          if (seg_hi[s] < seg_lo[s]) continue;
-          int j0 = a.y_cell_start[col + seg_lo[s]];
-          int j1 = a.y_cell_start[col + seg_hi[s] + 1];
-          if (j1 <= j0) continue;
-          int flag = 0;
-          if (a.auto_) {
-            // Count each unordered pair once: j > i in cell order.
-            if (j1 <= i0 + 1) continue;
-            if (j0 < i0 + ni) {
-              j0 = max(j0, i0 + 1);
-              flag = kCodeMask;
-            }
-          }
-          if (FILL) {
-            PairItem it;
-            it.i0 = i0;
-            it.ni = ni;
-            it.j0 = j0;
-            it.nj = j1 - j0;
-            it.code = ((kx + 1) * 9 + (ky + 1) * 3 + (seg_k[s] + 1)) | flag;
-            out[count] = it;
-          }
-          ++count;
-        }
-      }
-    }
-  }
-  return count;
-}
 
 __global__ void k_worklist_count(WorkListArgs a, int* __restrict__ counts) {
   const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -170,23 +70,6 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_items(const int* counts, in
     run += c;
   }
   if (t == kScanBlock - 1) out[m] = (int)part[t];
-}
-
-// ---------------------------------------------------------------------------
-// Exact FP64 re-evaluation (the oracle's arithmetic, no contraction).
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ bool axis_sq(float ai, float aj, int k, double L, double* out) {
-  double d = __dsub_rn((double)aj, (double)ai);
-  if (k == 0) {
-    *out = __dmul_rn(d, d);
-    return true;
-  }
-  // Only the image that brings the points closer is a reference term
-  // (size_box - |delta|, cuda/kernels/2isoBPC_k.cuh:87).
-  if ((k > 0 && d >= 0.0) || (k < 0 && d < 0.0)) return false;
-  double t = __dsub_rn(L, fabs(d));
-  *out = __dmul_rn(t, t);
-  return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -240,19 +123,6 @@ __device__ __forceinline__ unsigned pair_row(const float4 q, float xs, float ys,
     const unsigned r = cz * bins + co;
     return m < bins ? r : (m == kAniOut8 ? out_row : amb_row);
   }
-}
-
-// Bin of one exact squared separation against thresholds held in LDS.
-__device__ __forceinline__ int class_lds(double d2, const double* T, int bins, double R, int le) {
-  if (le ? !(d2 <= R) : !(d2 < R)) return -1;
-  if (d2 >= T[bins]) return -1;
-  int lo = 0, hi = bins - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (T[mid] <= d2) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
 }
 
 // The oracle's arithmetic for one (pair, image): FP64 from the unscaled FP32

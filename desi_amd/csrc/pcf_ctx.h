@@ -25,7 +25,7 @@ struct pcf_ctx {
   std::string err;
   std::vector<std::unique_ptr<pcf::Catalog>> cats;
   pcf::DevBuf scratch_cells, scratch_perm, wl_counts, wl_items, table_buf, rules_buf, hist_buf,
-      misc_buf, stats_partials, scaled_x, scaled_y;
+      misc_buf, stats_partials, scaled_x, scaled_y, trip_idx, trip_pairs, trip_hist, wl_items2;
   pcf_stats last{};
   pcf::JobEngine* engine = nullptr;  // asynchronous serving engine (pcf_jobs.hip)
 };

@@ -160,6 +160,7 @@ struct WorkListArgs {
   float reach;              // rmax plus slack
   int auto_;
   int cylinder;             // 2ani: |dz| < rmax and d_perp < rmax (not a sphere)
+  const int* map;           // optional: j index -> position (pair-list offsets)
 };
 // Two-pass (count, scan, fill) work list on the device.  Returns the item
 // count through *nitems (a synchronous read of one integer).

@@ -1,0 +1,116 @@
+// worklist.h -- device enumeration of the flattened cell-pair work list,
+// shared by the pair and triplet kernels.
+#pragma once
+#include "pcf_internal.h"
+
+namespace pcf {
+
+// ---------------------------------------------------------------------------
+// Work list
+// ---------------------------------------------------------------------------
+// Tiles are 64 consecutive points of one (cx,cy) column (they may span a few
+// z-cells), so every lane of a wave holds a point; a tile's j-range in a
+// neighbour column covers the z-cells within reach of any of its cells.  The
+// thread of cell c emits the tiles whose first point lies in c.
+template <bool FILL>
+__device__ inline int enumerate_cell(const WorkListArgs& a, int cell, PairItem* out) {
+  const int ib = a.x_cell_start[cell];
+  const int ie = a.x_cell_start[cell + 1];
+  if (ie <= ib) return 0;
+  const GridGeom& g = a.g;
+  const int Pz = g.P[2], Py = g.P[1], Px = g.P[0];
+  const int cz = cell % Pz;
+  const int cy = (cell / Pz) % Py;
+  const int cx = cell / (Pz * Py);
+  const int colbase = cell - cz;
+  const int col_start = a.x_cell_start[colbase];
+  const int col_end = a.x_cell_start[colbase + Pz];
+  const float reach2 = a.reach * a.reach;
+  int count = 0;
+  // First tile boundary at or after ib (tiles start at col_start + 64k).
+  int i0 = col_start + ((ib - col_start + kTile - 1) / kTile) * kTile;
+  for (; i0 < ie; i0 += kTile) {
+    const int ni = min(kTile, col_end - i0);
+    // z-cell of the tile's last point.
+    int zb = cz;
+    while (zb + 1 < Pz && a.x_cell_start[colbase + zb + 1] <= i0 + ni - 1) ++zb;
+    for (int dx = -a.K[0]; dx <= a.K[0]; ++dx) {
+      int ccx = cx + dx, kx = 0;
+      if (ccx < 0) {
+        if (!g.bpc) continue;
+        ccx += Px;
+        kx = -1;
+      } else if (ccx >= Px) {
+        if (!g.bpc) continue;
+        ccx -= Px;
+        kx = 1;
+      }
+      const float gx = (float)max(0, abs(dx) - 1) * g.cs[0];
+      for (int dy = -a.K[1]; dy <= a.K[1]; ++dy) {
+        int ccy = cy + dy, ky = 0;
+        if (ccy < 0) {
+          if (!g.bpc) continue;
+          ccy += Py;
+          ky = -1;
+        } else if (ccy >= Py) {
+          if (!g.bpc) continue;
+          ccy -= Py;
+          ky = 1;
+        }
+        const float gy = (float)max(0, abs(dy) - 1) * g.cs[1];
+        const float rho2 = gx * gx + gy * gy;
+        if (rho2 >= reach2) continue;
+        // Sphere: the z reach shrinks with the lateral gap.  Cylinder (2ani):
+        // the z and perpendicular ranges are independent.
+        const float hz = a.cylinder ? a.reach : sqrtf(reach2 - rho2);
+        const int Kz = min(a.K[2], (int)ceilf(hz * g.inv_cs[2]));
+        const int col = (ccx * Py + ccy) * Pz;
+        const int zlo = cz - Kz, zhi = zb + Kz;
+        // Up to three z segments: wrapped below (kz=-1), inside, wrapped above.
+        // zlo > -Pz and zhi < 2*Pz because Kz < Pz.
+        int seg_lo[3], seg_hi[3], seg_k[3], nseg = 0;
+        if (g.bpc) {
+          if (zlo < 0) { seg_lo[nseg] = zlo + Pz; seg_hi[nseg] = Pz - 1; seg_k[nseg++] = -1; }
+          seg_lo[nseg] = max(zlo, 0); seg_hi[nseg] = min(zhi, Pz - 1); seg_k[nseg++] = 0;
+          if (zhi > Pz - 1) { seg_lo[nseg] = 0; seg_hi[nseg] = zhi - Pz; seg_k[nseg++] = 1; }
+        } else {
+          seg_lo[0] = max(zlo, 0); seg_hi[0] = min(zhi, Pz - 1); seg_k[0] = 0; nseg = 1;
+        }
+        for (int s = 0; s < nseg; ++s) {
+          if (seg_hi[s] < seg_lo[s]) continue;
+          int j0 = a.y_cell_start[col + seg_lo[s]];
+          int j1 = a.y_cell_start[col + seg_hi[s] + 1];
+          if (j1 <= j0) continue;
+          int flag = 0;
+          if (a.auto_) {
+            // Count each unordered pair once: j > i in cell order.
+            if (j1 <= i0 + 1) continue;
+            if (j0 < i0 + ni) {
+              j0 = max(j0, i0 + 1);
+              flag = kCodeMask;
+            }
+          }
+          if (a.map) {
+            j0 = a.map[j0];
+            j1 = a.map[j1];
+            if (j1 <= j0) continue;
+          }
+          if (FILL) {
+            PairItem it;
+            it.i0 = i0;
+            it.ni = ni;
+            it.j0 = j0;
+            it.nj = j1 - j0;
+            it.code = ((kx + 1) * 9 + (ky + 1) * 3 + (seg_k[s] + 1)) | flag;
+            out[count] = it;
+          }
+          ++count;
+        }
+      }
+    }
+  }
+  return count;
+}
+
+
+}  // namespace pcf

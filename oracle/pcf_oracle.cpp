@@ -496,11 +496,15 @@ inline int64_t triplet_bin(const Pt& p1, const Pt& p2, const Pt& p3, int f2, int
   // cos of the 1-2 and 1-3 directions to the line of sight z
   // (3ani_k.cuh:75-76; periodic: signed by the projected z term,
   // 3aniBPC_k.cuh:57,73).
+  // XXY allows point 3 on top of point 1 (d31 = 0): the cosine is then 0/0.
+  // The reference GPU's float-to-int conversion maps NaN to 0 (PTX cvt), so
+  // such a triplet lands in p = 0; x86's conversion would not.
+  auto to_bin = [](double x) { return std::isnan(x) ? 0 : (int)x; };
   const double cth12 = 1 + t12[2] / d12;
   const double cth31 = 1 + t31[2] / d31;
-  const int t = (int)(cth12 * r.ds_th);
+  const int t = to_bin(cth12 * r.ds_th);
   if (t > bn - 1) return -1;
-  const int p = (int)(cth31 * r.ds_th);
+  const int p = to_bin(cth31 * r.ds_th);
   if (p > bn - 1) return -1;
   return (((((int64_t)a * bn + b31) * bn + b23) * bn + t) * bn) + p;
 }
