@@ -30,6 +30,7 @@ from desi_amd import synth  # noqa: E402
 
 PEAK_FP32 = 157.3e12  # MI355X FP32 vector (packed) peak, FLOP/s
 FLOPS_PER_PAIR = 8    # 3 sub + 3 mul + 2 add (FMA = 2 flops)
+FLOPS_PER_TRIPLET = 16  # two squared distances per third-point evaluation
 N = 1 << 20
 BOX = 1000.0
 RMAX = 80.0
@@ -53,6 +54,60 @@ def run_step(e, cd, cr, precision, rebuild=True):
         kms += st["kernel_ms"]
         rech += st["rechecks"]
     return tot_pairs, evals, kms, rech
+
+
+def run_triplet_step(e, cd, cr, precision, rebuild=True):
+    """Config 2: 3iso with periodic boundaries, DDD + DDR + DRR + RRR."""
+    tot = 0
+    evals = 0
+    kms = 0.0
+    for hist, x, y in ((da.PCF_XXX, cd, -1), (da.PCF_XXY, cd, cr), (da.PCF_XXY, cr, cd),
+                       (da.PCF_XXX, cr, -1)):
+        job = da.make_job(da.PCF_3ISO, hist, bins=30, dmax=80.0, cat_x=x, cat_y=y, bpc=True,
+                          box=250.0, precision=precision, rebuild_grid=rebuild)
+        c, _ = e.histogram(job)
+        st = e.stats()
+        tot += int(c.sum())
+        evals += st["evaluations"]
+        kms += st["kernel_ms"]
+    return tot, evals, kms
+
+
+def triplet_leg(e, steps):
+    d = synth.neyman_scott(1024, 32, 5.0, 250.0, seed=11, weights="uniform")
+    r = synth.uniform(32768, 250.0, seed=12, weights="uniform")
+    cd, cr = e.upload(d), e.upload(r)
+    run_triplet_step(e, cd, cr, da.PCF_PREC_MIXED)
+    t0 = time.perf_counter()
+    tot = evals = 0
+    kms = 0.0
+    for _ in range(steps):
+        t, ev, k = run_triplet_step(e, cd, cr, da.PCF_PREC_MIXED)
+        tot += t
+        evals += ev
+        kms += k
+    t1 = time.perf_counter()
+    t64, _, k64 = run_triplet_step(e, cd, cr, da.PCF_PREC_FP64)
+    t2 = time.perf_counter()
+    if t64 != tot // steps:
+        raise SystemExit("plain FP64 and mixed triplet paths disagree on the count")
+    e.release(cd)
+    e.release(cr)
+    ms = (t1 - t0) * 1e3 / steps
+    ach = evals * FLOPS_PER_TRIPLET / (kms * 1e-3)
+    return {
+        "workload": "3iso periodic box 250, N=32768 Neyman-Scott (1024x32, sigma 5) + 32768 "
+                    "uniform, weights U[0.5,1.5], rmax 80, 30 bins, DDD+DDR+DRR+RRR",
+        "value": tot / ((t1 - t0) / steps), "unit": "triplets/s",
+        "ms_per_step": ms, "steps": steps,
+        "kernel_ms_per_step": kms / steps,
+        "evaluations_per_step": evals // steps,
+        "fp64_plain_ms_per_step": (t2 - t1) * 1e3,
+        "fp64_plain_kernel_ms_per_step": k64,
+        "speedup_vs_fp64_plain": (t2 - t1) * 1e3 / ms,
+        "roofline": {"bound": "valu", "achieved": ach / 1e12, "peak": PEAK_FP32 / 1e12,
+                     "unit": "TFLOP/s", "frac": ach / PEAK_FP32, "traffic": None},
+    }
 
 
 def cpu_baseline(d):
@@ -79,6 +134,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--triplet-steps", type=int, default=1)
+    ap.add_argument("--no-triplets", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine runs the pair counting on a single "
@@ -117,6 +174,7 @@ def main():
         ms_step64 = (t3 - t2) * 1e3 / args.fp64_steps
         if p64 // args.fp64_steps != pairs // args.steps:
             raise SystemExit("plain FP64 path and mixed path disagree on the pair count")
+        trip = None if args.no_triplets else triplet_leg(e, args.triplet_steps)
 
     achieved = evals * FLOPS_PER_PAIR / (kms * 1e-3)
     out = {
@@ -144,6 +202,8 @@ def main():
         "fp64_plain_kernel_ms_per_step": k64 / args.fp64_steps,
         "speedup_vs_fp64_plain": ms_step64 / ms_step,
     }
+    if trip is not None:
+        out["triplets"] = trip
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(d)
     print(json.dumps(out), flush=True)

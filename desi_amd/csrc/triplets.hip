@@ -39,15 +39,16 @@ constexpr int kTripBlock = 1024;    // 3iso: one workgroup per CU owns the LDS h
 constexpr int kAniTripBlock = 256;  // 3ani: global histogram, many workgroups per CU
 constexpr int kEmitBlock = 256;
 
+// One pair of the pair list, held by one lane.  The list is grouped by
+// (point 1, b12): key = point1 * bins + b12 (counting sort), so the pairs of
+// one point 1 and one b12 slice are contiguous.
 struct PairRec {
-  float xi, yi, zi;  // point 1, scaled by sigma
-  float xj, yj, zj;  // point 2, scaled, shifted into point 1's frame
-  float wi, wj;
-  int i, j;          // cell-ordered indices in the X grid
-  int meta;          // image code of point 2 (bits 0-4) | b12 << 8 | t << 20
-  int pad;
+  float xj, yj, zj;  // point 2 scaled by sigma, in point 1's frame (image applied)
+  float wj;          // weight of point 2
+  int j;             // cell-ordered index of point 2 in the X grid
+  int meta;          // image code of point 2 (bits 0-4) | b12 << 8 | t << 20 (3ani)
 };
-static_assert(sizeof(PairRec) == 48, "pair record layout");
+static_assert(sizeof(PairRec) == 24, "pair record layout");
 
 struct EmitArgs {
   int nitems;
@@ -125,7 +126,6 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_pairs(
     const float kxL = (float)(img / 9 - 1) * e.Ls, kyL = (float)((img / 3) % 3 - 1) * e.Ls,
                 kzL = (float)(img % 3 - 1) * e.Ls;
     const float sx = ps.x - kxL, sy = ps.y - kyL, sz = ps.z - kzL;
-    int my_cnt = 0;
     for (int j = j0; j < j0 + nj; ++j) {
       const float4 q = xs[j];
       const float dx = q.x - sx, dy = q.y - sy, dz = q.z - sz;
@@ -133,101 +133,174 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_pairs(
       unsigned row = table[min(__float2uint_rz(d2), ntm1)];
       if (masked && j <= my_i) row = bins;
       if (!active) row = bins;
-      int b12 = -1, t12 = 0, t21 = 0;
       // Everything emitted gets the exact bin: the pairs are few compared
       // with the triplets they feed.
       if (row < bins || row == amb_row) {
         const float4 qo = xo[j];
-        b12 = exact_pair(po, qo, img, e.ani, e.bpc, sT, e.bins, e.R, e.L, e.ds_th, &t12);
-        if (e.ani && b12 >= 0) {
-          // the same pair seen from point 2 (3ani needs both orders)
-          if (exact_pair(qo, po, img_neg(img), 1, e.bpc, sT, e.bins, e.R, e.L, e.ds_th, &t21) < 0)
-            t21 = -1;
-        }
+        int t12 = 0, t21 = -1;
+        const int b12 = exact_pair(po, qo, img, e.ani, e.bpc, sT, e.bins, e.R, e.L, e.ds_th, &t12);
         if (b12 >= 0) {
+          const bool fwd = !e.ani || t12 >= 0;
+          // the same pair seen from point 2 (3ani needs both orders)
+          if (e.ani &&
+              exact_pair(qo, po, img_neg(img), 1, e.bpc, sT, e.bins, e.R, e.L, e.ds_th, &t21) < 0)
+            t21 = -1;
+          const int key_i = my_i * e.bins + b12, key_j = j * e.bins + b12;
           if (PASS == 0) {
-            if (!e.ani || t12 >= 0) ++my_cnt;
-            if (e.ani && t21 >= 0) atomicAdd(&cnt[j], 1);
+            if (fwd) atomicAdd(&cnt[key_i], 1);
+            if (e.ani && t21 >= 0) atomicAdd(&cnt[key_j], 1);
           } else {
-            const int slot_i = (!e.ani || t12 >= 0) ? off[my_i] + atomicAdd(&cursor[my_i], 1) : 0;
-            // Both passes take identical decisions, so slot_i < off[my_i + 1];
-            // the bound is kept as a guard against writing past the owner's range.
-            if ((!e.ani || t12 >= 0) && slot_i < off[my_i + 1]) {
-              const int slot = slot_i;
-              PairRec r;
-              r.xi = ps.x; r.yi = ps.y; r.zi = ps.z;
-              r.xj = q.x + kxL; r.yj = q.y + kyL; r.zj = q.z + kzL;
-              r.wi = po.w; r.wj = qo.w;
-              r.i = my_i; r.j = j;
-              r.meta = img | (b12 << 8) | (t12 << 20);
-              r.pad = 0;
-              out[slot] = r;
+            // Both passes take identical decisions, so a slot never passes
+            // the end of its key's range; the bound is kept as a guard.
+            if (fwd) {
+              const int slot = off[key_i] + atomicAdd(&cursor[key_i], 1);
+              if (slot < off[key_i + 1]) {
+                PairRec r;
+                r.xj = q.x + kxL; r.yj = q.y + kyL; r.zj = q.z + kzL;
+                r.wj = qo.w;
+                r.j = j;
+                r.meta = img | (b12 << 8) | ((e.ani ? t12 : 0) << 20);
+                out[slot] = r;
+              }
             }
-            const int slot_j = (e.ani && t21 >= 0) ? off[j] + atomicAdd(&cursor[j], 1) : 0;
-            if (e.ani && t21 >= 0 && slot_j < off[j + 1]) {
-              const int slot = slot_j;
-              PairRec r;
-              r.xi = q.x; r.yi = q.y; r.zi = q.z;
-              r.xj = sx; r.yj = sy; r.zj = sz;
-              r.wi = qo.w; r.wj = po.w;
-              r.i = j; r.j = my_i;
-              r.meta = img_neg(img) | (b12 << 8) | (t21 << 20);
-              r.pad = 0;
-              out[slot] = r;
+            if (e.ani && t21 >= 0) {
+              const int slot = off[key_j] + atomicAdd(&cursor[key_j], 1);
+              if (slot < off[key_j + 1]) {
+                PairRec r;
+                r.xj = sx; r.yj = sy; r.zj = sz;
+                r.wj = po.w;
+                r.j = my_i;
+                r.meta = img_neg(img) | (b12 << 8) | (t21 << 20);
+                out[slot] = r;
+              }
             }
           }
         }
       }
     }
-    if (PASS == 0 && my_cnt) atomicAdd(&cnt[my_i], my_cnt);
   }
 }
 
 struct TripArgs {
-  int nitems;
+  int nx;         // points of X (point-1 tasks)
   int nt;
   float Ls;
   int bins;
-  int auto_;      // XXX: all sides > 0 and the third point is neither 1 nor 2
+  int auto_;      // XXX: all sides > 0, point 3 is neither point 1 nor point 2
   int bpc;
   double R, L, ds_th;
-  int lo, hi;     // b12 slice (3iso)
-  float reach2s;  // (rmax + slack)^2 in scaled units
-  float ds_th_f;
+  int lo, hi;     // b12 slice
+  float reach;    // rmax plus slack (unscaled)
+  int K[3];       // neighbour reach in cells
+  GridGeom g;
 };
 
-// Wave-wide min / max (butterfly over all 64 lanes).
-__device__ __forceinline__ float wave_min(float v) {
-  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
-  return v;
-}
-__device__ __forceinline__ float wave_max(float v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
-  return v;
+__device__ __forceinline__ int cell_axis_dev(float v, float lo, float inv_cs, int P) {
+  int c = (int)floorf((v - lo) * inv_cs);
+  c = c < 0 ? 0 : c;
+  return c >= P ? P - 1 : c;
 }
 
-constexpr float kCosGuard = 1.0f / 1024.0f;  // cosine-bin edge band of the FP32 path
+// Calls f(k0, k1, sx, sy, sz) for every contiguous range [k0, k1) of the
+// K grid that may hold a point within `reach` of point p, with the periodic
+// image s (the range's points are seen at k + s*L).  Columns are pruned by
+// the lateral distance from p, z-ranges by the remaining reach.
+template <class F>
+__device__ __forceinline__ void for_each_segment(const float4 p, const TripArgs& a,
+                                                 const int* __restrict__ cs_k, F f) {
+  const GridGeom& g = a.g;
+  const int Px = g.P[0], Py = g.P[1], Pz = g.P[2];
+  const int cx = cell_axis_dev(p.x, g.lo[0], g.inv_cs[0], Px);
+  const int cy = cell_axis_dev(p.y, g.lo[1], g.inv_cs[1], Py);
+  const int cz = cell_axis_dev(p.z, g.lo[2], g.inv_cs[2], Pz);
+  const float r2 = a.reach * a.reach;
+  for (int dx = -a.K[0]; dx <= a.K[0]; ++dx) {
+    const int vx = cx + dx;
+    int ccx = vx, sx = 0;
+    if (ccx < 0) {
+      if (!g.bpc) continue;
+      ccx += Px;
+      sx = -1;
+    } else if (ccx >= Px) {
+      if (!g.bpc) continue;
+      ccx -= Px;
+      sx = 1;
+    }
+    const float x0 = g.lo[0] + (float)vx * g.cs[0];
+    const float gx = fmaxf(0.f, fmaxf(x0 - p.x, p.x - (x0 + g.cs[0])));
+    for (int dy = -a.K[1]; dy <= a.K[1]; ++dy) {
+      const int vy = cy + dy;
+      int ccy = vy, sy = 0;
+      if (ccy < 0) {
+        if (!g.bpc) continue;
+        ccy += Py;
+        sy = -1;
+      } else if (ccy >= Py) {
+        if (!g.bpc) continue;
+        ccy -= Py;
+        sy = 1;
+      }
+      const float y0 = g.lo[1] + (float)vy * g.cs[1];
+      const float gy = fmaxf(0.f, fmaxf(y0 - p.y, p.y - (y0 + g.cs[1])));
+      const float rho2 = gx * gx + gy * gy;
+      if (rho2 >= r2) continue;
+      const float hz = sqrtf(r2 - rho2);
+      int zlo = (int)floorf((p.z - hz - g.lo[2]) * g.inv_cs[2]);
+      int zhi = (int)floorf((p.z + hz - g.lo[2]) * g.inv_cs[2]);
+      zlo = max(zlo, cz - a.K[2]);
+      zhi = min(zhi, cz + a.K[2]);
+      const int col = (ccx * Py + ccy) * Pz;
+      if (!g.bpc) {
+        zlo = max(zlo, 0);
+        zhi = min(zhi, Pz - 1);
+        if (zhi >= zlo) {
+          const int k0 = cs_k[col + zlo], k1 = cs_k[col + zhi + 1];
+          if (k1 > k0) f(k0, k1, sx, sy, 0);
+        }
+      } else {
+        if (zlo < 0) {
+          const int k0 = cs_k[col + zlo + Pz], k1 = cs_k[col + Pz];
+          if (k1 > k0) f(k0, k1, sx, sy, -1);
+        }
+        const int a0 = max(zlo, 0), a1 = min(zhi, Pz - 1);
+        if (a1 >= a0) {
+          const int k0 = cs_k[col + a0], k1 = cs_k[col + a1 + 1];
+          if (k1 > k0) f(k0, k1, sx, sy, 0);
+        }
+        if (zhi > Pz - 1) {
+          const int k0 = cs_k[col], k1 = cs_k[col + zhi - Pz + 1];
+          if (k1 > k0) f(k0, k1, sx, sy, 1);
+        }
+      }
+    }
+  }
+}
 
-// ANI = 0: V histogram [b12 - lo][b23][b31] in LDS.  ANI = 1: the ordered 5-D
-// histogram in global memory.
-template <int ANI, int WEIGHTED>
+// Lanes hold 64 pairs (1, 2) of one point 1 in a b12 slice; the third points
+// stream through scalar loads from point 1's neighbour segments.  The 1-3 side
+// is uniform across the wave: a third point out of point 1's range is skipped
+// by the whole wave.  ANI = 0: V histogram [b12 - lo][b23][b31] in LDS.
+// ANI = 1: the ordered 5-D histogram in global memory.  PLAIN = 1: every side
+// through the FP64 arithmetic (the plain FP64 path).
+template <int ANI, int WEIGHTED, int PLAIN>
 __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
+    const float4* __restrict__ xs, const float4* __restrict__ xo,
     const float4* __restrict__ ks_all, const float4* __restrict__ ko_all,
-    const float4* __restrict__ xo_all, const PairItem* __restrict__ items,
-    int* __restrict__ counter, const PairRec* __restrict__ pairs,
-    const uint8_t* __restrict__ table, const double* __restrict__ T,
-    unsigned long long* __restrict__ hist, double* __restrict__ whist,
-    unsigned long long* __restrict__ stats, TripArgs a) {
+    const int* __restrict__ k_cell_start, const PairRec* __restrict__ pairs,
+    const int* __restrict__ poff, int* __restrict__ counter, const uint8_t* __restrict__ table,
+    const double* __restrict__ T, unsigned long long* __restrict__ hist,
+    double* __restrict__ whist, unsigned long long* __restrict__ stats, TripArgs a) {
   constexpr int BLOCK = ANI ? kAniTripBlock : kTripBlock;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ uint8_t tab[kMixedTableMax];
   __shared__ double sT[kMaxTripBins + 2];
   const int bins = a.bins;
   const int bins2 = bins * bins;
-  const int nrows = ANI ? 0 : (a.hi - a.lo) * bins2 + 1;  // + dummy row
+  const int nrows = ANI ? 0 : (a.hi - a.lo) * bins2;
   uint32_t* hc = reinterpret_cast<uint32_t*>(smem);
   double* hw = reinterpret_cast<double*>(smem + ((nrows * 4 + 15) & ~15));
-  for (int t = threadIdx.x; t < a.nt; t += BLOCK) tab[t] = table[t];
+  if (!PLAIN)
+    for (int t = threadIdx.x; t < a.nt; t += BLOCK) tab[t] = table[t];
   for (int t = threadIdx.x; t <= bins; t += BLOCK) sT[t] = T[t];
   if (!ANI) {
     for (int t = threadIdx.x; t < nrows; t += BLOCK) {
@@ -240,155 +313,139 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
   const int lane = threadIdx.x & (kWave - 1);
   const unsigned ntm1 = (unsigned)(a.nt - 1);
   const unsigned ubins = (unsigned)bins;
-  const unsigned dummy = (unsigned)(nrows - 1);
   unsigned long long evals = 0, nre = 0;
-  const int64_t b5 = ANI ? (int64_t)bins2 * bins2 * bins : 0;
 
   for (;;) {
-    int it = 0;
-    if (lane == 0) it = atomicAdd(counter, 1);
-    it = __builtin_amdgcn_readfirstlane(it);
-    if (it >= a.nitems) break;
-    const int k0 = __builtin_amdgcn_readfirstlane(items[it].i0);
-    const int nk = __builtin_amdgcn_readfirstlane(items[it].ni);
-    const int p0 = __builtin_amdgcn_readfirstlane(items[it].j0);
-    const int np = __builtin_amdgcn_readfirstlane(items[it].nj);
-    const int img = __builtin_amdgcn_readfirstlane(items[it].code) & 31;
-    const bool active = lane < nk;
-    const int kidx = k0 + lane;
-    float4 kp = make_float4(1e18f, 1e18f, 1e18f, 0.f), ko = kp;
-    if (active) {
-      kp = ks_all[kidx];
-      ko = ko_all[kidx];
-    }
-    // The tile (third points) is shifted by the item's image: k' = k - s*L.
-    const int sxi = img / 9 - 1, syi = (img / 3) % 3 - 1, szi = img % 3 - 1;
-    const float kx = kp.x - (float)sxi * a.Ls, ky = kp.y - (float)syi * a.Ls,
-                kz = kp.z - (float)szi * a.Ls;
-    // Bounding box of the tile, for skipping pairs whose point 2 is out of reach.
-    const float bxl = wave_min(active ? kx : 3e38f), bxh = wave_max(active ? kx : -3e38f);
-    const float byl = wave_min(active ? ky : 3e38f), byh = wave_max(active ? ky : -3e38f);
-    const float bzl = wave_min(active ? kz : 3e38f), bzh = wave_max(active ? kz : -3e38f);
-    // 3ani, periodic: the reference's z term of the 1-3 side is |dz| - L*f3,
-    // i.e. minus |dz'| on a projected axis (3aniBPC_k.cuh:73).
-    const float zsign = (a.bpc && szi != 0) ? -1.f : 1.f;
-    const double wk = (double)ko.w;
+    int task = 0;
+    if (lane == 0) task = atomicAdd(counter, 1);
+    const int i = __builtin_amdgcn_readfirstlane(task);
+    if (i >= a.nx) break;
+    const int pbeg = __builtin_amdgcn_readfirstlane(poff[i * bins + a.lo]);
+    const int pend = __builtin_amdgcn_readfirstlane(poff[i * bins + a.hi]);
+    if (pend <= pbeg) continue;
+    const float4 is = xs[i];   // point 1, scaled
+    const float4 io = xo[i];   // point 1, original
+    const double wi = (double)io.w;
 
-    for (int p = p0; p < p0 + np; ++p) {
-      const PairRec r = pairs[p];
+    for (int pc = pbeg; pc < pend; pc += kWave) {
+      const int p = pc + lane;
+      const bool active = p < pend;
+      PairRec r;
+      r.xj = r.yj = r.zj = 1e18f;
+      r.wj = 0.f;
+      r.j = -1;
+      r.meta = 0;
+      if (active) r = pairs[p];
       const int b12 = (r.meta >> 8) & 0xfff;
-      if (!ANI && (b12 < a.lo || b12 >= a.hi)) continue;
-      {
-        const float ex = fmaxf(0.f, fmaxf(bxl - r.xj, r.xj - bxh));
-        const float ey = fmaxf(0.f, fmaxf(byl - r.yj, r.yj - byh));
-        const float ez = fmaxf(0.f, fmaxf(bzl - r.zj, r.zj - bzh));
-        if (ex * ex + ey * ey + ez * ez >= a.reach2s) continue;
-      }
-      evals += (unsigned long long)nk;
-      const float ax = kx - r.xi, ay = ky - r.yi, az = kz - r.zi;
-      const float d31 = __builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax));
-      const float cx = kx - r.xj, cy = ky - r.yj, cz = kz - r.zj;
-      const float d23 = __builtin_fmaf(cz, cz, __builtin_fmaf(cy, cy, cx * cx));
-      // Table codes: bin, bins = certainly out, bins+1 = ambiguous.  An
-      // ambiguous side with the other certainly out is sent to the exact path
-      // too (conservative).
-      const unsigned c31 = tab[min(__float2uint_rz(d31), ntm1)];
-      const unsigned c23 = tab[min(__float2uint_rz(d23), ntm1)];
-      const unsigned m = max(c23, c31);
-      unsigned row;
-      bool amb = m > ubins;
-      int pb = 0;
-      if (!ANI) {
-        row = m < ubins ? (unsigned)(b12 - a.lo) * bins2 + c23 * ubins + c31 : dummy;
-      } else {
-        // cosine bin of the 1-3 direction in FP32 (the z term is |dz| - L*f
-        // when periodic); near a bin edge -> exact path
-        const float dzz = a.bpc ? zsign * fabsf(az) : az;
-        const float cth = __builtin_fmaf(dzz, __builtin_amdgcn_rsqf(fmaxf(d31, 1e-30f)), 1.f);
-        const float pf = cth * a.ds_th_f;
-        pb = (int)pf;
-        const float fr = pf - (float)pb;
-        if (m < ubins && (fr < kCosGuard || fr > 1.f - kCosGuard)) amb = true;
-        row = (m < ubins && pb <= bins - 1) ? 0u : 1u;  // 0 = deposit, 1 = none
-      }
-      if (!active) amb = false;
-      const unsigned long long ab = __ballot(amb);
-      unsigned b23 = c23, b31 = c31;
-      if (ab) {
-        nre += __popcll(ab);
-        if (amb) {
-          bool ok = true;
-          if (a.auto_ && (kidx == r.i || kidx == r.j)) ok = false;
-          int f2 = 0, f3 = 0;
-          float4 io = make_float4(0, 0, 0, 0), jo = io;
-          if (ok) {
-            io = xo_all[r.i];
-            jo = xo_all[r.j];
-            const int im2 = r.meta & 31;
-            f2 = ((im2 / 9 - 1) != 0 ? 1 : 0) | (((im2 / 3) % 3 - 1) != 0 ? 2 : 0) |
-                 ((im2 % 3 - 1) != 0 ? 4 : 0);
-            // third point's image relative to point 1 is -s (k' = k - s*L)
-            f3 = image_flags(io, ko, img, true);
-            ok = f3 >= 0;
-          }
-          if (ok) {
-            const float a1[3] = {io.x, io.y, io.z}, a2[3] = {jo.x, jo.y, jo.z},
-                        a3[3] = {ko.x, ko.y, ko.z};
-            double t31[3], t23[3];
+      const int t12 = (r.meta >> 20) & 0xfff;
+      const int im2 = r.meta & 31;
+      const int f2 = ((im2 / 9 - 1) != 0 ? 1 : 0) | (((im2 / 3) % 3 - 1) != 0 ? 2 : 0) |
+                     ((im2 % 3 - 1) != 0 ? 4 : 0);
+      const double w12 = wi * (double)r.wj;
+      const unsigned nact = (unsigned)__popcll(__ballot(active));
+      // iso LDS row base of this pair; ani global index base
+      const unsigned rbase = (unsigned)(b12 - a.lo) * (unsigned)bins2;
+      const int64_t abase = ((int64_t)b12 * bins * bins * bins) * bins + (int64_t)t12 * bins;
+      float4 jo = make_float4(0.f, 0.f, 0.f, 0.f);
+      bool jo_loaded = false;
+
+      for_each_segment(io, a, k_cell_start, [&](int k0, int k1, int sx, int sy, int sz) {
+        const float sxL = (float)sx * a.Ls, syL = (float)sy * a.Ls, szL = (float)sz * a.Ls;
+        const int simg = (sx + 1) * 9 + (sy + 1) * 3 + (sz + 1);
+        for (int k = k0; k < k1; ++k) {
+          if (a.auto_ && k == i) continue;
+          const float4 kq = ks_all[k];
+          const float kx = kq.x + sxL, ky = kq.y + syL, kz = kq.z + szL;
+          // ---- side 1-3 (uniform across the wave)
+          const float ax = kx - is.x, ay = ky - is.y, az = kz - is.z;
+          const float d31 = __builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax));
+          unsigned c31 = PLAIN ? ubins + 1 : tab[min(__float2uint_rz(d31), ntm1)];
+          if (c31 == ubins) continue;  // certainly out of range of point 1
+          int b31 = (int)c31, pb = 0, f3 = 0;
+          float4 kco = make_float4(0.f, 0.f, 0.f, 0.f);
+          double t31[3] = {0.0, 0.0, 0.0};
+          const bool need31 = ANI || c31 > ubins;  // exact 1-3 side (uniform)
+          if (need31) {
+            kco = ko_all[k];
+            f3 = image_flags(io, kco, simg, false);
+            if (f3 < 0) continue;
+            const float a1[3] = {io.x, io.y, io.z}, a3[3] = {kco.x, kco.y, kco.z};
             for (int q = 0; q < 3; ++q) {
-              const double d31q = __dsub_rn((double)a3[q], (double)a1[q]);
-              const double d23q = __dsub_rn((double)a3[q], (double)a2[q]);
-              if (a.bpc) {
-                const int g2 = (f2 >> q) & 1, g3 = (f3 >> q) & 1;
-                t31[q] = proj_term(d31q, a.L, g3);
-                t23[q] = proj_term(d23q, a.L, g2 != g3);
-              } else {
-                t31[q] = d31q;
-                t23[q] = d23q;
-              }
+              const double d = __dsub_rn((double)a3[q], (double)a1[q]);
+              t31[q] = a.bpc ? proj_term(d, a.L, (f3 >> q) & 1) : d;
             }
-            const double s31 =
-                __dadd_rn(__dadd_rn(__dmul_rn(t31[0], t31[0]), __dmul_rn(t31[1], t31[1])),
-                          __dmul_rn(t31[2], t31[2]));
-            const double s23 =
-                __dadd_rn(__dadd_rn(__dmul_rn(t23[0], t23[0]), __dmul_rn(t23[1], t23[1])),
-                          __dmul_rn(t23[2], t23[2]));
-            if (a.auto_ && !(s23 > 0.0 && s31 > 0.0)) ok = false;
-            const int e23 = class_lds(s23, sT, bins, a.R, 0);
-            const int e31 = class_lds(s31, sT, bins, a.R, 0);
-            if (e23 < 0 || e31 < 0) ok = false;
-            if (ok) {
-              b23 = (unsigned)e23;
-              b31 = (unsigned)e31;
-              if (ANI) {
-                const double cth = __dadd_rn(1.0, t31[2] / sqrt(s31));
-                pb = (int)__dmul_rn(cth, a.ds_th);
-                if (pb > bins - 1) ok = false;
+            const double s31 = __dadd_rn(
+                __dadd_rn(__dmul_rn(t31[0], t31[0]), __dmul_rn(t31[1], t31[1])),
+                __dmul_rn(t31[2], t31[2]));
+            if (a.auto_ && !(s31 > 0.0)) continue;
+            b31 = class_lds(s31, sT, bins, a.R, 0);
+            if (b31 < 0) continue;
+            if (ANI) {
+              const double cth = __dadd_rn(1.0, t31[2] / sqrt(s31));
+              const double pp = __dmul_rn(cth, a.ds_th);
+              pb = isnan(pp) ? 0 : (int)pp;
+              if (pb > bins - 1) continue;
+            }
+          }
+          evals += nact;
+          const double wk = (double)kq.w;  // the scaled copy keeps the weight
+          // ---- side 2-3 (per lane)
+          const float cx = kx - r.xj, cy = ky - r.yj, cz = kz - r.zj;
+          const float d23 = __builtin_fmaf(cz, cz, __builtin_fmaf(cy, cy, cx * cx));
+          unsigned c23 = PLAIN ? ubins + 1 : tab[min(__float2uint_rz(d23), ntm1)];
+          bool amb = active && c23 > ubins;
+          if (__ballot(amb)) {
+            if (!PLAIN) nre += __popcll(__ballot(amb));
+            if (amb) {
+              c23 = ubins;  // excluded unless the exact path finds a bin
+              if (!(a.auto_ && k == r.j)) {
+                if (!jo_loaded) {
+                  jo = xo[r.j];
+                  jo_loaded = true;
+                }
+                int g3 = f3;
+                if (!need31) {
+                  kco = ko_all[k];
+                  g3 = image_flags(io, kco, simg, false);
+                }
+                if (g3 >= 0) {
+                  const float a2[3] = {jo.x, jo.y, jo.z}, a3[3] = {kco.x, kco.y, kco.z};
+                  double t23[3];
+                  for (int q = 0; q < 3; ++q) {
+                    const double d = __dsub_rn((double)a3[q], (double)a2[q]);
+                    t23[q] = a.bpc ? proj_term(d, a.L, ((f2 >> q) & 1) != ((g3 >> q) & 1)) : d;
+                  }
+                  const double s23 = __dadd_rn(
+                      __dadd_rn(__dmul_rn(t23[0], t23[0]), __dmul_rn(t23[1], t23[1])),
+                      __dmul_rn(t23[2], t23[2]));
+                  if (!(a.auto_ && !(s23 > 0.0))) {
+                    const int e23 = class_lds(s23, sT, bins, a.R, 0);
+                    if (e23 >= 0) c23 = (unsigned)e23;
+                  }
+                }
               }
             }
           }
-          if (!ANI) row = ok ? (unsigned)(b12 - a.lo) * bins2 + b23 * ubins + b31 : dummy;
-          else row = ok ? 0u : 1u;
+          if (active && c23 < ubins) {
+            if (!ANI) {
+              const unsigned row = rbase + c23 * ubins + (unsigned)b31;
+              atomicAdd(&hc[row], 1u);
+              if (WEIGHTED) atomicAdd(&hw[row], __dmul_rn(w12, wk));
+            } else {
+              const int64_t idx = abase + ((int64_t)b31 * bins * bins * bins +
+                                           (int64_t)c23 * bins * bins) + pb;
+              atomicAdd(&hist[idx], 1ull);
+              if (WEIGHTED) atomicAdd(&whist[idx], __dmul_rn(w12, wk));
+            }
+          }
         }
-      }
-      if (!ANI) {
-        atomicAdd(&hc[row], 1u);
-        if (WEIGHTED)
-          atomicAdd(&hw[row], __dmul_rn(__dmul_rn((double)r.wi, (double)r.wj), wk));
-      } else if (row == 0u && active) {
-        const int t12 = (r.meta >> 20) & 0xfff;
-        const int64_t idx = ((((int64_t)b12 * bins + b31) * bins + b23) * bins + t12) * bins + pb;
-        atomicAdd(&hist[idx], 1ull);
-        if (WEIGHTED)
-          atomicAdd(&whist[idx], __dmul_rn(__dmul_rn((double)r.wi, (double)r.wj), wk));
-        (void)b5;
-      }
+      });
     }
   }
   if (!ANI) {
     __syncthreads();
     const int base = a.lo * bins2;
-    for (int t = threadIdx.x; t < nrows - 1; t += BLOCK) {
+    for (int t = threadIdx.x; t < nrows; t += BLOCK) {
       if (hc[t]) {
         atomicAdd(&hist[base + t], (unsigned long long)hc[t]);
         if (WEIGHTED) atomicAdd(&whist[base + t], hw[t]);
@@ -474,6 +531,7 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   const int64_t b64 = bins;
   const int64_t nflat = ani ? b64 * b64 * b64 * b64 * b64 : b64 * b64 * b64;
   const bool weighted = !(X->all_w1 && K->all_w1);
+  const bool plain = job->precision == PCF_PREC_FP64;
 
   GridGeom g;
   int Kc[3];
@@ -530,7 +588,8 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
     ksc = c->scaled_y.as<float4>();
   }
 
-  // 1. Pair list over X (unordered pairs i < j).
+  // 1. Pair list over X (unordered pairs i < j), counting-sorted by
+  //    key = point1 * bins + b12.
   WorkListArgs wa;
   wa.x_cell_start = gx->cell_start;
   wa.y_cell_start = gx->cell_start;
@@ -543,12 +602,14 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   int npitems = 0;
   HIPCHK(c, build_pair_worklist(wa, c->wl_counts, c->wl_items, &npitems, c->stream));
   const int64_t nx = gx->n;
-  HIPCHK(c, c->trip_idx.ensure(sizeof(int) * (size_t)(3 * (nx + 1) + 8)));
+  const int64_t nkeys = nx * bins;
+  if (nkeys + 1 > (int64_t)INT32_MAX) return fail(c, PCF_ERR_UNSUPPORTED, "too many pair keys");
+  HIPCHK(c, c->trip_idx.ensure(sizeof(int) * (size_t)(3 * (nkeys + 1) + 8)));
   int* d_cnt = c->trip_idx.as<int>();
-  int* d_off = d_cnt + (nx + 1);
-  int* d_cur = d_off + (nx + 1);
-  HIPCHK(c, launch_zero(d_cnt, sizeof(int) * (size_t)(nx + 1), c->stream));
-  HIPCHK(c, launch_zero(d_cur, sizeof(int) * (size_t)(nx + 1), c->stream));
+  int* d_off = d_cnt + (nkeys + 1);
+  int* d_cur = d_off + (nkeys + 1);
+  HIPCHK(c, launch_zero(d_cnt, sizeof(int) * (size_t)(nkeys + 1), c->stream));
+  HIPCHK(c, launch_zero(d_cur, sizeof(int) * (size_t)(nkeys + 1), c->stream));
   EmitArgs ea;
   ea.nitems = npitems;
   ea.nt = nt;
@@ -567,10 +628,10 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
                        (PairRec*)nullptr, ea);
     HIPCHK(c, hipGetLastError());
   }
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanBlock), 0, c->stream, d_cnt, nx, d_off);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanBlock), 0, c->stream, d_cnt, nkeys, d_off);
   HIPCHK(c, hipGetLastError());
   int npairs = 0;
-  HIPCHK(c, hipMemcpyAsync(&npairs, d_off + nx, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&npairs, d_off + nkeys, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (npairs < 0) return fail(c, PCF_ERR_UNSUPPORTED, "pair list exceeds 2^31 records");
   HIPCHK(c, c->trip_pairs.ensure(sizeof(PairRec) * (size_t)std::max(npairs, 1)));
@@ -582,21 +643,7 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
     HIPCHK(c, hipGetLastError());
   }
 
-  // 2. Triplet work list: tiles of third points against pair ranges.
-  WorkListArgs ta;
-  ta.x_cell_start = gk->cell_start;
-  ta.y_cell_start = gx->cell_start;
-  ta.g = g;
-  for (int a = 0; a < 3; ++a) ta.K[a] = Kc[a];
-  ta.reach = reach;
-  ta.auto_ = 0;
-  ta.cylinder = 0;
-  ta.map = d_off;
-  int ntitems = 0;
-  if (npairs > 0)
-    HIPCHK(c, build_pair_worklist(ta, c->wl_counts, c->wl_items2, &ntitems, c->stream));
-
-  // 3. Histograms.
+  // 2. Histograms.
   const size_t hbytes = sizeof(unsigned long long) * (size_t)nflat;
   const size_t wbytes = sizeof(double) * (size_t)nflat;
   HIPCHK(c, c->trip_hist.ensure(2 * (hbytes + wbytes) + 64));
@@ -611,7 +658,7 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   HIPCHK(c, launch_zero(d_stats, 64, c->stream));
 
   TripArgs t;
-  t.nitems = ntitems;
+  t.nx = (int)nx;
   t.nt = nt;
   t.Ls = L * sigma;
   t.bins = bins;
@@ -620,10 +667,11 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   t.R = kr.R;
   t.L = (double)L;
   t.ds_th = (double)ds_th_f;
-  t.ds_th_f = ds_th_f;
-  t.reach2s = (float)((double)reach * reach * (double)inv_h);
+  t.reach = reach;
+  for (int q = 0; q < 3; ++q) t.K[q] = Kc[q];
+  t.g = g;
   HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-  if (ntitems > 0) {
+  if (npairs > 0) {
     if (!ani) {
       // LDS budget of one 1024-thread workgroup per CU: the slice of b12 rows
       // whose counts (and FP64 weights) fit beside the table.
@@ -633,22 +681,22 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
       for (int lo = 0; lo < bins; lo += S) {
         t.lo = lo;
         t.hi = std::min(bins, lo + S);
-        const int nrows = (t.hi - t.lo) * bins * bins + 1;
+        const int nrows = (t.hi - t.lo) * bins * bins;
         const size_t lds = ((size_t)nrows * 4 + 15) / 16 * 16 + (weighted ? (size_t)nrows * 8 : 0);
         HIPCHK(c, launch_zero(d_counter, sizeof(int), c->stream));
-        if (weighted) {
-          HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_triplets<0, 1>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-          hipLaunchKernelGGL((k_triplets<0, 1>), dim3(c->ncu), dim3(kTripBlock), lds, c->stream,
-                             ksc, gk->pts, gx->pts, c->wl_items2.as<PairItem>(), d_counter,
-                             d_pairs, d_ktab, d_T, d_V, d_VW, d_stats, t);
-        } else {
-          HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_triplets<0, 0>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-          hipLaunchKernelGGL((k_triplets<0, 0>), dim3(c->ncu), dim3(kTripBlock), lds, c->stream,
-                             ksc, gk->pts, gx->pts, c->wl_items2.as<PairItem>(), d_counter,
-                             d_pairs, d_ktab, d_T, d_V, d_VW, d_stats, t);
-        }
+#define PCF_TRIP_ISO(W, P)                                                                     \
+  do {                                                                                         \
+    HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_triplets<0, W, P>),          \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));      \
+    hipLaunchKernelGGL((k_triplets<0, W, P>), dim3(c->ncu), dim3(kTripBlock), lds, c->stream,  \
+                       xs, gx->pts, ksc, gk->pts, gk->cell_start, d_pairs, d_off, d_counter,   \
+                       d_ktab, d_T, d_V, d_VW, d_stats, t);                                    \
+  } while (0)
+        if (weighted && plain) PCF_TRIP_ISO(1, 1);
+        else if (weighted) PCF_TRIP_ISO(1, 0);
+        else if (plain) PCF_TRIP_ISO(0, 1);
+        else PCF_TRIP_ISO(0, 0);
+#undef PCF_TRIP_ISO
         HIPCHK(c, hipGetLastError());
       }
     } else {
@@ -656,14 +704,15 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
       t.hi = bins;
       HIPCHK(c, launch_zero(d_counter, sizeof(int), c->stream));
       const int blocks = c->ncu * 4;
-      if (weighted)
-        hipLaunchKernelGGL((k_triplets<1, 1>), dim3(blocks), dim3(kAniTripBlock), 0, c->stream,
-                           ksc, gk->pts, gx->pts, c->wl_items2.as<PairItem>(), d_counter, d_pairs,
-                           d_ktab, d_T, d_V, d_VW, d_stats, t);
-      else
-        hipLaunchKernelGGL((k_triplets<1, 0>), dim3(blocks), dim3(kAniTripBlock), 0, c->stream,
-                           ksc, gk->pts, gx->pts, c->wl_items2.as<PairItem>(), d_counter, d_pairs,
-                           d_ktab, d_T, d_V, d_VW, d_stats, t);
+#define PCF_TRIP_ANI(W, P)                                                                     \
+  hipLaunchKernelGGL((k_triplets<1, W, P>), dim3(blocks), dim3(kAniTripBlock), 0, c->stream,   \
+                     xs, gx->pts, ksc, gk->pts, gk->cell_start, d_pairs, d_off, d_counter,     \
+                     d_ktab, d_T, d_V, d_VW, d_stats, t)
+      if (weighted && plain) PCF_TRIP_ANI(1, 1);
+      else if (weighted) PCF_TRIP_ANI(1, 0);
+      else if (plain) PCF_TRIP_ANI(0, 1);
+      else PCF_TRIP_ANI(0, 0);
+#undef PCF_TRIP_ANI
       HIPCHK(c, hipGetLastError());
     }
   }
@@ -692,7 +741,7 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   pcf_stats st{};
   st.evaluations = hc[nflat];
   st.rechecks = hc[nflat + 1];
-  st.work_items = (uint64_t)ntitems;
+  st.work_items = (uint64_t)npairs;
   float ms = 0.f;
   HIPCHK(c, hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));
   st.kernel_ms = ms;

@@ -17,13 +17,14 @@ pytestmark = pytest.mark.gpu
 WREL = 1e-12
 
 
-def _run(engine, a, b, *, ani, bpc, box, bins, dmax, cpr=0):
+def _run(engine, a, b, *, ani, bpc, box, bins, dmax, cpr=0, prec=da.PCF_PREC_MIXED):
     ca = engine.upload(a)
     cb = engine.upload(b) if b is not None else -1
     try:
         job = da.make_job(da.PCF_3ANI if ani else da.PCF_3ISO,
                           da.PCF_XXX if b is None else da.PCF_XXY, bins=bins, dmax=dmax,
-                          cat_x=ca, cat_y=cb, bpc=bpc, box=box, cells_per_rmax=cpr)
+                          cat_x=ca, cat_y=cb, bpc=bpc, box=box, cells_per_rmax=cpr,
+                          precision=prec)
         c, w = engine.histogram(job)
         st = engine.stats()
     finally:
@@ -33,10 +34,11 @@ def _run(engine, a, b, *, ani, bpc, box, bins, dmax, cpr=0):
     return c, w, st
 
 
-def _check(engine, a, b, *, ani, bpc, box, bins, dmax, cpr=0):
+def _check(engine, a, b, *, ani, bpc, box, bins, dmax, cpr=0, prec=da.PCF_PREC_MIXED):
     print(f"case 3{'ani' if ani else 'iso'} bpc={bpc} n={len(a)} cross={b is not None} "
-          f"bins={bins} dmax={dmax} cpr={cpr}", flush=True)
-    c, w, st = _run(engine, a, b, ani=ani, bpc=bpc, box=box, bins=bins, dmax=dmax, cpr=cpr)
+          f"bins={bins} dmax={dmax} cpr={cpr} prec={prec}", flush=True)
+    c, w, st = _run(engine, a, b, ani=ani, bpc=bpc, box=box, bins=bins, dmax=dmax, cpr=cpr,
+                    prec=prec)
     co, wo = O.triplets(a, b, ani=ani, bpc=bpc, box=box, bins=bins, dmax=dmax)
     bad = np.nonzero(c != co)[0]
     assert len(bad) == 0, (f"{len(bad)} bins differ, e.g. {bad[:8]}: gpu {c[bad[:8]]} "
@@ -61,6 +63,16 @@ def test_3ani_sample(engine, bpc, cross):
     d = catalog("data_500.dat")
     r = catalog("rand0_500.dat")
     _check(engine, d, r if cross else None, ani=True, bpc=bpc, box=250.0, bins=15, dmax=60.0)
+
+
+@pytest.mark.parametrize("ani", [False, True])
+@pytest.mark.parametrize("bpc", [False, True])
+def test_plain_fp64_path(engine, ani, bpc):
+    d = synth.neyman_scott(32, 16, 5.0, 250.0, seed=23, weights="uniform")
+    r = synth.uniform(700, 250.0, seed=24, weights="uniform")
+    bins = 15 if ani else 30
+    _check(engine, d, r, ani=ani, bpc=bpc, box=250.0, bins=bins, dmax=80.0,
+           prec=da.PCF_PREC_FP64)
 
 
 @pytest.mark.parametrize("ani", [False, True])
