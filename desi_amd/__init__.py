@@ -29,6 +29,7 @@ EXPORTED_SYMBOLS = (
     "pcf_hist_size", "pcf_create", "pcf_destroy", "pcf_last_error", "pcf_create_error",
     "pcf_catalog_upload", "pcf_catalog_upload_device", "pcf_catalog_release",
     "pcf_catalog_size", "pcf_histogram", "pcf_last_stats", "pcf_analytic_rr",
+    "pcf_analytic_3iso",
     "pcf_set_hbm_budget", "pcf_submit", "pcf_poll", "pcf_wait",
 )
 
@@ -86,6 +87,8 @@ def load_library(path=LIB_PATH):
     L.pcf_histogram.argtypes = [vp, ctypes.POINTER(PcfJob), vp, vp]
     L.pcf_last_stats.argtypes = [vp, ctypes.POINTER(PcfStats)]
     L.pcf_analytic_rr.argtypes = [vp, c_i32, c_i64, ctypes.c_float, ctypes.c_float, vp]
+    L.pcf_analytic_3iso.argtypes = [vp, c_i32, c_i32, ctypes.c_float, ctypes.c_float, c_i64,
+                                    c_i32, c_i32, vp, vp, vp, vp]
     L.pcf_set_hbm_budget.argtypes = [vp, c_i64, c_i32]
     L.pcf_submit.argtypes = [vp, ctypes.POINTER(PcfJob), ctypes.POINTER(c_i64)]
     L.pcf_poll.argtypes = [vp, c_i64, ctypes.POINTER(c_i32)]
@@ -177,6 +180,20 @@ class Engine:
         rr = np.zeros(bins, np.float64)
         self._check(self.lib.pcf_analytic_rr(self.ctx, bins, np_, dmax, box, rr.ctypes.data))
         return rr
+
+    def analytic_3iso(self, cat, bins, dmax, box, np_=0, ptt=0, bn_ref=0):
+        """Analytic DDR (= DRR) and RRR of the periodic 3iso case plus the
+        intermediate ff_av and ff_av_ref (pcf_analytic_3iso)."""
+        ptt_e = ptt if ptt > 0 else 100
+        ref_e = bn_ref if bn_ref > 0 else 200
+        ddr = np.zeros(bins ** 3, np.float64)
+        rrr = np.zeros(bins ** 3, np.float64)
+        ff = np.zeros(bins, np.float64)
+        ffr = np.zeros(bins * ref_e, np.float64)
+        self._check(self.lib.pcf_analytic_3iso(self.ctx, cat, bins, dmax, box, int(np_), ptt_e,
+                                               ref_e, ddr.ctypes.data, rrr.ctypes.data,
+                                               ff.ctypes.data, ffr.ctypes.data))
+        return ddr, rrr, ff, ffr
 
     # ---- asynchronous (serving) ------------------------------------------
     def set_hbm_budget(self, nbytes, max_inflight=4):

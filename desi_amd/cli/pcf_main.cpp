@@ -310,9 +310,23 @@ int main(int argc, char** argv) {
     const int type = (o.type == "3iso") ? PCF_3ISO : PCF_3ANI;
     const string tag = (type == PCF_3ISO ? "iso_" : "ani_") + string(o.bpc ? "BPC_" : "");
     if (type == PCF_3ISO && o.bpc && o.analytic) {
-      cout << "3iso -bpc -a: analytic triplet randoms are not available in this build." << endl;
+      // cuda/pcf3isoBPC.cuh:177-375: periodic DDD, analytic RRR and DDR, DRR = DDR.
+      if ((rc = H(type, PCF_XXX, cd, -1, "DDDiso_BPCanalytic_" + dn)) != PCF_OK)
+        return die(run.ctx, rc, "DDD");
+      const size_t b3 = (size_t)o.bn * o.bn * o.bn;
+      std::vector<double> ddr(b3), rrr(b3);
+      if ((rc = pcf_analytic_3iso(run.ctx, cd, o.bn, o.dmax, run.o.size_box, o.np, 0, 0,
+                                  ddr.data(), rrr.data(), nullptr, nullptr)) != PCF_OK)
+        return die(run.ctx, rc, "analytic DDR/RRR");
+      run.save(type, "DDRiso_BPCanalytic_" + dn, ddr);
+      run.save(type, "DRRiso_BPCanalytic_" + dn, ddr);
+      run.save(type, "RRRiso_BPCanalytic_" + dn, rrr);
+      auto t1 = std::chrono::steady_clock::now();
+      cout << "Spent " << std::chrono::duration<double, std::milli>(t1 - t0).count()
+           << " miliseconds to compute and save all the histograms." << endl;
       pcf_destroy(run.ctx);
-      return 1;
+      cout << "Program terminated..." << endl;
+      return 0;
     }
     if ((rc = H(type, PCF_XXX, cd, -1, "DDD" + tag + dn)) != PCF_OK)
       return die(run.ctx, rc, "DDD");

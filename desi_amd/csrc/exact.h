@@ -1,5 +1,7 @@
 // exact.h -- the FP64 arithmetic of the CPU oracle, for the in-kernel exact
-// path of the pair and triplet kernels (no contraction: explicit _rn ops).
+// path of the pair and triplet kernels.  The library is compiled with
+// -ffp-contract=off (HIP's __dadd_rn/__dmul_rn are plain + and *, which the
+// default contraction would fuse into FMAs): every operation rounds as written.
 #pragma once
 #include "pcf_internal.h"
 

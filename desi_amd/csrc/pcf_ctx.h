@@ -71,4 +71,9 @@ void free_catalog(Catalog* cat);
 // Defined in triplets.hip.
 int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights);
 
+// Defined in analytic.hip.
+int run_analytic_3iso(pcf_ctx* c, pcf_catalog data, int bins, float dmax, float box, int64_t np,
+                      int ptt, int bn_ref, double* ddr, double* rrr, double* ff_out,
+                      double* ffr_out);
+
 }  // namespace pcf

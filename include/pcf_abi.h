@@ -140,6 +140,22 @@ int pcf_last_stats(const pcf_ctx* ctx, pcf_stats* out);
  * computed on the device in FP64 with 64-bit integer arithmetic. */
 int pcf_analytic_rr(pcf_ctx* ctx, int32_t bins, int64_t np, float dmax, float box, double* rr);
 
+/* Analytic random triplets of the periodic 3iso case (PCF 3iso -bpc -a):
+ * pcf_3isoBPC_analytic (cuda/pcf3isoBPC.cuh:177-375) minus its DDD, which is
+ * pcf_histogram(PCF_3ISO, PCF_XXX, bpc = 1).  From the data catalog's fine
+ * periodic pair histograms (DD2iso_forBPC, cuda/kernels/3isoBPC_k.cuh:1694)
+ * and their analytic randoms it forms ff_av / ff_av_ref (make_ff_av,
+ * make_ff_av_ref, :1984-2019) and per bin cube the analytic RRR and DDR
+ * (make_histo_analitic, :2021-2119); the reference writes DDR to both its
+ * DDR and DRR files.  np <= 0 means the catalog size; ptt and bn_ref <= 0 mean
+ * the reference's 100 and 200.  ddr and rrr receive bins^3 values in the
+ * [i][j][k] layout; ff_av (bins) and ff_av_ref (bins*bn_ref) may be NULL.
+ * FP64 distances and 64-bit bin arithmetic (the reference's FP32 distances
+ * and 32-bit 3i^2+3i+1 are not reproduced). */
+int pcf_analytic_3iso(pcf_ctx* ctx, pcf_catalog data, int32_t bins, float dmax, float box,
+                      int64_t np, int32_t ptt, int32_t bn_ref, double* ddr, double* rrr,
+                      double* ff_av, double* ff_av_ref);
+
 /* ---- asynchronous form (serving) ------------------------------------- *
  * pcf_submit enqueues a histogram job; the engine admits it when enough HBM
  * (the budget set by pcf_set_hbm_budget) and a stream slot are free, runs

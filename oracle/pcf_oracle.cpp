@@ -742,3 +742,173 @@ int pcf_oracle_cpp32_3iso(const float* a_, int64_t na, const float* b_, int64_t 
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// Analytic periodic 3iso (-bpc -a): cuda/pcf3isoBPC.cuh:177-375 with the
+// kernels DD2iso_forBPC, RR2iso_BPC_analytic, make_ff_av, make_ff_av_ref and
+// make_histo_analitic (cuda/kernels/3isoBPC_k.cuh:1694-2119,
+// cuda/kernels/2isoBPC_k.cuh:559-575).
+//   fine DD: periodic ordered pairs binned twice, at nb1 = ptt*bn and
+//     nb2 = ptt*bn_ref*bn bins with the FP32 scale (float)nb/dmax
+//     (3isoBPC_k.cuh:1724); FP64 distances here (the reference uses FP32);
+//     weights w_i*w_j summed in double.
+//   fine RR: alpha*(3i^2+3i+1), dr = dmax/nb (FP32), beta = np^2/L^3 (FP32),
+//     64-bit (the reference's int overflows at i >= 26755, defect fixed).
+//   ff_av[i] = sum_j (ri + rj)(DD/RR - 1)/ptt, ff_av_ref likewise
+//     (3isoBPC_k.cuh:1984-2019), summed in index order.
+//   RRR, DDR per bin cube: closed form when all 8 corners satisfy the triangle
+//     inequality, else a bn_ref^3 sub-cube sum (3isoBPC_k.cuh:2021-2119),
+//     summed in (u, v, w) order.  DRR := DDR (cuda/pcf3isoBPC.cuh:340).
+// Pinned by: the closed form of full cubes; the reference CPU program's
+// ff_av (cpp/3PCFisotropic/BPC, FP32 distances: agreement to the fine-bin
+// edge-flip noise); GPU parity at reduced ptt/bn_ref.  The sub-cube refinement
+// itself is formula-pinned (no runnable reference output exists for it).
+int pcf_oracle_analytic_3iso(const float* a_, int64_t na, int bins, float dmax, float box,
+                             int64_t np, int ptt, int bn_ref, int nthreads, double* ddr,
+                             double* rrr, double* ff_av_out, double* ff_av_ref_out) {
+  if (bins <= 0 || ptt <= 0 || bn_ref <= 0 || !(2.0 * dmax < box)) return -1;
+  const Pt* a = reinterpret_cast<const Pt*>(a_);
+  const int64_t nb1 = (int64_t)ptt * bins, nb2 = (int64_t)ptt * bn_ref * bins;
+  const float dsf1 = (float)nb1 / dmax, dsf2 = (float)nb2 / dmax;
+  const double ds1 = dsf1, ds2 = dsf2;
+  PairRule r = make_pair_rule(0, 1, 1, bins, dmax, box);
+  std::vector<double> DD1(nb1, 0.0), DD2(nb2, 0.0);
+  Grid g;
+  grid_geometry(a, na, nullptr, 0, (double)dmax * 1.0000001 + 1e-6, 1, (double)box, &g);
+  g.build(a, na);
+  const int nt = nthreads_or(nthreads);
+  const int64_t ncell = (int64_t)g.P[0] * g.P[1] * g.P[2];
+#pragma omp parallel num_threads(nt)
+  {
+    std::vector<double> L1(nb1, 0.0), L2(nb2, 0.0);
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t ca = 0; ca < ncell; ++ca) {
+      int cx = (int)(ca / ((int64_t)g.P[1] * g.P[2]));
+      int cy = (int)((ca / g.P[2]) % g.P[1]);
+      int cz = (int)(ca % g.P[2]);
+      for_neighbours(g, 1, cx, cy, cz, [&](int64_t cb, int kx, int ky, int kz) {
+        for (int64_t i = g.start[ca]; i < g.start[ca + 1]; ++i)
+          for (int64_t j = g.start[cb]; j < g.start[cb + 1]; ++j) {
+            double tx, ty, tz;
+            if (!axis_sq(g.pts[i].x, g.pts[j].x, kx, r.L, &tx)) continue;
+            if (!axis_sq(g.pts[i].y, g.pts[j].y, ky, r.L, &ty)) continue;
+            if (!axis_sq(g.pts[i].z, g.pts[j].z, kz, r.L, &tz)) continue;
+            const double d2 = tx + ty + tz;
+            if (!(d2 < r.R && d2 > 0)) continue;
+            const double d = std::sqrt(d2);
+            const double w = (double)g.pts[i].w * (double)g.pts[j].w;
+            const int64_t b1 = (int64_t)(d * ds1), b2 = (int64_t)(d * ds2);
+            if (b1 <= nb1 - 1) L1[b1] += w;
+            if (b2 <= nb2 - 1) L2[b2] += w;
+          }
+      });
+    }
+#pragma omp critical
+    {
+      for (int64_t k = 0; k < nb1; ++k) DD1[k] += L1[k];
+      for (int64_t k = 0; k < nb2; ++k) DD2[k] += L2[k];
+    }
+  }
+  // Fine analytic RR (cuda/pcf3isoBPC.cuh:233-238).
+  const float np2 = (float)((double)np * (double)np);
+  const float L3 = box * box * box;
+  const double beta = (double)(float)(np2 / L3);
+  auto rr_of = [&](int64_t nb, int64_t i) {
+    const double dr = (double)(float)(dmax / (float)nb);
+    const double alpha = 8 * dr * dr * dr * std::acos(0.0) * beta / 3;
+    return alpha * (double)(3 * i * i + 3 * i + 1);
+  };
+  std::vector<double> ff(bins, 0.0), ffr((size_t)bins * bn_ref, 0.0);
+  for (int i = 0; i < bins; ++i) {
+    const double ri = (double)(float)(i * dmax) / (double)bins;
+    for (int j = 0; j < ptt; ++j) {
+      const double rj = (j + 0.5) * (double)dmax / (double)nb1;
+      const int64_t q = (int64_t)i * ptt + j;
+      ff[i] += (ri + rj) * (DD1[q] / rr_of(nb1, q) - 1) / (double)ptt;
+    }
+  }
+  {
+    const double dr = (double)dmax / (double)bins;
+    const double dr_ref = dr / bn_ref;
+    const double dr_ptt_ref = (double)dmax / (double)nb2;
+    for (int i = 0; i < bins; ++i)
+      for (int j = 0; j < bn_ref; ++j)
+        for (int k = 0; k < ptt; ++k) {
+          const double ri = i * dr, rj = j * dr_ref, rk = (k + 0.5) * dr_ptt_ref;
+          const int64_t q = ((int64_t)i * bn_ref) * ptt + (int64_t)j * ptt + k;
+          ffr[(size_t)i * bn_ref + j] += (ri + rj + rk) * ((DD2[q] / rr_of(nb2, q)) - 1) / (double)ptt;
+        }
+  }
+  // Cubes (3isoBPC_k.cuh:2021-2119; constants cuda/pcf3isoBPC.cuh:240-245).
+  const double dr = (double)dmax / (double)bins, dr2 = dr * 0.5;
+  const double V = (double)(box * box);
+  const double beta3 = (double)np / V;
+  double gama = 8 * (4 * std::acos(0.0) * std::acos(0.0)) * ((double)np * beta3 * beta3);
+  gama /= V;
+  const double dr_ref = dr / bn_ref, dr_ref2 = dr_ref * 0.5;
+  const double alpha = gama * dr * dr * dr, alpha_ref = gama * dr_ref * dr_ref * dr_ref;
+  if (ff_av_out) std::memcpy(ff_av_out, ff.data(), sizeof(double) * bins);
+  if (ff_av_ref_out) std::memcpy(ff_av_ref_out, ffr.data(), sizeof(double) * bins * bn_ref);
+  if (!ddr || !rrr) return 0;  // ff_av / ff_av_ref only
+  const int64_t b3 = (int64_t)bins * bins * bins;
+  std::memset(ddr, 0, sizeof(double) * b3);
+  std::memset(rrr, 0, sizeof(double) * b3);
+  auto corners = [](double r1, double r2, double r3, double h) {
+    int v = 0;
+    for (int aa = 0; aa < 2; ++aa)
+      for (int bb = 0; bb < 2; ++bb)
+        for (int cc = 0; cc < 2; ++cc) {
+          const double s1 = r1 + (double)aa * h, s2 = r2 + (double)bb * h, s3 = r3 + (double)cc * h;
+          if (s1 + s2 >= s3 && s1 + s3 >= s2 && s2 + s3 >= s1) ++v;
+        }
+    return v;
+  };
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+  for (int64_t t = 0; t < b3; ++t) {
+    const int i = (int)(t / ((int64_t)bins * bins)), j = (int)((t / bins) % bins), k = (int)(t % bins);
+    const double ri = (double)i * dr, rj = (double)j * dr, rk = (double)k * dr;
+    const int vin = corners(ri, rj, rk, dr);
+    if (vin == 8) {
+      const double sv = alpha * (ri + dr2) * (rj + dr2) * (rk + dr2);
+      double f = 1;
+      f += ff[i] / (3 * (ri + dr2));
+      f += ff[j] / (3 * (rj + dr2));
+      f += ff[k] / (3 * (rk + dr2));
+      f *= sv;
+      rrr[t] = sv;
+      ddr[t] = f;
+    } else if (vin > 0) {
+      double S = 0.0, F = 0.0;
+      bool con = false;
+      for (int u = 0; u < bn_ref; ++u) {
+        const double ru = ri + (double)u * dr_ref;
+        for (int v = 0; v < bn_ref; ++v) {
+          const double rv = rj + (double)v * dr_ref;
+          for (int w = 0; w < bn_ref; ++w) {
+            const double rw = rk + (double)w * dr_ref;
+            if (corners(ru, rv, rw, dr_ref) == 8) {
+              const double c = (ru + dr_ref2) * (rv + dr_ref2) * (rw + dr_ref2);
+              S += c;
+              double f = 1;
+              f += ffr[(size_t)i * bn_ref + u] / (3 * (ru + dr_ref2));
+              f += ffr[(size_t)j * bn_ref + v] / (3 * (rv + dr_ref2));
+              f += ffr[(size_t)k * bn_ref + w] / (3 * (rw + dr_ref2));
+              f *= c;
+              F += f;
+              con = true;
+            }
+          }
+        }
+      }
+      if (con) {
+        rrr[t] = S * alpha_ref;
+        ddr[t] = F * alpha_ref;
+      }
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"

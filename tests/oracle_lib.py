@@ -35,6 +35,9 @@ def lib():
         L.pcf_oracle_analytic_rr.restype = None
         L.pcf_oracle_cpp32_3iso.argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32, ctypes.c_float,
                                             _i32, _f64p]
+        L.pcf_oracle_analytic_3iso.argtypes = [_f32p, _i64, _i32, ctypes.c_float, ctypes.c_float,
+                                               _i64, _i32, _i32, _i32, _f64p, _f64p, _f64p,
+                                               _f64p]
         for name in ("pcf_oracle_triplets", "pcf_oracle_triplets_brute"):
             if hasattr(L, name):
                 getattr(L, name).argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32, _i32,
@@ -106,6 +109,24 @@ def analytic_rr(bins, np_, dmax, box):
     h = np.zeros(bins, np.float64)
     lib().pcf_oracle_analytic_rr(bins, np_, dmax, box, _p(h, _f64p))
     return h
+
+
+def analytic_3iso(a, *, bins, dmax, box, np_=0, ptt=100, bn_ref=200, threads=0, cubes=True):
+    """Analytic periodic 3iso: (DDR, RRR, ff_av, ff_av_ref); cubes=False stops
+    after ff_av / ff_av_ref (DDR and RRR are then None)."""
+    a = _cat(a)
+    ddr = np.zeros(bins ** 3, np.float64) if cubes else None
+    rrr = np.zeros(bins ** 3, np.float64) if cubes else None
+    ff = np.zeros(bins, np.float64)
+    ffr = np.zeros(bins * bn_ref, np.float64)
+    rc = lib().pcf_oracle_analytic_3iso(_p(a, _f32p), len(a), bins, dmax, box,
+                                        int(np_ if np_ > 0 else len(a)), ptt, bn_ref, threads,
+                                        _p(ddr, _f64p) if cubes else None,
+                                        _p(rrr, _f64p) if cubes else None, _p(ff, _f64p),
+                                        _p(ffr, _f64p))
+    if rc != 0:
+        raise ValueError(f"oracle analytic 3iso rc={rc}")
+    return ddr, rrr, ff, ffr
 
 
 def triplets(a, b=None, *, ani=False, bpc=False, box=0.0, bins=30, dmax=80.0, threads=0,

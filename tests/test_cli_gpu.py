@@ -1,0 +1,102 @@
+"""End to end through the drop-in command line (desi_amd/bin/PCF): the files it
+writes hold the oracle's histograms (counts printed with 12 significant digits,
+so integers below 1e12 are exact), under the reference's file names."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import desi_amd as da
+import oracle_lib as O
+from conftest import GOLDEN, catalog
+
+pytestmark = pytest.mark.gpu
+
+CATS = os.path.join(GOLDEN, "catalogs")
+
+
+def _run(tmp_path, *args, data_dir=CATS):
+    res = tmp_path / "results"
+    res.mkdir(exist_ok=True)
+    env = dict(os.environ, PCF_DATA_DIR=str(data_dir), PCF_RESULTS_DIR=str(res))
+    print("cli", " ".join(args), flush=True)
+    p = subprocess.run([da.CLI_PATH, *args], capture_output=True, text=True, env=env,
+                       timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    return res
+
+
+def _load(path):
+    return np.array(open(path).read().split(), dtype=np.float64)
+
+
+def test_2iso_files(tmp_path):
+    res = _run(tmp_path, "2iso", "-n", "1000", "-f", "data_1K.dat", "-r", "rand0_1K.dat",
+               "-b", "30", "-d", "80")
+    d, r = catalog("data_1K.dat"), catalog("rand0_1K.dat")
+    box = float(np.ceil(max(d[:, :3].max(), r[:, :3].max()) + 1))
+    for name, (a, b) in (("DDiso_data_1K.dat", (d, None)), ("RRiso_rand0_1K.dat", (r, None)),
+                         ("DRiso_rand0_1K.dat", (d, r))):
+        c, _ = O.pairs(a, b, bins=30, dmax=80.0, box=box)
+        assert np.array_equal(_load(res / name), c.astype(np.float64)), name
+
+
+def test_2ani_bpc_files(tmp_path):
+    res = _run(tmp_path, "2ani", "-n", "500", "-f", "data_500.dat", "-r", "rand0_500.dat",
+               "-b", "15", "-d", "60", "-bpc", "-s", "250")
+    d, r = catalog("data_500.dat"), catalog("rand0_500.dat")
+    c, _ = O.pairs(d, r, ani=True, bpc=True, box=250.0, bins=15, dmax=60.0)
+    assert np.array_equal(_load(res / "DRani_BPC_rand0_500.dat"), c.astype(np.float64))
+
+
+def test_3iso_bpc_files(tmp_path):
+    res = _run(tmp_path, "3iso", "-n", "1000", "-f", "data_1K.dat", "-r", "rand0_1K.dat",
+               "-b", "20", "-d", "60", "-bpc", "-s", "250")
+    d, r = catalog("data_1K.dat"), catalog("rand0_1K.dat")
+    for name, (a, b) in (("DDDiso_BPC_data_1K.dat", (d, None)),
+                         ("DDRiso_BPC_rand0_1K.dat", (d, r)),
+                         ("DRRiso_BPC_rand0_1K.dat", (r, d))):
+        c, _ = O.triplets(a, b, bpc=True, box=250.0, bins=20, dmax=60.0)
+        assert np.array_equal(_load(res / name), c.astype(np.float64)), name
+
+
+def test_random_directory(tmp_path):
+    """-rd: every file of the directory is a random catalog, in sorted order,
+    each giving its own RR and DR files."""
+    rd = tmp_path / "cats"
+    (rd / "rands").mkdir(parents=True)
+    shutil.copy(os.path.join(CATS, "data_500.dat"), rd / "data_500.dat")
+    shutil.copy(os.path.join(CATS, "rand0_500.dat"), rd / "rands" / "b.dat")
+    r2 = catalog("rand0_1K.dat")[500:1000]
+    np.savetxt(rd / "rands" / "a.dat", r2, fmt="%.6f")
+    res = _run(tmp_path, "2iso", "-n", "500", "-f", "data_500.dat", "-rd", "rands/", "-b", "10",
+               "-d", "50", "-bpc", "-s", "250", data_dir=rd)
+    d = catalog("data_500.dat")
+    for fname, r in (("a.dat", np.loadtxt(rd / "rands" / "a.dat", dtype=np.float32)),
+                     ("b.dat", catalog("rand0_500.dat"))):
+        c, _ = O.pairs(r, None, bpc=True, box=250.0, bins=10, dmax=50.0)
+        assert np.array_equal(_load(res / f"RRiso_BPC_{fname}"), c.astype(np.float64))
+        c, _ = O.pairs(d, r, bpc=True, box=250.0, bins=10, dmax=50.0)
+        assert np.array_equal(_load(res / f"DRiso_BPC_{fname}"), c.astype(np.float64))
+
+
+def test_3iso_analytic_files(tmp_path, engine):
+    res = _run(tmp_path, "3iso", "-n", "5000", "-f", "data.dat", "-b", "30", "-d", "60", "-bpc",
+               "-a", "-s", "250")
+    d = catalog("data.dat")[:5000]
+    ddr = _load(res / "DDRiso_BPCanalytic_data.dat")
+    assert np.array_equal(ddr, _load(res / "DRRiso_BPCanalytic_data.dat"))
+    rrr = _load(res / "RRRiso_BPCanalytic_data.dat")
+    c = engine.upload(d)
+    try:
+        g_ddr, g_rrr, _, _ = engine.analytic_3iso(c, bins=30, dmax=60.0, box=250.0)
+        job = da.make_job(da.PCF_3ISO, da.PCF_XXX, bins=30, dmax=60.0, cat_x=c, bpc=True,
+                          box=250.0)
+        ddd, _ = engine.histogram(job)
+    finally:
+        engine.release(c)
+    np.testing.assert_allclose(ddr, g_ddr, rtol=1e-11, atol=0)
+    np.testing.assert_allclose(rrr, g_rrr, rtol=1e-11, atol=0)
+    assert np.array_equal(_load(res / "DDDiso_BPCanalytic_data.dat"), ddd.astype(np.float64))
