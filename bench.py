@@ -37,6 +37,22 @@ RMAX = 80.0
 BINS = 30
 
 
+PAIR_PROFILE = os.path.join(ROOT, "profiles", "k_pairs_mixed_2iso_N1M_box1000_rmax80.md")
+
+
+def profiled_traffic(path, kernel):
+    """HBM bytes per dispatch of `kernel` from a committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the reading of a plain
+    copy of known size), or None when the summary is absent."""
+    try:
+        for line in open(path):
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                return float(line.split(":")[-1])
+    except OSError:
+        pass
+    return None
+
+
 def run_step(e, cd, cr, precision, rebuild=True):
     tot_pairs = 0
     evals = 0
@@ -110,6 +126,37 @@ def triplet_leg(e, steps):
     }
 
 
+def analytic_leg(e):
+    """3iso -bpc -a on the config-2 data catalog: fine periodic pair histograms
+    (3,000 and 600,000 bins), ff_av / ff_av_ref and the analytic RRR / DDR bin
+    cubes with the 200^3 sub-cube refinement (pcf_analytic_3iso)."""
+    d = synth.neyman_scott(1024, 32, 5.0, 250.0, seed=11, weights="uniform")
+    cd = e.upload(d)
+    e.analytic_3iso(cd, bins=30, dmax=80.0, box=250.0)
+    t0 = time.perf_counter()
+    e.analytic_3iso(cd, bins=30, dmax=80.0, box=250.0)
+    t1 = time.perf_counter()
+    st = e.stats()
+    e.release(cd)
+    return {"workload": "3iso -bpc -a, N=32768 Neyman-Scott, box 250, rmax 80, 30 bins, "
+                        "ptt 100, bn_ref 200 (DDR = DRR and RRR; DDD is the triplet leg)",
+            "ms": (t1 - t0) * 1e3, "kernel_ms": st["kernel_ms"]}
+
+
+def copy_probe():
+    """A plain device copy of known size, the calibration run for the HBM byte
+    counters (FETCH_SIZE / WRITE_SIZE per dispatch)."""
+    import torch
+    n = 1 << 28  # 256 Mi float32 = 1 GiB
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n}),
+          flush=True)
+
+
 def cpu_baseline(d):
     """The oracle (a C++/OpenMP restatement, FP64 canonical mode) on a bounded
     sample: the DD histogram of the data points inside the sub-box [0,500)^3."""
@@ -136,7 +183,13 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--triplet-steps", type=int, default=1)
     ap.add_argument("--no-triplets", action="store_true")
+    ap.add_argument("--no-analytic", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the 1 GiB device copy used to calibrate HBM byte counters")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine runs the pair counting on a single "
               "device (multi-GPU sharding is a design note only).", file=sys.stderr)
@@ -175,6 +228,7 @@ def main():
         if p64 // args.fp64_steps != pairs // args.steps:
             raise SystemExit("plain FP64 path and mixed path disagree on the pair count")
         trip = None if args.no_triplets else triplet_leg(e, args.triplet_steps)
+        ana = None if args.no_analytic else analytic_leg(e)
 
     achieved = evals * FLOPS_PER_PAIR / (kms * 1e-3)
     out = {
@@ -194,7 +248,8 @@ def main():
                                "30 bins, DD+RR+DR, no BPC",
                    "n_data": N, "n_random": N, "box": BOX, "rmax": RMAX, "bins": BINS},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None},
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
+                     "traffic": profiled_traffic(PAIR_PROFILE, "k_pairs_mixed<0, 0, 6>")},
         "pair_kernel_ms_per_step": kms / args.steps,
         "evaluations_per_step": evals // args.steps,
         "rechecks_per_step": rech // args.steps,
@@ -204,6 +259,8 @@ def main():
     }
     if trip is not None:
         out["triplets"] = trip
+    if ana is not None:
+        out["analytic_3iso"] = ana
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(d)
     print(json.dumps(out), flush=True)

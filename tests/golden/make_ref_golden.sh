@@ -22,4 +22,14 @@ run 2iso_bpca data.dat ref
 run 2ani_normal data.dat rand0.dat ref
 run 2ani_bpc data.dat rand0.dat ref
 run 3iso_normal data_10K.dat rand0_10K.dat ref
-run 3ani_normal data_5K.dat rand0_5K.dat ref
+# cpp/3PCFisotropic/BPC computes only ff_av / ff_av_ref of the analytic path
+# (its RRR/DDR loop is commented out) and writes them to its working directory.
+CWD="$ROOT/oracle/_ref/run/a/b/c"
+rm -f "$CWD"/ff_av.dat "$CWD"/ff_av_ref.dat "$CWD"/DD.dat
+"$ROOT/oracle/run_ref.sh" 3iso_bpc data.dat ref
+mkdir -p "$OUT/3iso_bpca"
+cp "$CWD/ff_av.dat" "$CWD/ff_av_ref.dat" "$OUT/3iso_bpca/"
+echo "3iso_bpc data.dat ref (writes ff_av.dat, ff_av_ref.dat to its working directory)" \
+  > "$OUT/3iso_bpca/COMMAND"
+# cpp/3PCFanisotropic/normal writes ~50 MB histograms to its working directory;
+# its outputs are not kept as fixtures.
