@@ -126,6 +126,63 @@ def triplet_leg(e, steps):
     }
 
 
+def batch_leg(e, steps, no_tune=False):
+    """2ani over many randoms in one persistent launch: Neyman-Scott data
+    (8192 parents x 32, sigma 8, N = 262,144, box 1000) against 64 uniform
+    random catalogs with N_i = round(exp(U[ln 2e4, ln 4e5])), rmax 120,
+    30 x 30 bins, no BPC: DD, 64 RR and 64 DR = 129 histograms per step.
+    Reported: the batched step (plan reused, one graph launch) against the
+    same 129 histograms as separate pcf_histogram calls."""
+    d = synth.neyman_scott(8192, 32, 8.0, BOX, seed=21)
+    rng = np.random.default_rng(22)
+    sizes = [int(round(np.exp(rng.uniform(np.log(2e4), np.log(4e5))))) for _ in range(64)]
+    cd = e.upload(d)
+    crs = [e.upload(synth.uniform(n, BOX, seed=100 + i)) for i, n in enumerate(sizes)]
+    jobs = [da.make_job(da.PCF_2ANI, da.PCF_XX, bins=30, dmax=120.0, cat_x=cd)]
+    for cr in crs:
+        jobs.append(da.make_job(da.PCF_2ANI, da.PCF_XX, bins=30, dmax=120.0, cat_x=cr))
+        jobs.append(da.make_job(da.PCF_2ANI, da.PCF_XY, bins=30, dmax=120.0, cat_x=cd, cat_y=cr))
+    tune = None if no_tune else e.autotune(jobs)
+    t0 = time.perf_counter()
+    res = e.histogram_batch(jobs)  # plan: grids, work lists, graph capture
+    t1 = time.perf_counter()
+    first = e.stats()
+    kms = 0.0
+    evals = 0
+    for _ in range(steps):
+        res = e.histogram_batch(jobs)
+        st = e.stats()
+        assert st["graph_replay"] == 1
+        kms += st["kernel_ms"]
+        evals += st["evaluations"]
+    t2 = time.perf_counter()
+    useful = sum(int(c.sum()) for c, _ in res)
+    single_ms = 0.0
+    t3 = time.perf_counter()
+    for k, j in enumerate(jobs):
+        c, _ = e.histogram(j)
+        single_ms += e.stats()["kernel_ms"]
+        if not np.array_equal(c, res[k][0]):
+            raise SystemExit(f"batched and single histograms differ (job {k})")
+    t4 = time.perf_counter()
+    e.release(cd)
+    for cr in crs:
+        e.release(cr)
+    ms = (t2 - t1) * 1e3 / steps
+    ach = evals * FLOPS_PER_PAIR / (kms * 1e-3)
+    return {"workload": "2ani N=262144 Neyman-Scott (8192x32, sigma 8) vs 64 uniform randoms "
+                        f"(N {min(sizes)}..{max(sizes)}, total {sum(sizes)}), box 1000, rmax 120, "
+                        "30x30 bins, DD + 64 RR + 64 DR in one persistent launch",
+            "value": useful / (ms * 1e-3), "unit": "pairs/s", "ms_per_step": ms, "steps": steps,
+            "kernel_ms_per_step": kms / steps, "evaluations_per_step": evals // steps,
+            "first_call_ms": (t1 - t0) * 1e3, "chunks": first["chunks"], "autotune": tune,
+            "work_items": first["work_items"],
+            "unbatched_ms": (t4 - t3) * 1e3, "unbatched_kernel_ms": single_ms,
+            "speedup_vs_unbatched": (t4 - t3) * 1e3 / ms,
+            "roofline": {"bound": "valu", "achieved": ach / 1e12, "peak": PEAK_FP32 / 1e12,
+                         "unit": "TFLOP/s", "frac": ach / PEAK_FP32, "traffic": None}}
+
+
 def analytic_leg(e):
     """3iso -bpc -a on the config-2 data catalog: fine periodic pair histograms
     (3,000 and 600,000 bins), ff_av / ff_av_ref and the analytic RRR / DDR bin
@@ -184,6 +241,10 @@ def main():
     ap.add_argument("--triplet-steps", type=int, default=1)
     ap.add_argument("--no-triplets", action="store_true")
     ap.add_argument("--no-analytic", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (default launch configuration)")
+    ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the 1 GiB device copy used to calibrate HBM byte counters")
     args = ap.parse_args()
@@ -199,6 +260,14 @@ def main():
     r = synth.uniform(N, BOX, seed=2)
     with da.Engine(0) as e:
         cd, cr = e.upload(d), e.upload(r)
+        tuning = None
+        if not args.no_tune:
+            # One-time per workload (cached on the context, like the resident
+            # grids): cell size, LDS layout and occupancy of the pair kernel.
+            tuning = {}
+            for name, hist, y in (("XX", da.PCF_XX, -1), ("XY", da.PCF_XY, cr)):
+                tuning[name] = e.autotune(da.make_job(da.PCF_2ISO, hist, bins=BINS, dmax=RMAX,
+                                                      cat_x=cd, cat_y=y))
         for _ in range(args.warmup):
             run_step(e, cd, cr, da.PCF_PREC_MIXED)
         t0 = time.perf_counter()
@@ -229,6 +298,7 @@ def main():
             raise SystemExit("plain FP64 path and mixed path disagree on the pair count")
         trip = None if args.no_triplets else triplet_leg(e, args.triplet_steps)
         ana = None if args.no_analytic else analytic_leg(e)
+        bat = None if args.no_batch else batch_leg(e, args.batch_steps, args.no_tune)
 
     achieved = evals * FLOPS_PER_PAIR / (kms * 1e-3)
     out = {
@@ -257,10 +327,14 @@ def main():
         "fp64_plain_kernel_ms_per_step": k64 / args.fp64_steps,
         "speedup_vs_fp64_plain": ms_step64 / ms_step,
     }
+    if tuning is not None:
+        out["autotune"] = tuning
     if trip is not None:
         out["triplets"] = trip
     if ana is not None:
         out["analytic_3iso"] = ana
+    if bat is not None:
+        out["batched_2ani_64_randoms"] = bat
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(d)
     print(json.dumps(out), flush=True)

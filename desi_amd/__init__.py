@@ -29,7 +29,7 @@ EXPORTED_SYMBOLS = (
     "pcf_hist_size", "pcf_create", "pcf_destroy", "pcf_last_error", "pcf_create_error",
     "pcf_catalog_upload", "pcf_catalog_upload_device", "pcf_catalog_release",
     "pcf_catalog_size", "pcf_histogram", "pcf_last_stats", "pcf_analytic_rr",
-    "pcf_analytic_3iso",
+    "pcf_analytic_3iso", "pcf_histogram_batch", "pcf_autotune",
     "pcf_set_hbm_budget", "pcf_submit", "pcf_poll", "pcf_wait",
 )
 
@@ -39,17 +39,29 @@ class PcfJob(ctypes.Structure):
                 ("bins", ctypes.c_int32), ("dmax", ctypes.c_float), ("box", ctypes.c_float),
                 ("precision", ctypes.c_int32), ("cat_x", ctypes.c_int32),
                 ("cat_y", ctypes.c_int32), ("partitions", ctypes.c_int32),
-                ("cells_per_rmax", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+                ("cells_per_rmax", ctypes.c_int32), ("rebuild_grid", ctypes.c_int32),
+                ("lds_copies_log2", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 2)]
+
+
+class PcfTuning(ctypes.Structure):
+    _fields_ = [("cells_per_rmax", ctypes.c_int32), ("lds_copies_log2", ctypes.c_int32),
+                ("blocks_per_cu", ctypes.c_int32), ("trials", ctypes.c_int32),
+                ("best_ms", ctypes.c_double), ("default_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class PcfStats(ctypes.Structure):
     _fields_ = [("evaluations", ctypes.c_uint64), ("rechecks", ctypes.c_uint64),
                 ("work_items", ctypes.c_uint64), ("kernel_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("cells_per_rmax", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 3)]
+                ("graph_replay", ctypes.c_int32), ("chunks", ctypes.c_int32),
+                ("pairs_listed", ctypes.c_int32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class PcfError(RuntimeError):
@@ -87,6 +99,8 @@ def load_library(path=LIB_PATH):
     L.pcf_histogram.argtypes = [vp, ctypes.POINTER(PcfJob), vp, vp]
     L.pcf_last_stats.argtypes = [vp, ctypes.POINTER(PcfStats)]
     L.pcf_analytic_rr.argtypes = [vp, c_i32, c_i64, ctypes.c_float, ctypes.c_float, vp]
+    L.pcf_histogram_batch.argtypes = [vp, ctypes.POINTER(PcfJob), c_i32, vp, vp]
+    L.pcf_autotune.argtypes = [vp, ctypes.POINTER(PcfJob), c_i32, ctypes.POINTER(PcfTuning)]
     L.pcf_analytic_3iso.argtypes = [vp, c_i32, c_i32, ctypes.c_float, ctypes.c_float, c_i64,
                                     c_i32, c_i32, vp, vp, vp, vp]
     L.pcf_set_hbm_budget.argtypes = [vp, c_i64, c_i32]
@@ -103,7 +117,8 @@ def hist_size(type_, bins):
 
 
 def make_job(type_, hist, *, bins, dmax, cat_x, cat_y=-1, bpc=False, box=0.0,
-             precision=PCF_PREC_MIXED, cells_per_rmax=0, partitions=35, rebuild_grid=False):
+             precision=PCF_PREC_MIXED, cells_per_rmax=0, partitions=35, rebuild_grid=False,
+             lds_copies_log2=0, blocks_per_cu=0):
     j = PcfJob()
     j.type = int(type_)
     j.hist = int(hist)
@@ -116,7 +131,9 @@ def make_job(type_, hist, *, bins, dmax, cat_x, cat_y=-1, bpc=False, box=0.0,
     j.cat_y = int(cat_y)
     j.partitions = int(partitions)
     j.cells_per_rmax = int(cells_per_rmax)
-    j.reserved[0] = int(bool(rebuild_grid))
+    j.rebuild_grid = int(bool(rebuild_grid))
+    j.lds_copies_log2 = int(lds_copies_log2)
+    j.blocks_per_cu = int(blocks_per_cu)
     return j
 
 
@@ -180,6 +197,27 @@ class Engine:
         rr = np.zeros(bins, np.float64)
         self._check(self.lib.pcf_analytic_rr(self.ctx, bins, np_, dmax, box, rr.ctypes.data))
         return rr
+
+    def histogram_batch(self, jobs):
+        """Several pair histograms in one launch (pcf_histogram_batch): a list
+        of (counts, weights) per job."""
+        n = len(jobs)
+        arr = (PcfJob * n)(*jobs)
+        hs = hist_size(jobs[0].type, jobs[0].bins)
+        c = np.zeros(n * hs, np.uint64)
+        w = np.zeros(n * hs, np.float64)
+        self._check(self.lib.pcf_histogram_batch(self.ctx, arr, n, c.ctypes.data, w.ctypes.data))
+        return [(c[k * hs:(k + 1) * hs], w[k * hs:(k + 1) * hs]) for k in range(n)]
+
+    def autotune(self, jobs):
+        """Tune the pair kernels for one job or a batch (pcf_autotune); the
+        choice is cached on the context and used by later calls."""
+        if isinstance(jobs, PcfJob):
+            jobs = [jobs]
+        arr = (PcfJob * len(jobs))(*jobs)
+        t = PcfTuning()
+        self._check(self.lib.pcf_autotune(self.ctx, arr, len(jobs), ctypes.byref(t)))
+        return t.as_dict()
 
     def analytic_3iso(self, cat, bins, dmax, box, np_=0, ptt=0, bn_ref=0):
         """Analytic DDR (= DRR) and RRR of the periodic 3iso case plus the

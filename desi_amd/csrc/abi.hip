@@ -314,18 +314,6 @@ void free_catalog(Catalog* cat) {
 // ---------------------------------------------------------------------------
 namespace {
 
-int lds_copies(int nrows, int bytes_per_row_copy, int budget) {
-  int c = 64;
-  while (c > 1 && nrows * c * bytes_per_row_copy > budget) c >>= 1;
-  return c;
-}
-
-int ilog2(int v) {
-  int s = 0;
-  while ((1 << s) < v) ++s;
-  return s;
-}
-
 int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights) {
   const bool auto_ = job->hist == PCF_XX;
   Catalog* X = get_cat(c, job->cat_x);
@@ -358,7 +346,7 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   if (choose_geometry(X, auto_ ? nullptr : Y, job->dmax, bpc, L, job->cells_per_rmax, &g, K,
                       &reach, &m) != 0)
     return fail(c, PCF_ERR_UNSUPPORTED, "geometry");
-  const bool rebuild = job->reserved[0] != 0;
+  const bool rebuild = job->rebuild_grid != 0;
   int rc = PCF_OK;
   HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
   DevGrid* gx = get_grid(c, X, g, rebuild, &rc);
@@ -440,10 +428,9 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   pl.L = L * sigma;
   pl.bins = bins;
   pl.nflat = nflat;
-  const int lds_budget = 48 * 1024;
-  const int rows = nflat + 2;
-  const int per = 4 + (weighted ? 8 : 0);
-  pl.cshift = ilog2(lds_copies(rows, per, lds_budget));
+  const PairShape sh =
+      pair_shape(nflat, bins, weighted, plain, job->lds_copies_log2, job->blocks_per_cu);
+  pl.cshift = sh.cshift;
   pl.hist = d_hist;
   pl.whist = d_w;
   pl.stats = d_stats;
@@ -455,12 +442,7 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   pl.Lx = (double)L;
   pl.le = rules.le;
   pl.zero = rules.zero;
-  const size_t lds_bytes = (plain ? 0 : (size_t)kMixedTableMax) +
-                           (size_t)rows * (1 << pl.cshift) * per + (size_t)(bins + 2) * 8;
-  const int threads = plain ? kBlock : kMixedBlock;
-  int per_cu = (int)std::max<size_t>(1, (160 * 1024) / (lds_bytes + 512));
-  per_cu = std::min(per_cu, 2048 / threads);
-  pl.nblocks = std::max(1, c->ncu * per_cu);
+  pl.nblocks = std::max(1, c->ncu * sh.per_cu);
   if (nitems > 0) {
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     HIPCHK(c, launch_pairs(pl, c->stream));
@@ -561,6 +543,8 @@ void pcf_destroy(pcf_ctx* c) {
   destroy_engine(c->engine);
   c->engine = nullptr;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  destroy_batch_plan(c->batch);
+  c->batch = nullptr;
   for (auto& cat : c->cats)
     if (cat) free_catalog(cat.get());
   for (DevBuf* b : {&c->scratch_cells, &c->scratch_perm, &c->wl_counts, &c->wl_items,
@@ -583,6 +567,7 @@ static int upload_common(pcf_ctx* c, const void* src, int64_t n, hipMemcpyKind k
   HIPCHK(c, hipSetDevice(c->device));
   auto cat = std::make_unique<Catalog>();
   cat->n = n;
+  cat->serial = c->next_serial++;
   HIPCHK(c, hipMalloc(&cat->raw, sizeof(float4) * (n > 0 ? n : 1)));
   if (n > 0) {
     hipError_t e = hipMemcpyAsync(cat->raw, src, sizeof(float4) * n, kind, c->stream);
@@ -641,7 +626,12 @@ int pcf_histogram(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weig
     case PCF_2ANI:
       if (job->hist != PCF_XX && job->hist != PCF_XY)
         return fail(c, PCF_ERR_ARG, "pair types take PCF_XX or PCF_XY");
-      return run_pairs(c, job, counts, weights);
+      {
+        pcf_job tuned = *job;
+        if (get_cat(c, job->cat_x) && (job->hist == PCF_XX || get_cat(c, job->cat_y)))
+          apply_tuning(c, &tuned, 1);
+        return run_pairs(c, &tuned, counts, weights);
+      }
     case PCF_3ISO:
     case PCF_3ANI:
       if (job->hist != PCF_XXX && job->hist != PCF_XXY)

@@ -1,0 +1,442 @@
+// batch.hip -- many pair histograms in one persistent launch, captured in a
+// HIP graph (pcf_histogram_batch).
+//
+// The reference's many-random drivers launch XX and XY once per random file,
+// each over its own node grid (cuda/pcf2ani.cuh:90-98, cuda/pcf2iso.cuh:95-105);
+// a 64-catalog run is 129 launches with uneven sizes and a device-wide sync.
+// Here every catalog of the batch is gridded on one geometry, the work lists of
+// all histograms are concatenated and cut into job-pure chunks, and one
+// persistent k_pairs_mixed<..., BATCH = 1> grid walks them (workgroups take
+// chunks, waves take items, each chunk ends with a flush of the workgroup's LDS
+// histogram into its job's rows).  The plan (grids, lists, chunks, job table,
+// device buffers) is kept on the context; a repeated call with the same jobs
+// over the same resident catalogs replays the captured graph (scaled copies,
+// histogram clear, counting kernel) without re-planning.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "pcf_ctx.h"
+
+namespace pcf {
+
+void make_rules(int type, int bpc, int auto_, int bins, float dmax, float box, BinRules* r);
+hipError_t build_batch_worklists(const std::vector<WorkListArgs>& args, DevBuf& counts,
+                                 DevBuf& items, std::vector<int>* job_offsets, hipStream_t s);
+
+struct BatchPlan {
+  // signature
+  std::vector<pcf_job> jobs;
+  std::vector<uint64_t> serials;      // catalog serial per job side (x, y)
+  std::vector<const void*> grid_pts;  // grid point arrays the plan was built on
+  // geometry / rules
+  GridGeom g{};
+  int nflat = 0, bins = 0, ani = 0, weighted = 0, cshift = 0, nt = 0, nblocks = 0, m = 0;
+  float inv_h = 1.f, sigma = 1.f;
+  BinRules rules{};
+  int total_items = 0;
+  std::vector<int> job_offsets;
+  std::vector<BatchChunk> chunks;
+  std::vector<int> mult;  // 2 for auto histograms (unordered pairs counted once)
+  // scaled copies: one per distinct catalog
+  std::vector<const float4*> scale_src;
+  std::vector<int64_t> scale_n;
+  std::vector<int64_t> scale_off;
+  // device memory owned by the plan
+  DevBuf counts, items, jobs_d, chunks_d, table_d, rules_d, hist_d, scaled;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t evals_per_run = 0;
+  ~BatchPlan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (DevBuf* b : {&counts, &items, &jobs_d, &chunks_d, &table_d, &rules_d, &hist_d, &scaled})
+      b->release();
+  }
+};
+
+void destroy_batch_plan(BatchPlan* p) { delete p; }
+
+namespace {
+
+bool same_job(const pcf_job& a, const pcf_job& b) {
+  return a.type == b.type && a.hist == b.hist && a.bpc == b.bpc && a.bins == b.bins &&
+         a.dmax == b.dmax && a.box == b.box && a.precision == b.precision &&
+         a.cat_x == b.cat_x && a.cat_y == b.cat_y && a.cells_per_rmax == b.cells_per_rmax &&
+         a.lds_copies_log2 == b.lds_copies_log2 && a.blocks_per_cu == b.blocks_per_cu;
+}
+
+// Graph body: scaled copies, histogram + counter clear, the batched kernel.
+// prepare: only set the kernel attributes (not allowed during capture).
+hipError_t enqueue_plan(pcf_ctx* c, BatchPlan* p, size_t njobs, bool prepare) {
+  hipError_t e;
+  float4* sc = p->scaled.as<float4>();
+  const size_t hist_bytes = sizeof(unsigned long long) * p->nflat * njobs;
+  const size_t w_bytes = sizeof(double) * p->nflat * njobs;
+  if (!prepare) {
+    for (size_t k = 0; k < p->scale_src.size(); ++k)
+      if ((e = launch_scale_points(p->scale_src[k], p->scale_n[k], p->sigma,
+                                   sc + p->scale_off[k], c->stream)) != hipSuccess)
+        return e;
+    if ((e = launch_zero(p->hist_d.p, hist_bytes + w_bytes + 64, c->stream)) != hipSuccess)
+      return e;
+  }
+  unsigned long long* d_stats =
+      reinterpret_cast<unsigned long long*>(p->hist_d.as<char>() + hist_bytes + w_bytes);
+  PairLaunch pl{};
+  pl.items = p->items.as<PairItem>();
+  pl.nitems = p->total_items;
+  pl.counter = reinterpret_cast<int*>(d_stats + 4);
+  pl.table = p->table_d.as<uint8_t>();
+  pl.nt = p->nt;
+  pl.inv_h = p->inv_h;
+  pl.L = (float)p->rules.L * p->sigma;
+  pl.bins = p->bins;
+  pl.nflat = p->nflat;
+  pl.cshift = p->cshift;
+  pl.hist = p->hist_d.as<unsigned long long>();
+  pl.whist = reinterpret_cast<double*>(p->hist_d.as<char>() + hist_bytes);
+  pl.stats = d_stats;
+  pl.rules = p->rules_d.as<BinRules>();
+  pl.ani = p->ani;
+  pl.weighted = p->weighted;
+  pl.plain64 = 0;
+  pl.nblocks = p->nblocks;
+  pl.R = p->rules.R;
+  pl.Lx = p->rules.L;
+  pl.le = p->rules.le;
+  pl.zero = p->rules.zero;
+  pl.batch = 1;
+  pl.jobs = p->jobs_d.as<BatchJob>();
+  pl.chunks = p->chunks_d.as<BatchChunk>();
+  pl.nchunks = (int)p->chunks.size();
+  pl.prepare_only = prepare ? 1 : 0;
+  if (pl.nchunks == 0) return hipSuccess;
+  return launch_pairs(pl, c->stream);
+}
+
+// Build (or rebuild) the plan of a batch.  Returns PCF_OK or an error code.
+int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p) {
+  const pcf_job& j0 = jobs[0];
+  const bool ani = j0.type == PCF_2ANI;
+  const int bins = j0.bins;
+  const int bpc = j0.bpc ? 1 : 0;
+  const float L = j0.box;
+  // Distinct catalogs and the joint bounds / occupancy used for the geometry.
+  std::vector<pcf_catalog> ids;
+  auto add_id = [&](pcf_catalog id) {
+    if (std::find(ids.begin(), ids.end(), id) == ids.end()) ids.push_back(id);
+  };
+  for (int k = 0; k < njobs; ++k) {
+    add_id(jobs[k].cat_x);
+    if (jobs[k].hist == PCF_XY) add_id(jobs[k].cat_y);
+  }
+  Catalog U;
+  bool any = false;
+  double nsum = 0;
+  float maxabs = 0;
+  bool all_w1 = true;
+  for (pcf_catalog id : ids) {
+    Catalog* X = get_cat(c, id);
+    nsum += (double)X->n;
+    maxabs = std::max(maxabs, X->maxabs);
+    all_w1 = all_w1 && X->all_w1;
+    if (!X->n) continue;
+    for (int a = 0; a < 3; ++a) {
+      U.lo[a] = any ? std::min(U.lo[a], X->lo[a]) : X->lo[a];
+      U.hi[a] = any ? std::max(U.hi[a], X->hi[a]) : X->hi[a];
+    }
+    any = true;
+  }
+  U.n = any ? (int64_t)std::llround(nsum / (double)ids.size()) : 0;
+  int K[3];
+  float reach;
+  if (choose_geometry(&U, nullptr, j0.dmax, bpc, L, j0.cells_per_rmax, &p->g, K, &reach, &p->m) !=
+      0)
+    return fail(c, PCF_ERR_UNSUPPORTED, "geometry");
+  // Grids (resident, cached per catalog and geometry).
+  int rc = PCF_OK;
+  std::vector<DevGrid*> grid_of(c->cats.size(), nullptr);
+  for (pcf_catalog id : ids) {
+    bool rebuild = false;
+    for (int k = 0; k < njobs; ++k)
+      if ((jobs[k].cat_x == id || (jobs[k].hist == PCF_XY && jobs[k].cat_y == id)) &&
+          jobs[k].rebuild_grid)
+        rebuild = true;
+    DevGrid* gr = get_grid(c, get_cat(c, id), p->g, rebuild, &rc);
+    if (!gr) return rc;
+    grid_of[id] = gr;
+  }
+  // Rules: one table for the batch.  Near-zero table cells are marked
+  // ambiguous when any job excludes zero separations; each job's exact path
+  // applies its own zero rule.
+  BinRules tab_rules;
+  make_rules(j0.type, bpc, 1, bins, j0.dmax, L, &tab_rules);
+  make_rules(j0.type, bpc, 1, bins, j0.dmax, L, &p->rules);
+  const double A = bpc ? (double)maxabs + (double)L : 0.0;
+  std::vector<uint8_t> table;
+  if (build_table(tab_rules, A, kMixedTableMax, &table, &p->inv_h, &p->nt) != 0)
+    return fail(c, PCF_ERR_UNSUPPORTED, "bin table");
+  p->sigma = std::sqrt(p->inv_h);
+  p->bins = bins;
+  p->ani = ani ? 1 : 0;
+  p->nflat = ani ? bins * bins : bins;
+  p->weighted = all_w1 ? 0 : 1;
+  const PairShape sh = pair_shape(p->nflat, bins, p->weighted != 0, false, j0.lds_copies_log2,
+                                  j0.blocks_per_cu);
+  p->cshift = sh.cshift;
+  p->nblocks = std::max(1, c->ncu * sh.per_cu);
+
+  // Scaled copies of every distinct catalog, one buffer.
+  p->scale_src.clear();
+  p->scale_n.clear();
+  p->scale_off.clear();
+  std::vector<int64_t> off_of(c->cats.size(), 0);
+  int64_t tot = 0;
+  for (pcf_catalog id : ids) {
+    p->scale_src.push_back(grid_of[id]->pts);
+    p->scale_n.push_back(grid_of[id]->n);
+    p->scale_off.push_back(tot);
+    off_of[id] = tot;
+    tot += grid_of[id]->n;
+  }
+  HIPCHK(c, p->scaled.ensure(sizeof(float4) * (size_t)std::max<int64_t>(tot, 1)));
+
+  // Work lists of all jobs, concatenated.
+  std::vector<WorkListArgs> wa(njobs);
+  for (int k = 0; k < njobs; ++k) {
+    const bool auto_ = jobs[k].hist == PCF_XX;
+    DevGrid* gx = grid_of[jobs[k].cat_x];
+    DevGrid* gy = auto_ ? gx : grid_of[jobs[k].cat_y];
+    wa[k].x_cell_start = gx->cell_start;
+    wa[k].y_cell_start = gy->cell_start;
+    wa[k].g = p->g;
+    for (int a = 0; a < 3; ++a) wa[k].K[a] = K[a];
+    wa[k].reach = reach;
+    wa[k].auto_ = auto_ ? 1 : 0;
+    wa[k].cylinder = ani ? 1 : 0;
+    wa[k].map = nullptr;
+  }
+  HIPCHK(c, build_batch_worklists(wa, p->counts, p->items, &p->job_offsets, c->stream));
+  p->total_items = p->job_offsets[njobs];
+
+  // Chunks: job-pure runs of items, about 8 per workgroup of the grid.
+  const int target = std::max(64, std::min(4096, p->total_items / std::max(1, 8 * p->nblocks)));
+  p->chunks.clear();
+  for (int k = 0; k < njobs; ++k)
+    for (int b = p->job_offsets[k]; b < p->job_offsets[k + 1]; b += target)
+      p->chunks.push_back({b, std::min(b + target, p->job_offsets[k + 1]), k});
+
+  // Job table.
+  std::vector<BatchJob> jd(njobs);
+  p->mult.assign(njobs, 1);
+  float4* sc = p->scaled.as<float4>();
+  for (int k = 0; k < njobs; ++k) {
+    const bool auto_ = jobs[k].hist == PCF_XX;
+    const pcf_catalog yid = auto_ ? jobs[k].cat_x : jobs[k].cat_y;
+    BinRules r;
+    make_rules(j0.type, bpc, auto_ ? 1 : 0, bins, j0.dmax, L, &r);
+    jd[k].xs = sc + off_of[jobs[k].cat_x];
+    jd[k].ys = sc + off_of[yid];
+    jd[k].xo = grid_of[jobs[k].cat_x]->pts;
+    jd[k].yo = grid_of[yid]->pts;
+    jd[k].hoff = k * p->nflat;
+    jd[k].zero = r.zero;
+    p->mult[k] = auto_ ? 2 : 1;
+  }
+  HIPCHK(c, p->jobs_d.ensure(sizeof(BatchJob) * njobs));
+  HIPCHK(c, p->chunks_d.ensure(sizeof(BatchChunk) * std::max<size_t>(1, p->chunks.size())));
+  HIPCHK(c, p->table_d.ensure(p->nt));
+  HIPCHK(c, p->rules_d.ensure(sizeof(BinRules)));
+  HIPCHK(c, p->hist_d.ensure((sizeof(unsigned long long) + sizeof(double)) * p->nflat * njobs +
+                             64));
+  HIPCHK(c, hipMemcpyAsync(p->jobs_d.p, jd.data(), sizeof(BatchJob) * njobs,
+                           hipMemcpyHostToDevice, c->stream));
+  if (!p->chunks.empty())
+    HIPCHK(c, hipMemcpyAsync(p->chunks_d.p, p->chunks.data(),
+                             sizeof(BatchChunk) * p->chunks.size(), hipMemcpyHostToDevice,
+                             c->stream));
+  HIPCHK(c, hipMemcpyAsync(p->table_d.p, table.data(), p->nt, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(p->rules_d.p, &p->rules, sizeof(BinRules), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));  // host vectors above go out of scope
+
+  // Signature.
+  p->jobs.assign(jobs, jobs + njobs);
+  p->serials.clear();
+  p->grid_pts.clear();
+  for (pcf_catalog id : ids) {
+    p->serials.push_back(get_cat(c, id)->serial);
+    p->grid_pts.push_back(grid_of[id]->pts);
+  }
+
+  // Capture the run into a graph.
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->exec = nullptr;
+  p->graph = nullptr;
+  HIPCHK(c, enqueue_plan(c, p, (size_t)njobs, true));
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = enqueue_plan(c, p, (size_t)njobs, false);
+  hipGraph_t graph = nullptr;
+  hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    return hipfail(c, e != hipSuccess ? e : e2, "batch graph capture");
+  }
+  p->graph = graph;
+  HIPCHK(c, hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+  return PCF_OK;
+}
+
+// The plan still describes these jobs over the same resident data.
+bool plan_valid(pcf_ctx* c, const BatchPlan* p, const pcf_job* jobs, int njobs) {
+  if (!p || !p->exec || (int)p->jobs.size() != njobs) return false;
+  for (int k = 0; k < njobs; ++k) {
+    if (!same_job(p->jobs[k], jobs[k]) || jobs[k].rebuild_grid) return false;
+  }
+  std::vector<pcf_catalog> ids;
+  for (int k = 0; k < njobs; ++k) {
+    for (pcf_catalog id : {jobs[k].cat_x, jobs[k].hist == PCF_XY ? jobs[k].cat_y : jobs[k].cat_x})
+      if (std::find(ids.begin(), ids.end(), id) == ids.end()) ids.push_back(id);
+  }
+  if (ids.size() != p->serials.size()) return false;
+  for (size_t i = 0; i < ids.size(); ++i) {
+    Catalog* X = get_cat(c, ids[i]);
+    if (!X || X->serial != p->serials[i]) return false;
+    bool found = false;
+    for (auto& gr : X->grids)
+      if (gr->valid && gr->g.same(p->g) && gr->pts == p->grid_pts[i]) found = true;
+    if (!found) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+int run_pairs_batch(pcf_ctx* c, const pcf_job* jobs_in, int njobs, uint64_t* counts,
+                    double* weights) {
+  if (njobs <= 0 || !jobs_in) return fail(c, PCF_ERR_ARG, "empty batch");
+  std::vector<pcf_job> tuned(jobs_in, jobs_in + njobs);
+  const pcf_job* jobs = tuned.data();
+  const pcf_job& j0 = jobs[0];
+  if (j0.type != PCF_2ISO && j0.type != PCF_2ANI)
+    return fail(c, PCF_ERR_UNSUPPORTED, "batched launches are for pair histograms");
+  for (int k = 0; k < njobs; ++k) {
+    const pcf_job& j = jobs[k];
+    if (j.type != j0.type || j.bins != j0.bins || j.dmax != j0.dmax || j.bpc != j0.bpc ||
+        (j0.bpc && j.box != j0.box) || j.precision != j0.precision ||
+        j.cells_per_rmax != j0.cells_per_rmax || j.lds_copies_log2 != j0.lds_copies_log2 ||
+        j.blocks_per_cu != j0.blocks_per_cu)
+      return fail(c, PCF_ERR_ARG, "jobs of a batch must share type, bins, dmax, box, precision");
+    if (j.hist != PCF_XX && j.hist != PCF_XY) return fail(c, PCF_ERR_ARG, "pair histogram kind");
+    if (!get_cat(c, j.cat_x) || (j.hist == PCF_XY && !get_cat(c, j.cat_y)))
+      return fail(c, PCF_ERR_ARG, "unknown catalog id");
+  }
+  apply_tuning(c, tuned.data(), njobs);
+  const int64_t hs = pcf_hist_size(j0.type, j0.bins);
+  // The plain FP64 path and out-of-range arguments go through the single-job
+  // driver (same kernels, one launch per job).
+  bool single = j0.precision == PCF_PREC_FP64;
+  if (!single) {
+    // Argument checks of the single-job driver (range, box) on each job.
+    const int bins = j0.bins;
+    if (bins <= 0 || bins > 253 || (j0.type == PCF_2ANI && bins > 120))
+      return fail(c, PCF_ERR_ARG, "bins out of range");
+    if (!(j0.dmax > 0) || !std::isfinite(j0.dmax)) return fail(c, PCF_ERR_ARG, "dmax must be > 0");
+    if (j0.bpc) {
+      if (!(j0.box > 0)) return fail(c, PCF_ERR_ARG, "box side required for periodic boundaries");
+      if (!(2.0 * j0.dmax < j0.box))
+        return fail(c, PCF_ERR_UNSUPPORTED, "periodic boundaries need 2*dmax < box side");
+      for (int k = 0; k < njobs; ++k)
+        for (pcf_catalog id : {jobs[k].cat_x, jobs[k].hist == PCF_XY ? jobs[k].cat_y : jobs[k].cat_x}) {
+          Catalog* X = get_cat(c, id);
+          for (int a = 0; a < 3; ++a)
+            if (X->n && (X->lo[a] < 0.f || !(X->hi[a] < j0.box)))
+              return fail(c, PCF_ERR_RANGE, "coordinates outside [0, box) with periodic boundaries");
+        }
+    }
+  }
+  if (single) {
+    pcf_stats agg{};
+    for (int k = 0; k < njobs; ++k) {
+      int rc = pcf_histogram(c, &jobs[k], counts ? counts + hs * k : nullptr,
+                             weights ? weights + hs * k : nullptr);
+      if (rc != PCF_OK) return rc;
+      agg.evaluations += c->last.evaluations;
+      agg.rechecks += c->last.rechecks;
+      agg.work_items += c->last.work_items;
+      agg.kernel_ms += c->last.kernel_ms;
+      agg.total_ms += c->last.total_ms;
+      agg.cells_per_rmax = c->last.cells_per_rmax;
+    }
+    c->last = agg;
+    return PCF_OK;
+  }
+
+  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  const bool replay = plan_valid(c, c->batch, jobs, njobs);
+  if (!replay) {
+    if (!c->batch) c->batch = new BatchPlan();
+    int rc = make_plan(c, jobs, njobs, c->batch);
+    if (rc != PCF_OK) {
+      delete c->batch;
+      c->batch = nullptr;
+      return rc;
+    }
+  }
+  BatchPlan* p = c->batch;
+  HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  HIPCHK(c, hipGraphLaunch(p->exec, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+  const size_t nf = (size_t)p->nflat * njobs;
+  std::vector<unsigned long long> hc(nf + 8);
+  std::vector<double> hw(nf);
+  const size_t hist_bytes = sizeof(unsigned long long) * nf;
+  HIPCHK(c, hipMemcpyAsync(hc.data(), p->hist_d.p, hist_bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(hw.data(), p->hist_d.as<char>() + hist_bytes, sizeof(double) * nf,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(hc.data() + nf, p->hist_d.as<char>() + hist_bytes + sizeof(double) * nf,
+                           4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < njobs; ++k) {
+    const uint64_t mult = (uint64_t)p->mult[k];
+    for (int b = 0; b < p->nflat; ++b) {
+      const size_t q = (size_t)k * p->nflat + b;
+      if (counts) counts[(size_t)k * hs + b] = (uint64_t)hc[q] * mult;
+      if (weights)
+        weights[(size_t)k * hs + b] =
+            p->weighted ? hw[q] * (double)mult : (double)((uint64_t)hc[q] * mult);
+    }
+  }
+  pcf_stats st{};
+  st.evaluations = hc[nf];
+  st.rechecks = hc[nf + 1];
+  st.work_items = (uint64_t)p->total_items;
+  float ms = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));
+  st.kernel_ms = ms;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[3]));
+  st.total_ms = ms;
+  st.cells_per_rmax = p->m;
+  st.graph_replay = replay ? 1 : 0;
+  st.chunks = (int32_t)p->chunks.size();
+  c->last = st;
+  return PCF_OK;
+}
+
+}  // namespace pcf
+
+extern "C" int pcf_histogram_batch(pcf_ctx* c, const pcf_job* jobs, int32_t njobs,
+                                   uint64_t* counts, double* weights) {
+  using namespace pcf;
+  if (!c) return PCF_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  c->err.clear();
+  return run_pairs_batch(c, jobs, njobs, counts, weights);
+}

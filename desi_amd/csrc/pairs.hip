@@ -23,6 +23,8 @@
 //   * counts go to a per-lane-column LDS histogram (no bank conflicts among the
 //     lanes of a wave), merged into the global uint64 / FP64 histogram once per
 //     workgroup of a persistent grid that pulls items from an atomic counter.
+#include <vector>
+
 #include "pcf_internal.h"
 #include "worklist.h"
 #include "exact.h"
@@ -87,13 +89,8 @@ struct KArgs {
   double Lx;    // box side, unscaled
   int le;
   int zero;
+  int claim;    // work items a wave claims per atomic (mixed kernel)
 };
-
-__device__ __forceinline__ int wave_fetch(int* counter, int lane) {
-  int it = 0;
-  if (lane == 0) it = atomicAdd(counter, 1);
-  return __builtin_amdgcn_readfirstlane(it);
-}
 
 // FP32 bulk + FP64 edge re-evaluation.
 //
@@ -156,14 +153,22 @@ constexpr int kMaxTable = kMixedTableMax;
 
 // xp/yp: points scaled by sigma (bulk arithmetic); xp_orig/yp_orig: the same
 // points unscaled (exact path, weights).
-template <int ANI, int WEIGHTED, int CS>
+//
+// BATCH = 1: many histograms in one persistent launch.  The work list is the
+// concatenation of the jobs' lists, cut into chunks that never straddle two
+// jobs; a workgroup takes a chunk (global counter), its waves take the chunk's
+// items (LDS counter), and at the end of the chunk the workgroup flushes its
+// LDS histogram into that job's rows and clears it.  Point arrays and the zero
+// rule come from the chunk's job descriptor; xp..yp_orig are unused.
+template <int ANI, int WEIGHTED, int CS, int BATCH>
 __global__ __launch_bounds__(kBlockMixed, 4) void k_pairs_mixed(
     const float4* __restrict__ xp, const float4* __restrict__ yp,
     const float4* __restrict__ xp_orig, const float4* __restrict__ yp_orig,
     const PairItem* __restrict__ items, int* __restrict__ counter,
     const uint8_t* __restrict__ table, const BinRules* __restrict__ rules,
     unsigned long long* __restrict__ hist, double* __restrict__ whist,
-    unsigned long long* __restrict__ stats, KArgs k) {
+    unsigned long long* __restrict__ stats, KArgs k, const BatchJob* __restrict__ jobs,
+    const BatchChunk* __restrict__ chunks) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ uint8_t tab[kMaxTable];
   const int nrows = k.nflat + 2;
@@ -190,9 +195,9 @@ __global__ __launch_bounds__(kBlockMixed, 4) void k_pairs_mixed(
   unsigned long long evals = 0;
   unsigned long long nre = 0;
 
-  for (;;) {
-    const int it = wave_fetch(counter, lane);
-    if (it >= k.nitems) break;
+  auto run_item = [&](const int it, const float4* __restrict__ xp,
+                      const float4* __restrict__ yp, const float4* __restrict__ xp_orig,
+                      const float4* __restrict__ yp_orig, const int zero) {
     const int i0 = __builtin_amdgcn_readfirstlane(items[it].i0);
     const int ni = __builtin_amdgcn_readfirstlane(items[it].ni);
     const int j0 = __builtin_amdgcn_readfirstlane(items[it].j0);
@@ -227,7 +232,7 @@ __global__ __launch_bounds__(kBlockMixed, 4) void k_pairs_mixed(
       if (amb) {
         nre += __popcll(amb);
         if (r == amb_row)
-          r = exact_row<ANI>(pi, yp_orig[j], img, sT, k.bins, k.R, k.le, k.zero, k.Lx, out_row);
+          r = exact_row<ANI>(pi, yp_orig[j], img, sT, k.bins, k.R, k.le, zero, k.Lx, out_row);
       }
     };
     auto quad = [&](int j, const float4 q0, const float4 q1, const float4 q2, const float4 q3) {
@@ -269,18 +274,63 @@ __global__ __launch_bounds__(kBlockMixed, 4) void k_pairs_mixed(
       fix(r, j);
       deposit(r, q.w);
     }
-  }
-  __syncthreads();
-  for (int r = threadIdx.x; r < k.nflat; r += kBlockMixed) {
-    unsigned long long c = 0;
-    double w = 0.0;
-    for (int cpy = 0; cpy < ncopy; ++cpy) {
-      c += hc[(r << CS) + cpy];
-      if (WEIGHTED) w += hw[(r << CS) + cpy];
+  };
+  // Sum of the LDS copies of each row into rows [hoff, hoff + nflat) of the
+  // global histogram; `clear` zeroes the LDS rows for the next chunk.
+  auto flush = [&](int hoff, bool clear) {
+    for (int r = threadIdx.x; r < nrows; r += kBlockMixed) {
+      unsigned long long c = 0;
+      double w = 0.0;
+      for (int cpy = 0; cpy < ncopy; ++cpy) {
+        c += hc[(r << CS) + cpy];
+        if (WEIGHTED) w += hw[(r << CS) + cpy];
+        if (clear) {
+          hc[(r << CS) + cpy] = 0u;
+          if (WEIGHTED) hw[(r << CS) + cpy] = 0.0;
+        }
+      }
+      if (c && r < k.nflat) {
+        atomicAdd(&hist[hoff + r], c);
+        if (WEIGHTED) atomicAdd(&whist[hoff + r], w);
+      }
     }
-    if (c) {
-      atomicAdd(&hist[r], c);
-      if (WEIGHTED) atomicAdd(&whist[r], w);
+  };
+
+  if constexpr (!BATCH) {
+    // A wave claims k.claim consecutive items per atomic: the claim's round
+    // trip to L2 (the wave waits on it) is paid once per k.claim items.
+    for (;;) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(counter, k.claim);
+      base = __builtin_amdgcn_readfirstlane(base);
+      if (base >= k.nitems) break;
+      const int end = min(base + k.claim, k.nitems);
+      for (int it = base; it < end; ++it) run_item(it, xp, yp, xp_orig, yp_orig, k.zero);
+    }
+    __syncthreads();
+    flush(0, false);
+  } else {
+    __shared__ int s_chunk, s_next;
+    for (;;) {
+      if (threadIdx.x == 0) {
+        s_chunk = atomicAdd(counter, 1);
+        s_next = 0;
+      }
+      __syncthreads();
+      const int c = s_chunk;
+      if (c >= k.nitems) break;  // nitems = number of chunks in batch mode
+      const int cb = chunks[c].begin, ce = chunks[c].end;
+      const BatchJob jd = jobs[chunks[c].job];
+      for (;;) {
+        int it = 0;
+        if (lane == 0) it = cb + atomicAdd(&s_next, 1);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= ce) break;
+        run_item(it, jd.xs, jd.ys, jd.xo, jd.yo, jd.zero);
+      }
+      __syncthreads();
+      flush(jd.hoff, true);
+      __syncthreads();
     }
   }
   if (lane == 0) atomicAdd(&stats[0], evals);
@@ -336,8 +386,14 @@ __global__ __launch_bounds__(kBlock) void k_pairs_fp64(
   const int lane = threadIdx.x & (kWave - 1);
   const int copy = lane & (ncopy - 1);
   unsigned long long evals = 0;
-  for (;;) {
-    const int it = wave_fetch(counter, lane);
+  int it = 0, claim_end = 0;
+  for (;; ++it) {
+    if (it >= claim_end) {  // claim k.claim items per atomic, as the mixed kernel
+      int base = 0;
+      if (lane == 0) base = atomicAdd(counter, k.claim);
+      it = __builtin_amdgcn_readfirstlane(base);
+      claim_end = min(it + k.claim, k.nitems);
+    }
     if (it >= k.nitems) break;
     const int i0 = __builtin_amdgcn_readfirstlane(items[it].i0);
     const int ni = __builtin_amdgcn_readfirstlane(items[it].ni);
@@ -439,9 +495,55 @@ hipError_t build_pair_worklist(const WorkListArgs& a, DevBuf& counts, DevBuf& it
   return hipGetLastError();
 }
 
+__global__ void k_gather_offsets(const int* __restrict__ off, int64_t stride, int njobs,
+                                 int* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k <= njobs) out[k] = off[(int64_t)k * stride];
+}
+
+// Work lists of several jobs over one geometry, concatenated in job order:
+// one count pass per job, one scan over all (job, cell) counts, one fill pass
+// per job.  job_offsets[k] is the first item of job k, job_offsets[njobs] the
+// total (one synchronous read of njobs + 1 integers).
+hipError_t build_batch_worklists(const std::vector<WorkListArgs>& args, DevBuf& counts,
+                                 DevBuf& items, std::vector<int>* job_offsets, hipStream_t s) {
+  hipError_t e;
+  const int nj = (int)args.size();
+  const int64_t nc = args[0].g.ncell();
+  const int64_t m = nc * nj;
+  if (m + 2 > INT32_MAX) return hipErrorInvalidValue;
+  if ((e = counts.ensure(sizeof(int) * (size_t)(2 * m + nj + 4))) != hipSuccess) return e;
+  int* cnt = counts.as<int>();
+  int* off = cnt + m + 1;
+  int* jo = off + m + 1;
+  const int blocks = (int)((nc + 127) / 128);
+  for (int k = 0; k < nj; ++k) {
+    hipLaunchKernelGGL(k_worklist_count, dim3(blocks), dim3(128), 0, s, args[k], cnt + k * nc);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_scan_items, dim3(1), dim3(kScanBlock), 0, s, cnt, m, off);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_gather_offsets, dim3((nj + 256) / 256), dim3(256), 0, s, off, nc, nj, jo);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  job_offsets->assign(nj + 1, 0);
+  if ((e = hipMemcpyAsync(job_offsets->data(), jo, sizeof(int) * (nj + 1), hipMemcpyDeviceToHost,
+                          s)) != hipSuccess)
+    return e;
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  const int total = (*job_offsets)[nj];
+  if (total == 0) return hipSuccess;
+  if ((e = items.ensure(sizeof(PairItem) * (size_t)total)) != hipSuccess) return e;
+  for (int k = 0; k < nj; ++k) {
+    hipLaunchKernelGGL(k_worklist_fill, dim3(blocks), dim3(128), 0, s, args[k], off + k * nc,
+                       items.as<PairItem>());
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 hipError_t launch_pairs(const PairLaunch& pl, hipStream_t s) {
   KArgs k;
-  k.nitems = pl.nitems;
+  k.nitems = pl.batch ? pl.nchunks : pl.nitems;
   k.nt = pl.nt;
   k.inv_h = pl.inv_h;
   k.L = pl.L;
@@ -452,6 +554,13 @@ hipError_t launch_pairs(const PairLaunch& pl, hipStream_t s) {
   k.Lx = pl.Lx;
   k.le = pl.le;
   k.zero = pl.zero;
+  // Claim granularity: up to 8 items per atomic while every wave still gets
+  // about 8 claims (tail balance).
+  {
+    const long long waves = (long long)pl.nblocks * (kBlockMixed / kWave);
+    long long cl = waves > 0 ? (long long)pl.nitems / (waves * 8) : 1;
+    k.claim = (int)(cl < 1 ? 1 : (cl > 8 ? 8 : cl));
+  }
   const int ncopy = 1 << pl.cshift;
   dim3 grid(pl.nblocks), block(kBlock);
   hipError_t e;
@@ -462,12 +571,18 @@ hipError_t launch_pairs(const PairLaunch& pl, hipStream_t s) {
     const size_t hw_bytes = pl.weighted ? (size_t)(pl.nflat + 2) * ncopy * 8 : 0;
     const size_t lds = hc_bytes + t_bytes + hw_bytes;
     const dim3 mblock(kBlockMixed);
-#define PCF_LAUNCH_MIXED(A, W, C)                                                              \
+#define PCF_LAUNCH_MIXED1(A, W, C, B)                                                          \
   do {                                                                                         \
-    if ((e = set_lds(k_pairs_mixed<A, W, C>, lds)) != hipSuccess) return e;                    \
-    hipLaunchKernelGGL((k_pairs_mixed<A, W, C>), grid, mblock, lds, s, pl.xs, pl.ys, pl.xp,    \
+    if ((e = set_lds(k_pairs_mixed<A, W, C, B>, lds)) != hipSuccess) return e;                 \
+    if (pl.prepare_only) return hipSuccess;                                                    \
+    hipLaunchKernelGGL((k_pairs_mixed<A, W, C, B>), grid, mblock, lds, s, pl.xs, pl.ys, pl.xp, \
                        pl.yp, pl.items, pl.counter, pl.table, pl.rules, pl.hist, pl.whist,     \
-                       pl.stats, k);                                                           \
+                       pl.stats, k, pl.jobs, pl.chunks);                                       \
+  } while (0)
+#define PCF_LAUNCH_MIXED(A, W, C)                                  \
+  do {                                                             \
+    if (pl.batch) PCF_LAUNCH_MIXED1(A, W, C, 1);                   \
+    else PCF_LAUNCH_MIXED1(A, W, C, 0);                            \
   } while (0)
 #define PCF_LAUNCH_CS(A, W)                                   \
   switch (pl.cshift) {                                        \
@@ -485,6 +600,7 @@ hipError_t launch_pairs(const PairLaunch& pl, hipStream_t s) {
     else { PCF_LAUNCH_CS(1, 1) }
 #undef PCF_LAUNCH_CS
 #undef PCF_LAUNCH_MIXED
+#undef PCF_LAUNCH_MIXED1
   } else {
     const size_t hc_bytes = (size_t)(((pl.nflat + 1) * ncopy * 4 + 15) & ~15);
     const size_t hw_bytes = pl.weighted ? (size_t)(pl.nflat + 1) * ncopy * 8 : 0;

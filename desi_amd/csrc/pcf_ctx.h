@@ -2,6 +2,8 @@
 // shared by the pair, triplet and serving translation units.
 #pragma once
 
+#include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -10,7 +12,9 @@
 
 namespace pcf {
 struct JobEngine;
+struct BatchPlan;
 void destroy_engine(JobEngine* e);
+void destroy_batch_plan(BatchPlan* p);
 void engine_drain(JobEngine* e);
 }  // namespace pcf
 
@@ -27,7 +31,10 @@ struct pcf_ctx {
   pcf::DevBuf scratch_cells, scratch_perm, wl_counts, wl_items, table_buf, rules_buf, hist_buf,
       misc_buf, stats_partials, scaled_x, scaled_y, trip_idx, trip_pairs, trip_hist, wl_items2;
   pcf_stats last{};
-  pcf::JobEngine* engine = nullptr;  // asynchronous serving engine (pcf_jobs.hip)
+  pcf::JobEngine* engine = nullptr;  // asynchronous serving engine (jobs.hip)
+  pcf::BatchPlan* batch = nullptr;   // last batched launch plan + graph (batch.hip)
+  std::map<std::string, pcf_tuning> tuned;  // autotuner results by workload (tune.hip)
+  uint64_t next_serial = 1;
 };
 
 
@@ -70,6 +77,13 @@ void free_catalog(Catalog* cat);
 
 // Defined in triplets.hip.
 int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights);
+
+// Defined in batch.hip.
+int run_pairs_batch(pcf_ctx* c, const pcf_job* jobs, int njobs, uint64_t* counts,
+                    double* weights);
+
+// Defined in tune.hip.
+void apply_tuning(pcf_ctx* c, pcf_job* jobs, int njobs);
 
 // Defined in analytic.hip.
 int run_analytic_3iso(pcf_ctx* c, pcf_catalog data, int bins, float dmax, float box, int64_t np,

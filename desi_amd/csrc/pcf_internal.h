@@ -82,6 +82,7 @@ struct Catalog {
   float maxabs = 0;
   bool all_w1 = true;
   bool finite = true;
+  uint64_t serial = 0;  // unique per upload within a context (ids are reused)
   std::vector<std::unique_ptr<DevGrid>> grids;  // resident grids by geometry
 };
 
@@ -115,6 +116,20 @@ hipError_t build_grid(const float4* raw, int64_t n, const GridGeom& g, DevGrid* 
                       DevBuf& scratch_cells, DevBuf& scratch_perm, hipStream_t s);
 hipError_t build_grid64(DevGrid* grid, hipStream_t s);
 
+// One histogram of a batched pair launch (k_pairs_mixed<..., BATCH = 1>).
+struct BatchJob {
+  const float4* xs;  // i side, scaled copy
+  const float4* ys;  // j side, scaled copy
+  const float4* xo;  // i side, cell-ordered originals
+  const float4* yo;  // j side, cell-ordered originals
+  int hoff;          // first row of this job's histogram in the batch output
+  int zero;          // zero-separation rule of this job (BinRules::zero)
+};
+// A run of work items [begin, end) that all belong to job `job`.
+struct BatchChunk {
+  int begin, end, job;
+};
+
 struct PairLaunch {
   const float4* xp;   // i side, original coordinates
   const float4* yp;   // j side, original coordinates
@@ -144,9 +159,52 @@ struct PairLaunch {
   double Lx;    // box side, unscaled
   int le;
   int zero;
+  // batched launch (mixed path only): chunks of the concatenated work list
+  int batch = 0;
+  int prepare_only = 0;  // only set the kernel's LDS attribute (before graph capture)
+  const BatchJob* jobs = nullptr;
+  const BatchChunk* chunks = nullptr;
+  int nchunks = 0;
 };
 constexpr int kMixedBlock = 512;       // threads per workgroup of the mixed kernel
 constexpr int kMixedTableMax = 28672;  // static LDS bytes of its d2 -> bin table
+
+// Launch shape of the pair kernels: 2^cshift LDS copies of each histogram row
+// (lanes spread their atomics over the copies) and the resident workgroups per
+// CU of the persistent grid.  Defaults: as many copies as fit 48 KB, as many
+// workgroups as the 160 KB of LDS per CU hold.  The overrides (the job's
+// lds_copies_log2 = log2 + 1 and blocks_per_cu; 0 = default) are the
+// autotuner's knobs; no choice changes a result.
+struct PairShape {
+  int cshift;
+  int per_cu;
+  size_t lds_bytes;
+};
+inline PairShape pair_shape(int nflat, int bins, bool weighted, bool plain, int lds_log2p1,
+                            int per_cu_override) {
+  const int rows = nflat + 2;
+  const int per = 4 + (weighted ? 8 : 0);
+  int copies = 64;
+  while (copies > 1 && (size_t)rows * copies * per > 48 * 1024) copies >>= 1;
+  int cs = 0;
+  while ((1 << cs) < copies) ++cs;
+  if (lds_log2p1 > 0) {
+    cs = lds_log2p1 - 1;
+    if (cs > 6) cs = 6;
+    while (cs > 0 && (size_t)rows * ((size_t)1 << cs) * per > 120 * 1024) --cs;
+  }
+  PairShape sh;
+  sh.cshift = cs;
+  sh.lds_bytes = (plain ? 0 : (size_t)kMixedTableMax) + (size_t)rows * ((size_t)1 << cs) * per +
+                 (size_t)(bins + 2) * 8;
+  const int threads = plain ? 256 : kMixedBlock;
+  int per_cu = (int)((160 * 1024) / (sh.lds_bytes + 512));
+  if (per_cu < 1) per_cu = 1;
+  if (per_cu > 2048 / threads) per_cu = 2048 / threads;
+  if (per_cu_override > 0 && per_cu_override < per_cu) per_cu = per_cu_override;
+  sh.per_cu = per_cu;
+  return sh;
+}
 hipError_t launch_pairs(const PairLaunch& pl, hipStream_t s);
 hipError_t launch_scale_points(const float4* in, int64_t n, float sigma, float4* out,
                                hipStream_t s);

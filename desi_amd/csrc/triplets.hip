@@ -542,9 +542,9 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
     return fail(c, PCF_ERR_UNSUPPORTED, "geometry");
   int rc = PCF_OK;
   HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-  DevGrid* gx = get_grid(c, X, g, job->reserved[0] != 0, &rc);
+  DevGrid* gx = get_grid(c, X, g, job->rebuild_grid != 0, &rc);
   if (!gx) return rc;
-  DevGrid* gk = auto_ ? gx : get_grid(c, K, g, job->reserved[0] != 0, &rc);
+  DevGrid* gk = auto_ ? gx : get_grid(c, K, g, job->rebuild_grid != 0, &rc);
   if (!gk) return rc;
 
   // Exact rules: floored ds (3iso_k.cuh:44, 3isoBPC_k.cuh:218, 3ani_k.cuh:45),
@@ -748,7 +748,7 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[3]));
   st.total_ms = ms;
   st.cells_per_rmax = m;
-  st.reserved[0] = npairs;
+  st.pairs_listed = (int32_t)npairs;
   c->last = st;
   return PCF_OK;
 }

@@ -85,8 +85,12 @@ typedef struct pcf_job {
   pcf_catalog cat_x;   /* X: the auto catalog, or D of a cross histogram  */
   pcf_catalog cat_y;   /* Y: R of a cross histogram; -1 for XX / XXX      */
   int32_t partitions;  /* -p; informational                               */
-  int32_t cells_per_rmax; /* 0 = autotuned cell size, else rmax/cell edge  */
-  int32_t reserved[5];
+  int32_t cells_per_rmax; /* cell edge = rmax / cells_per_rmax; 0 = the    */
+                          /* autotuned value (pcf_autotune) or a default  */
+  int32_t rebuild_grid;   /* 1: re-grid the catalogs even if resident      */
+  int32_t lds_copies_log2; /* LDS histogram copies per row, log2 + 1; 0 = auto */
+  int32_t blocks_per_cu;  /* resident workgroups per CU; 0 = auto          */
+  int32_t reserved[2];
 } pcf_job;
 
 /* Per-call statistics of the last completed histogram. */
@@ -97,7 +101,9 @@ typedef struct pcf_stats {
   double kernel_ms;      /* device time of the counting kernel(s)         */
   double total_ms;       /* device time of grid build + counting + merge  */
   int32_t cells_per_rmax;
-  int32_t reserved[3];
+  int32_t graph_replay;  /* pcf_histogram_batch: 1 if the captured graph was replayed */
+  int32_t chunks;        /* pcf_histogram_batch: work chunks of the launch  */
+  int32_t pairs_listed;  /* triplets: entries of the (1,2) pair list        */
 } pcf_stats;
 
 /* Sizes of a histogram: bins (2iso), bins^2 (2ani), bins^3 (3iso),
@@ -131,6 +137,39 @@ int64_t pcf_catalog_size(pcf_ctx* ctx, pcf_catalog id);
  * Replaces the kernel launches + D2H copies of one histogram in the
  * reference drivers (e.g. cuda/pcf2iso.cuh:95-124). */
 int pcf_histogram(pcf_ctx* ctx, const pcf_job* job, uint64_t* counts, double* weights);
+
+/* Several pair histograms in one persistent launch.  Replaces the per-random
+ * loops of the reference's many-random drivers, which launch XX and XY once
+ * per random file (cuda/pcf2iso.cuh:96-104, cuda/pcf2ani.cuh:90-98).  All jobs
+ * share type (PCF_2ISO or PCF_2ANI), bins, dmax, bpc, box, precision and
+ * cells_per_rmax; hist is PCF_XX or PCF_XY; the catalogs may differ in size.
+ * counts / weights receive njobs consecutive histograms of pcf_hist_size
+ * values (either may be NULL), each equal to what pcf_histogram returns for
+ * that job (counts bit-exact, weights to 1e-12 relative).  The plan and its
+ * HIP graph stay on the context: repeating the same batch over the same
+ * resident catalogs replays the graph (pcf_last_stats: graph_replay, chunks). */
+int pcf_histogram_batch(pcf_ctx* ctx, const pcf_job* jobs, int32_t njobs, uint64_t* counts,
+                        double* weights);
+
+/* Autotuner of the pair kernels (no counterpart in the reference, whose
+ * launch shape is fixed: 8x8 threads per block, one thread per node pair,
+ * cuda/pcf2iso.cuh:74-80, and whose -p partitions flag sets the node size,
+ * cuda/main.cu:86).  Runs the given job, or batch of pair jobs (njobs > 1, as
+ * for pcf_histogram_batch), on its own catalogs under candidate cell sizes,
+ * LDS histogram layouts and occupancies, verifies that every candidate gives
+ * the default configuration's counts, and caches the fastest on the context:
+ * later pcf_histogram / pcf_histogram_batch calls on the same workload (same
+ * type, bins, dmax, box, precision and catalog sizes) whose knobs are 0 use it.
+ * out (may be NULL) receives the choice. */
+typedef struct pcf_tuning {
+  int32_t cells_per_rmax;
+  int32_t lds_copies_log2; /* log2 + 1 of the LDS copies; 0 = default        */
+  int32_t blocks_per_cu;   /* 0 = default                                    */
+  int32_t trials;          /* histogram runs spent                           */
+  double best_ms;          /* counting-kernel time of the choice             */
+  double default_ms;       /* same, default configuration                    */
+} pcf_tuning;
+int pcf_autotune(pcf_ctx* ctx, const pcf_job* jobs, int32_t njobs, pcf_tuning* out);
 
 /* Statistics of the last pcf_histogram / pcf_wait on this context. */
 int pcf_last_stats(const pcf_ctx* ctx, pcf_stats* out);

@@ -1,0 +1,128 @@
+"""Batched pair histograms in one persistent launch (pcf_histogram_batch):
+every histogram of the batch equals the single-job result (counts bit-exact,
+weights to 1e-12 relative) and the oracle; a repeated batch replays the
+captured graph; a changed catalog invalidates the plan."""
+import numpy as np
+import pytest
+
+import desi_amd as da
+from desi_amd import synth
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _jobs(type_, d, rs, *, bins, dmax, bpc=False, box=0.0):
+    jobs = [da.make_job(type_, da.PCF_XX, bins=bins, dmax=dmax, cat_x=d, bpc=bpc, box=box)]
+    for r in rs:
+        jobs.append(da.make_job(type_, da.PCF_XX, bins=bins, dmax=dmax, cat_x=r, bpc=bpc, box=box))
+        jobs.append(da.make_job(type_, da.PCF_XY, bins=bins, dmax=dmax, cat_x=d, cat_y=r, bpc=bpc,
+                                box=box))
+    return jobs
+
+
+def _singles(engine, jobs):
+    out = []
+    for j in jobs:
+        out.append(engine.histogram(j))
+    return out
+
+
+def _same(batch, single, rel=1e-12):
+    for k, ((cb, wb), (cs, ws)) in enumerate(zip(batch, single)):
+        assert np.array_equal(cb, cs), f"job {k}: counts differ"
+        assert np.all(np.abs(wb - ws) <= rel * np.maximum(np.abs(ws), 1e-300)), f"job {k} weights"
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_2ani_many_randoms(engine, weighted):
+    """The 64-random 2ani shape at small size: clustered D, uneven uniform
+    randoms, rmax 120, 30 x 30 bins, no BPC."""
+    w = "uniform" if weighted else None
+    d = synth.neyman_scott(48, 32, 8.0, 1000.0, seed=51, weights=w)
+    rng = np.random.default_rng(52)
+    sizes = [int(round(np.exp(rng.uniform(np.log(300), np.log(3000))))) for _ in range(6)]
+    rcats = [synth.uniform(n, 1000.0, seed=60 + i, weights=w) for i, n in enumerate(sizes)]
+    print("case 2ani batch weighted", weighted, "sizes", sizes, flush=True)
+    cd = engine.upload(d)
+    crs = [engine.upload(r) for r in rcats]
+    try:
+        jobs = _jobs(da.PCF_2ANI, cd, crs, bins=30, dmax=120.0)
+        res = engine.histogram_batch(jobs)
+        st = engine.stats()
+        assert st["graph_replay"] == 0 and st["chunks"] > 0
+        # replay of the captured graph (before single jobs re-grid the catalogs
+        # on their own geometries and evict the batch's grids)
+        res2 = engine.histogram_batch(jobs)
+        assert engine.stats()["graph_replay"] == 1
+        for (c1, w1), (c2, w2) in zip(res, res2):
+            assert np.array_equal(c1, c2)
+            assert np.allclose(w1, w2, rtol=1e-12, atol=0)
+        _same(res, _singles(engine, jobs))
+        # the oracle on DD and one (RR, DR)
+        c, _ = O.pairs(d, None, ani=True, bins=30, dmax=120.0)
+        assert np.array_equal(res[0][0], c)
+        c, _ = O.pairs(d, rcats[2], ani=True, bins=30, dmax=120.0)
+        assert np.array_equal(res[6][0], c)
+    finally:
+        engine.release(cd)
+        for c in crs:
+            engine.release(c)
+
+
+def test_2iso_periodic_zero_rules(engine):
+    """XX excludes coincident points, XY keeps them: duplicated lattice points
+    in one batch with both kinds of job."""
+    a = synth.lattice(800, 32.0, 0.5, seed=71)
+    a = np.concatenate([a, a[:100]])  # exact duplicates
+    b = np.concatenate([synth.lattice(600, 32.0, 0.5, seed=72), a[:50]])
+    ca, cb = engine.upload(a), engine.upload(b)
+    try:
+        jobs = [da.make_job(da.PCF_2ISO, da.PCF_XX, bins=8, dmax=6.0, cat_x=ca, bpc=True, box=32.0),
+                da.make_job(da.PCF_2ISO, da.PCF_XY, bins=8, dmax=6.0, cat_x=ca, cat_y=cb, bpc=True,
+                            box=32.0),
+                da.make_job(da.PCF_2ISO, da.PCF_XX, bins=8, dmax=6.0, cat_x=cb, bpc=True, box=32.0)]
+        res = engine.histogram_batch(jobs)
+        for (c, _), (x, y) in zip(res, ((a, None), (a, b), (b, None))):
+            co, _ = O.pairs(x, y, bpc=True, box=32.0, bins=8, dmax=6.0)
+            assert np.array_equal(c, co)
+        _same(res, _singles(engine, jobs))
+    finally:
+        engine.release(ca)
+        engine.release(cb)
+
+
+def test_plan_invalidated_by_new_catalog(engine):
+    d = synth.uniform(1500, 500.0, seed=81)
+    r1 = synth.uniform(1200, 500.0, seed=82)
+    r2 = synth.uniform(900, 500.0, seed=83)
+    cd, cr = engine.upload(d), engine.upload(r1)
+    try:
+        jobs = _jobs(da.PCF_2ISO, cd, [cr], bins=20, dmax=60.0)
+        engine.histogram_batch(jobs)
+        engine.release(cr)
+        cr2 = engine.upload(r2)  # may reuse the id
+        cr = cr2
+        jobs = _jobs(da.PCF_2ISO, cd, [cr], bins=20, dmax=60.0)
+        res = engine.histogram_batch(jobs)
+        assert engine.stats()["graph_replay"] == 0
+        c, _ = O.pairs(d, r2, bins=20, dmax=60.0)
+        assert np.array_equal(res[2][0], c)
+    finally:
+        engine.release(cd)
+        engine.release(cr)
+
+
+def test_batch_argument_errors(engine):
+    d = synth.uniform(200, 100.0, seed=91)
+    cd = engine.upload(d)
+    try:
+        jobs = [da.make_job(da.PCF_2ISO, da.PCF_XX, bins=10, dmax=20.0, cat_x=cd),
+                da.make_job(da.PCF_2ISO, da.PCF_XX, bins=12, dmax=20.0, cat_x=cd)]
+        with pytest.raises(da.PcfError):
+            engine.histogram_batch(jobs)
+        jobs = [da.make_job(da.PCF_3ISO, da.PCF_XXX, bins=10, dmax=20.0, cat_x=cd)]
+        with pytest.raises(da.PcfError):
+            engine.histogram_batch(jobs)
+    finally:
+        engine.release(cd)
