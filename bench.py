@@ -183,6 +183,83 @@ def batch_leg(e, steps, no_tune=False):
                          "unit": "TFLOP/s", "frac": ach / PEAK_FP32, "traffic": None}}
 
 
+def serving_leg(requests=400, rate=100.0, lanes=4, max_batch=8):
+    """Open-loop serving: Poisson arrivals of a request mix over resident
+    catalogs, answered by the engine (pcf_submit / pcf_wait) with `lanes`
+    concurrent lanes and continuous batching.  Mix: 50% 2iso, 30% 2ani, 20%
+    3iso; bins in {20, 30, 40}; periodic for half the requests; catalogs from a
+    pool of 24 (sizes log-uniform in [1024, 262144], uniform or Neyman-Scott,
+    box 250 or 1000); rmax U[40, 120] for pairs, U[20, 40] and N <= 16384 for
+    triplets.  Latency = submit -> result ready, per request."""
+    rng = np.random.default_rng(31)
+    pool = []  # (box, n, xyzw)
+    for k in range(24):
+        box = 250.0 if k % 2 == 0 else 1000.0
+        n = int(round(np.exp(rng.uniform(np.log(1024), np.log(262144)))))
+        if k % 3 == 0:
+            xyzw = synth.neyman_scott(max(1, n // 32), 32, 5.0, box, seed=300 + k)
+        else:
+            xyzw = synth.uniform(n, box, seed=300 + k)
+        pool.append((box, len(xyzw), xyzw))
+    with da.Engine(0) as e:
+        ids = [e.upload(x) for _, _, x in pool]
+        e.set_hbm_budget(0, lanes)
+        e.set_max_batch(max_batch)
+        jobs = []
+        for _ in range(requests):
+            u = rng.random()
+            box = 250.0 if rng.random() < 0.5 else 1000.0
+            cand = [i for i, (b, n, _) in enumerate(pool) if b == box]
+            bpc = bool(rng.random() < 0.5)
+            bins = int(rng.choice([20, 30, 40]))
+            if u < 0.8:
+                type_ = da.PCF_2ISO if u < 0.5 else da.PCF_2ANI
+                dmax = float(rng.uniform(40.0, 120.0))
+                x = int(rng.choice(cand))
+                if rng.random() < 0.5:
+                    jobs.append(da.make_job(type_, da.PCF_XX, bins=bins, dmax=dmax, cat_x=ids[x],
+                                            bpc=bpc, box=box))
+                else:
+                    y = int(rng.choice(cand))
+                    jobs.append(da.make_job(type_, da.PCF_XY, bins=bins, dmax=dmax, cat_x=ids[x],
+                                            cat_y=ids[y], bpc=bpc, box=box))
+            else:
+                small = [i for i in cand if pool[i][1] <= 16384] or cand[:1]
+                x = int(rng.choice(small))
+                dmax = float(rng.uniform(20.0, 40.0))
+                jobs.append(da.make_job(da.PCF_3ISO, da.PCF_XXX, bins=bins, dmax=dmax,
+                                        cat_x=ids[x], bpc=bpc, box=box))
+        gaps = rng.exponential(1.0 / rate, requests)
+        tickets = []
+        t0 = time.perf_counter()
+        t_next = t0
+        for j, g in zip(jobs, gaps):
+            t_next += g
+            d = t_next - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets.append(e.submit(j))
+        lat, queue, useful, batched = [], [], 0, 0
+        for t, j in zip(tickets, jobs):
+            c, _, st = e.wait(t, j.type, j.bins)
+            lat.append(st["latency_ms"])
+            queue.append(st["queue_ms"])
+            batched += st["batched_jobs"] > 1
+            if j.type in (da.PCF_2ISO, da.PCF_2ANI):
+                useful += int(c.sum())
+        t1 = time.perf_counter()
+    lat = np.array(lat)
+    return {"workload": f"{requests} Poisson requests at {rate:g}/s: 50% 2iso, 30% 2ani, "
+                        "20% 3iso over 24 resident catalogs (N 1024..262144, box 250/1000, "
+                        "uniform or clustered), bins 20/30/40, rmax U[40,120] (3iso U[20,40], "
+                        f"N <= 16384), 50% periodic; {lanes} lanes, batching up to {max_batch}",
+            "requests": requests, "wall_s": t1 - t0, "requests_per_s": requests / (t1 - t0),
+            "p50_ms": float(np.percentile(lat, 50)), "p90_ms": float(np.percentile(lat, 90)),
+            "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
+            "mean_queue_ms": float(np.mean(queue)), "batched_requests": batched,
+            "pair_requests_useful_pairs_per_s": useful / (t1 - t0)}
+
+
 def analytic_leg(e):
     """3iso -bpc -a on the config-2 data catalog: fine periodic pair histograms
     (3,000 and 600,000 bins), ff_av / ff_av_ref and the analytic RRR / DDR bin
@@ -242,6 +319,8 @@ def main():
     ap.add_argument("--no-triplets", action="store_true")
     ap.add_argument("--no-analytic", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=400)
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (default launch configuration)")
     ap.add_argument("--batch-steps", type=int, default=3)
@@ -300,6 +379,7 @@ def main():
         ana = None if args.no_analytic else analytic_leg(e)
         bat = None if args.no_batch else batch_leg(e, args.batch_steps, args.no_tune)
 
+    serving = None if args.no_serving else serving_leg(args.serving_requests)
     achieved = evals * FLOPS_PER_PAIR / (kms * 1e-3)
     out = {
         "metric": "pairs/s (2iso DD+RR+DR, useful pairs)",
@@ -335,6 +415,8 @@ def main():
         out["analytic_3iso"] = ana
     if bat is not None:
         out["batched_2ani_64_randoms"] = bat
+    if serving is not None:
+        out["serving"] = serving
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(d)
     print(json.dumps(out), flush=True)

@@ -29,7 +29,7 @@ EXPORTED_SYMBOLS = (
     "pcf_hist_size", "pcf_create", "pcf_destroy", "pcf_last_error", "pcf_create_error",
     "pcf_catalog_upload", "pcf_catalog_upload_device", "pcf_catalog_release",
     "pcf_catalog_size", "pcf_histogram", "pcf_last_stats", "pcf_analytic_rr",
-    "pcf_analytic_3iso", "pcf_histogram_batch", "pcf_autotune",
+    "pcf_analytic_3iso", "pcf_histogram_batch", "pcf_autotune", "pcf_set_max_batch",
     "pcf_set_hbm_budget", "pcf_submit", "pcf_poll", "pcf_wait",
 )
 
@@ -58,7 +58,9 @@ class PcfStats(ctypes.Structure):
                 ("work_items", ctypes.c_uint64), ("kernel_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("cells_per_rmax", ctypes.c_int32),
                 ("graph_replay", ctypes.c_int32), ("chunks", ctypes.c_int32),
-                ("pairs_listed", ctypes.c_int32)]
+                ("pairs_listed", ctypes.c_int32), ("batched_jobs", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("queue_ms", ctypes.c_double),
+                ("latency_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -104,6 +106,7 @@ def load_library(path=LIB_PATH):
     L.pcf_analytic_3iso.argtypes = [vp, c_i32, c_i32, ctypes.c_float, ctypes.c_float, c_i64,
                                     c_i32, c_i32, vp, vp, vp, vp]
     L.pcf_set_hbm_budget.argtypes = [vp, c_i64, c_i32]
+    L.pcf_set_max_batch.argtypes = [vp, c_i32]
     L.pcf_submit.argtypes = [vp, ctypes.POINTER(PcfJob), ctypes.POINTER(c_i64)]
     L.pcf_poll.argtypes = [vp, c_i64, ctypes.POINTER(c_i32)]
     L.pcf_wait.argtypes = [vp, c_i64, vp, vp, ctypes.POINTER(PcfStats)]
@@ -236,6 +239,9 @@ class Engine:
     # ---- asynchronous (serving) ------------------------------------------
     def set_hbm_budget(self, nbytes, max_inflight=4):
         self._check(self.lib.pcf_set_hbm_budget(self.ctx, int(nbytes), int(max_inflight)))
+
+    def set_max_batch(self, n):
+        self._check(self.lib.pcf_set_max_batch(self.ctx, int(n)))
 
     def submit(self, job):
         t = ctypes.c_int64()

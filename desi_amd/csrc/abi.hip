@@ -259,51 +259,93 @@ int choose_geometry(const Catalog* X, const Catalog* Y, float dmax, int bpc, flo
   return 0;
 }
 
+static void free_grid(DevGrid* gr) {
+  if (gr->pts) (void)hipFree(gr->pts);
+  if (gr->pts64) (void)hipFree(gr->pts64);
+  if (gr->cell_start) (void)hipFree(gr->cell_start);
+  if (gr->ready) (void)hipEventDestroy(gr->ready);
+  gr->pts = nullptr;
+  gr->pts64 = nullptr;
+  gr->cell_start = nullptr;
+  gr->ready = nullptr;
+}
+
+// The catalog's grid for geometry g, built on this context's stream if it is
+// not resident (or rebuilt when asked and no other job holds it).  The grid is
+// pinned for the caller (release_grid / GridPins); if another stream built it,
+// this stream waits for that build.
 DevGrid* get_grid(pcf_ctx* c, Catalog* cat, const GridGeom& g, bool force_rebuild, int* rc) {
+  std::lock_guard<std::mutex> lk(c->store->mu);
   for (auto& gr : cat->grids) {
     if (gr->valid && gr->g.same(g)) {
-      if (force_rebuild) {
+      if (force_rebuild && gr->users == 0) {
         hipError_t e = build_grid(cat->raw, cat->n, g, gr.get(), c->scratch_cells,
                                   c->scratch_perm, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(gr->ready, c->stream);
         if (e != hipSuccess) {
           *rc = hipfail(c, e, "build_grid");
           return nullptr;
         }
+      } else {
+        hipError_t e = hipStreamWaitEvent(c->stream, gr->ready, 0);
+        if (e != hipSuccess) {
+          *rc = hipfail(c, e, "grid wait");
+          return nullptr;
+        }
       }
+      gr->users++;
       return gr.get();
     }
   }
-  // Keep a handful of geometries per catalog (DR/RR reuse, serving).
+  // Keep a handful of geometries per catalog (DR/RR reuse, serving): evict the
+  // oldest grid nobody holds.
   if (cat->grids.size() >= 4) {
-    DevGrid* old = cat->grids.front().get();
-    if (old->pts) (void)hipFree(old->pts);
-    if (old->pts64) (void)hipFree(old->pts64);
-    if (old->cell_start) (void)hipFree(old->cell_start);
-    cat->grids.erase(cat->grids.begin());
+    for (auto it = cat->grids.begin(); it != cat->grids.end(); ++it) {
+      if ((*it)->users == 0) {
+        (void)hipEventSynchronize((*it)->ready);
+        free_grid(it->get());
+        cat->grids.erase(it);
+        break;
+      }
+    }
   }
   auto gr = std::make_unique<DevGrid>();
-  hipError_t e = build_grid(cat->raw, cat->n, g, gr.get(), c->scratch_cells, c->scratch_perm,
-                            c->stream);
+  hipError_t e = hipEventCreateWithFlags(&gr->ready, hipEventDisableTiming);
+  if (e == hipSuccess)
+    e = build_grid(cat->raw, cat->n, g, gr.get(), c->scratch_cells, c->scratch_perm, c->stream);
+  if (e == hipSuccess) e = hipEventRecord(gr->ready, c->stream);
   if (e != hipSuccess) {
-    if (gr->pts) (void)hipFree(gr->pts);
-    if (gr->cell_start) (void)hipFree(gr->cell_start);
+    free_grid(gr.get());
     *rc = (e == hipErrorOutOfMemory) ? fail(c, PCF_ERR_NOMEM, "grid allocation")
                                      : hipfail(c, e, "build_grid");
     (void)hipGetLastError();
     return nullptr;
   }
+  gr->users = 1;
   cat->grids.push_back(std::move(gr));
   return cat->grids.back().get();
+}
+
+void release_grid(pcf_ctx* c, DevGrid* g) {
+  std::lock_guard<std::mutex> lk(c->store->mu);
+  if (g->users > 0) g->users--;
+}
+
+hipError_t ensure_grid64(pcf_ctx* c, DevGrid* g) {
+  std::lock_guard<std::mutex> lk(c->store->mu);
+  hipError_t e;
+  if (!g->pts64) {
+    if ((e = hipStreamWaitEvent(c->stream, g->ready, 0)) != hipSuccess) return e;
+    if ((e = build_grid64(g, c->stream)) != hipSuccess) return e;
+    return hipEventRecord(g->ready, c->stream);
+  }
+  return hipStreamWaitEvent(c->stream, g->ready, 0);
 }
 
 void free_catalog(Catalog* cat) {
   if (cat->raw) (void)hipFree(cat->raw);
   cat->raw = nullptr;
-  for (auto& gr : cat->grids) {
-    if (gr->pts) (void)hipFree(gr->pts);
-    if (gr->pts64) (void)hipFree(gr->pts64);
-    if (gr->cell_start) (void)hipFree(gr->cell_start);
-  }
+  for (auto& gr : cat->grids) free_grid(gr.get());
   cat->grids.clear();
 }
 
@@ -349,13 +391,14 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   const bool rebuild = job->rebuild_grid != 0;
   int rc = PCF_OK;
   HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-  DevGrid* gx = get_grid(c, X, g, rebuild, &rc);
+  GridPins pins(c);
+  DevGrid* gx = pins.add(get_grid(c, X, g, rebuild, &rc));
   if (!gx) return rc;
-  DevGrid* gy = auto_ ? gx : get_grid(c, Y, g, rebuild, &rc);
+  DevGrid* gy = auto_ ? gx : pins.add(get_grid(c, Y, g, rebuild, &rc));
   if (!gy) return rc;
   if (plain) {
-    HIPCHK(c, build_grid64(gx, c->stream));
-    if (!auto_) HIPCHK(c, build_grid64(gy, c->stream));
+    HIPCHK(c, ensure_grid64(c, gx));
+    if (!auto_) HIPCHK(c, ensure_grid64(c, gy));
   }
 
   // Rules, thresholds and the FP32 table.
@@ -521,6 +564,7 @@ int pcf_create(pcf_ctx** out, int32_t device) {
   }
   auto c = new pcf_ctx();
   c->device = device;
+  c->store = std::make_shared<CatalogStore>();
   if ((e = hipSetDevice(device)) != hipSuccess) goto fail;
   hipDeviceProp_t prop;
   if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) goto fail;
@@ -545,8 +589,12 @@ void pcf_destroy(pcf_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   destroy_batch_plan(c->batch);
   c->batch = nullptr;
-  for (auto& cat : c->cats)
-    if (cat) free_catalog(cat.get());
+  if (!c->lane && c->store) {
+    std::lock_guard<std::mutex> lk(c->store->mu);
+    for (auto& cat : c->store->cats)
+      if (cat) free_catalog(cat.get());
+    c->store->cats.clear();
+  }
   for (DevBuf* b : {&c->scratch_cells, &c->scratch_perm, &c->wl_counts, &c->wl_items,
                     &c->table_buf, &c->rules_buf, &c->hist_buf, &c->misc_buf, &c->stats_partials,
                     &c->scaled_x, &c->scaled_y, &c->trip_idx, &c->trip_pairs, &c->trip_hist,
@@ -567,7 +615,10 @@ static int upload_common(pcf_ctx* c, const void* src, int64_t n, hipMemcpyKind k
   HIPCHK(c, hipSetDevice(c->device));
   auto cat = std::make_unique<Catalog>();
   cat->n = n;
-  cat->serial = c->next_serial++;
+  {
+    std::lock_guard<std::mutex> lk(c->store->mu);
+    cat->serial = c->store->next_serial++;
+  }
   HIPCHK(c, hipMalloc(&cat->raw, sizeof(float4) * (n > 0 ? n : 1)));
   if (n > 0) {
     hipError_t e = hipMemcpyAsync(cat->raw, src, sizeof(float4) * n, kind, c->stream);
@@ -581,14 +632,16 @@ static int upload_common(pcf_ctx* c, const void* src, int64_t n, hipMemcpyKind k
     free_catalog(cat.get());
     return rc;
   }
-  for (size_t i = 0; i < c->cats.size(); ++i)
-    if (!c->cats[i]) {
-      c->cats[i] = std::move(cat);
+  std::lock_guard<std::mutex> lk(c->store->mu);
+  auto& cats = c->store->cats;
+  for (size_t i = 0; i < cats.size(); ++i)
+    if (!cats[i]) {
+      cats[i] = std::move(cat);
       *id = (pcf_catalog)i;
       return PCF_OK;
     }
-  c->cats.push_back(std::move(cat));
-  *id = (pcf_catalog)(c->cats.size() - 1);
+  cats.push_back(std::move(cat));
+  *id = (pcf_catalog)(cats.size() - 1);
   return PCF_OK;
 }
 
@@ -606,8 +659,9 @@ int pcf_catalog_release(pcf_ctx* c, pcf_catalog id) {
   if (!cat) return fail(c, PCF_ERR_ARG, "unknown catalog id");
   (void)hipStreamSynchronize(c->stream);
   if (c->engine) engine_drain(c->engine);
+  std::lock_guard<std::mutex> lk(c->store->mu);
   free_catalog(cat);
-  c->cats[id].reset();
+  c->store->cats[id].reset();
   return PCF_OK;
 }
 
@@ -621,6 +675,16 @@ int pcf_histogram(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weig
   if (!c || !job) return fail(c, PCF_ERR_ARG, "null argument");
   HIPCHK(c, hipSetDevice(c->device));
   c->err.clear();
+  return dispatch_histogram(c, job, counts, weights);
+}
+
+}  // extern "C"
+
+namespace pcf {
+
+// One histogram on this context's stream (also the engine lanes' entry).
+int dispatch_histogram(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights) {
+  if (!job) return fail(c, PCF_ERR_ARG, "null argument");
   switch (job->type) {
     case PCF_2ISO:
     case PCF_2ANI:
@@ -640,6 +704,30 @@ int pcf_histogram(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weig
   }
   return fail(c, PCF_ERR_ARG, "unknown histogram type");
 }
+
+// A serving lane: same device and resident catalogs as the parent, its own
+// stream, events and scratch buffers.
+int create_lane(pcf_ctx* parent, pcf_ctx** out) {
+  auto c = new pcf_ctx();
+  c->device = parent->device;
+  c->ncu = parent->ncu;
+  c->store = parent->store;
+  c->lane = true;
+  hipError_t e = hipSetDevice(c->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  for (auto& ev : c->ev)
+    if (e == hipSuccess) e = hipEventCreate(&ev);
+  if (e != hipSuccess) {
+    pcf_destroy(c);
+    return hipfail(parent, e, "engine lane");
+  }
+  *out = c;
+  return PCF_OK;
+}
+
+}  // namespace pcf
+
+extern "C" {
 
 int pcf_last_stats(const pcf_ctx* c, pcf_stats* out) {
   if (!c || !out) return PCF_ERR_ARG;

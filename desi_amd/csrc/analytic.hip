@@ -275,7 +275,8 @@ int run_analytic_3iso(pcf_ctx* c, pcf_catalog data, int bins, float dmax, float 
     return fail(c, PCF_ERR_UNSUPPORTED, "geometry");
   int rc = PCF_OK;
   HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-  DevGrid* gx = get_grid(c, X, g, false, &rc);
+  GridPins pins(c);
+  DevGrid* gx = pins.add(get_grid(c, X, g, false, &rc));
   if (!gx) return rc;
   WorkListArgs wa;
   wa.x_cell_start = gx->cell_start;

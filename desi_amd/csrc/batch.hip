@@ -32,6 +32,7 @@ struct BatchPlan {
   std::vector<pcf_job> jobs;
   std::vector<uint64_t> serials;      // catalog serial per job side (x, y)
   std::vector<const void*> grid_pts;  // grid point arrays the plan was built on
+  std::vector<DevGrid*> grids;        // and their grids (same order as serials)
   // geometry / rules
   GridGeom g{};
   int nflat = 0, bins = 0, ani = 0, weighted = 0, cshift = 0, nt = 0, nblocks = 0, m = 0;
@@ -119,7 +120,7 @@ hipError_t enqueue_plan(pcf_ctx* c, BatchPlan* p, size_t njobs, bool prepare) {
 }
 
 // Build (or rebuild) the plan of a batch.  Returns PCF_OK or an error code.
-int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p) {
+int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p, GridPins& pins) {
   const pcf_job& j0 = jobs[0];
   const bool ani = j0.type == PCF_2ANI;
   const int bins = j0.bins;
@@ -159,14 +160,19 @@ int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p) {
     return fail(c, PCF_ERR_UNSUPPORTED, "geometry");
   // Grids (resident, cached per catalog and geometry).
   int rc = PCF_OK;
-  std::vector<DevGrid*> grid_of(c->cats.size(), nullptr);
+  size_t ncat = 0;
+  {
+    std::lock_guard<std::mutex> lk(c->store->mu);
+    ncat = c->store->cats.size();
+  }
+  std::vector<DevGrid*> grid_of(ncat, nullptr);
   for (pcf_catalog id : ids) {
     bool rebuild = false;
     for (int k = 0; k < njobs; ++k)
       if ((jobs[k].cat_x == id || (jobs[k].hist == PCF_XY && jobs[k].cat_y == id)) &&
           jobs[k].rebuild_grid)
         rebuild = true;
-    DevGrid* gr = get_grid(c, get_cat(c, id), p->g, rebuild, &rc);
+    DevGrid* gr = pins.add(get_grid(c, get_cat(c, id), p->g, rebuild, &rc));
     if (!gr) return rc;
     grid_of[id] = gr;
   }
@@ -194,7 +200,7 @@ int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p) {
   p->scale_src.clear();
   p->scale_n.clear();
   p->scale_off.clear();
-  std::vector<int64_t> off_of(c->cats.size(), 0);
+  std::vector<int64_t> off_of(ncat, 0);
   int64_t tot = 0;
   for (pcf_catalog id : ids) {
     p->scale_src.push_back(grid_of[id]->pts);
@@ -268,9 +274,11 @@ int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p) {
   p->jobs.assign(jobs, jobs + njobs);
   p->serials.clear();
   p->grid_pts.clear();
+  p->grids.clear();
   for (pcf_catalog id : ids) {
     p->serials.push_back(get_cat(c, id)->serial);
     p->grid_pts.push_back(grid_of[id]->pts);
+    p->grids.push_back(grid_of[id]);
   }
 
   // Capture the run into a graph.
@@ -294,7 +302,9 @@ int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p) {
 }
 
 // The plan still describes these jobs over the same resident data.
-bool plan_valid(pcf_ctx* c, const BatchPlan* p, const pcf_job* jobs, int njobs) {
+// The plan still describes these jobs over the same resident data; if so its
+// grids are pinned (into `pins`) for this launch.
+bool plan_valid(pcf_ctx* c, const BatchPlan* p, const pcf_job* jobs, int njobs, GridPins& pins) {
   if (!p || !p->exec || (int)p->jobs.size() != njobs) return false;
   for (int k = 0; k < njobs; ++k) {
     if (!same_job(p->jobs[k], jobs[k]) || jobs[k].rebuild_grid) return false;
@@ -305,13 +315,22 @@ bool plan_valid(pcf_ctx* c, const BatchPlan* p, const pcf_job* jobs, int njobs) 
       if (std::find(ids.begin(), ids.end(), id) == ids.end()) ids.push_back(id);
   }
   if (ids.size() != p->serials.size()) return false;
+  std::lock_guard<std::mutex> lk(c->store->mu);
+  auto& cats = c->store->cats;
   for (size_t i = 0; i < ids.size(); ++i) {
-    Catalog* X = get_cat(c, ids[i]);
-    if (!X || X->serial != p->serials[i]) return false;
+    if (ids[i] < 0 || ids[i] >= (pcf_catalog)cats.size() || !cats[ids[i]]) return false;
+    Catalog* X = cats[ids[i]].get();
+    if (X->serial != p->serials[i]) return false;
     bool found = false;
     for (auto& gr : X->grids)
-      if (gr->valid && gr->g.same(p->g) && gr->pts == p->grid_pts[i]) found = true;
+      if (gr.get() == p->grids[i] && gr->valid && gr->g.same(p->g) && gr->pts == p->grid_pts[i])
+        found = true;
     if (!found) return false;
+  }
+  for (DevGrid* g : p->grids) {
+    g->users++;
+    pins.g.push_back(g);
+    (void)hipStreamWaitEvent(c->stream, g->ready, 0);
   }
   return true;
 }
@@ -379,10 +398,11 @@ int run_pairs_batch(pcf_ctx* c, const pcf_job* jobs_in, int njobs, uint64_t* cou
   }
 
   HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-  const bool replay = plan_valid(c, c->batch, jobs, njobs);
+  GridPins pins(c);
+  const bool replay = plan_valid(c, c->batch, jobs, njobs, pins);
   if (!replay) {
     if (!c->batch) c->batch = new BatchPlan();
-    int rc = make_plan(c, jobs, njobs, c->batch);
+    int rc = make_plan(c, jobs, njobs, c->batch, pins);
     if (rc != PCF_OK) {
       delete c->batch;
       c->batch = nullptr;

@@ -4,6 +4,7 @@
 
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -21,20 +22,32 @@ void engine_drain(JobEngine* e);
 // ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
+namespace pcf {
+// Resident catalogs (and their grids) and the tuning cache of a context,
+// shared with the serving engine's lanes, which run jobs concurrently on their
+// own streams.  `mu` guards the catalog table, every grid cache and the tuning
+// map; the device work itself is ordered by events (DevGrid::ready).
+struct CatalogStore {
+  std::vector<std::unique_ptr<Catalog>> cats;
+  std::map<std::string, pcf_tuning> tuned;
+  uint64_t next_serial = 1;
+  std::mutex mu;
+};
+}  // namespace pcf
+
 struct pcf_ctx {
   int device = 0;
   int ncu = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   std::string err;
-  std::vector<std::unique_ptr<pcf::Catalog>> cats;
+  std::shared_ptr<pcf::CatalogStore> store;
+  bool lane = false;  // an engine lane: shares the store of the context that made it
   pcf::DevBuf scratch_cells, scratch_perm, wl_counts, wl_items, table_buf, rules_buf, hist_buf,
       misc_buf, stats_partials, scaled_x, scaled_y, trip_idx, trip_pairs, trip_hist, wl_items2;
   pcf_stats last{};
   pcf::JobEngine* engine = nullptr;  // asynchronous serving engine (jobs.hip)
   pcf::BatchPlan* batch = nullptr;   // last batched launch plan + graph (batch.hip)
-  std::map<std::string, pcf_tuning> tuned;  // autotuner results by workload (tune.hip)
-  uint64_t next_serial = 1;
 };
 
 
@@ -60,9 +73,30 @@ inline int hipfail(pcf_ctx* c, hipError_t e, const char* what) {
   } while (0)
 
 inline Catalog* get_cat(pcf_ctx* c, pcf_catalog id) {
-  if (id < 0 || id >= (pcf_catalog)c->cats.size()) return nullptr;
-  return c->cats[id].get();
+  std::lock_guard<std::mutex> lk(c->store->mu);
+  if (id < 0 || id >= (pcf_catalog)c->store->cats.size()) return nullptr;
+  return c->store->cats[id].get();
 }
+
+// Grids handed out by get_grid are pinned (not evicted or rebuilt by another
+// lane) until released; GridPins releases them when it goes out of scope,
+// after the context's stream has drained.
+void release_grid(pcf_ctx* c, DevGrid* g);
+struct GridPins {
+  pcf_ctx* c;
+  std::vector<DevGrid*> g;
+  explicit GridPins(pcf_ctx* ctx) : c(ctx) {}
+  DevGrid* add(DevGrid* d) {
+    if (d) g.push_back(d);
+    return d;
+  }
+  ~GridPins() {
+    if (!g.empty()) (void)hipStreamSynchronize(c->stream);
+    for (DevGrid* d : g) release_grid(c, d);
+  }
+};
+// FP64 copy of a pinned grid (built once, under the store lock).
+hipError_t ensure_grid64(pcf_ctx* c, DevGrid* g);
 
 
 // Defined in abi.hip.
@@ -81,6 +115,11 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
 // Defined in batch.hip.
 int run_pairs_batch(pcf_ctx* c, const pcf_job* jobs, int njobs, uint64_t* counts,
                     double* weights);
+
+// Defined in abi.hip: one histogram on c's stream (pcf_histogram's body, also
+// run by the serving lanes), and a serving lane of a context.
+int dispatch_histogram(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights);
+int create_lane(pcf_ctx* parent, pcf_ctx** out);
 
 // Defined in tune.hip.
 void apply_tuning(pcf_ctx* c, pcf_job* jobs, int njobs);

@@ -73,6 +73,8 @@ struct DevGrid {
   int* cell_start = nullptr; // ncell+1
   int64_t n = 0;
   bool valid = false;
+  hipEvent_t ready = nullptr;  // recorded after the last (re)build, on the building stream
+  int users = 0;               // jobs holding the grid (no eviction or rebuild meanwhile)
 };
 
 struct Catalog {

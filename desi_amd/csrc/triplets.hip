@@ -542,9 +542,10 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
     return fail(c, PCF_ERR_UNSUPPORTED, "geometry");
   int rc = PCF_OK;
   HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-  DevGrid* gx = get_grid(c, X, g, job->rebuild_grid != 0, &rc);
+  GridPins pins(c);
+  DevGrid* gx = pins.add(get_grid(c, X, g, job->rebuild_grid != 0, &rc));
   if (!gx) return rc;
-  DevGrid* gk = auto_ ? gx : get_grid(c, K, g, job->rebuild_grid != 0, &rc);
+  DevGrid* gk = auto_ ? gx : pins.add(get_grid(c, K, g, job->rebuild_grid != 0, &rc));
   if (!gk) return rc;
 
   // Exact rules: floored ds (3iso_k.cuh:44, 3isoBPC_k.cuh:218, 3ani_k.cuh:45),

@@ -43,8 +43,10 @@ std::string tune_key(pcf_ctx* c, const pcf_job* jobs, int njobs) {
 void apply_tuning(pcf_ctx* c, pcf_job* jobs, int njobs) {
   for (int i = 0; i < njobs; ++i)
     if (jobs[i].cells_per_rmax || jobs[i].lds_copies_log2 || jobs[i].blocks_per_cu) return;
-  auto it = c->tuned.find(tune_key(c, jobs, njobs));
-  if (it == c->tuned.end()) return;
+  const std::string key = tune_key(c, jobs, njobs);
+  std::lock_guard<std::mutex> lk(c->store->mu);
+  auto it = c->store->tuned.find(key);
+  if (it == c->store->tuned.end()) return;
   for (int i = 0; i < njobs; ++i) {
     jobs[i].cells_per_rmax = it->second.cells_per_rmax;
     jobs[i].lds_copies_log2 = it->second.lds_copies_log2;
@@ -108,7 +110,10 @@ extern "C" int pcf_autotune(pcf_ctx* c, const pcf_job* jobs, int32_t njobs, pcf_
     j.rebuild_grid = 0;
   }
   const std::string key = tune_key(c, base.data(), njobs);
-  c->tuned.erase(key);
+  {
+    std::lock_guard<std::mutex> lk(c->store->mu);
+    c->store->tuned.erase(key);
+  }
 
   // Default configuration: the reference result and time (two runs, the first
   // also builds the grids).
@@ -187,7 +192,10 @@ extern "C" int pcf_autotune(pcf_ctx* c, const pcf_job* jobs, int32_t njobs, pcf_
   res.trials = trials;
   res.best_ms = t_best;
   res.default_ms = t_def;
-  c->tuned[key] = res;
+  {
+    std::lock_guard<std::mutex> lk(c->store->mu);
+    c->store->tuned[key] = res;
+  }
   if (out) *out = res;
   return PCF_OK;
 }

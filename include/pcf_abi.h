@@ -104,6 +104,10 @@ typedef struct pcf_stats {
   int32_t graph_replay;  /* pcf_histogram_batch: 1 if the captured graph was replayed */
   int32_t chunks;        /* pcf_histogram_batch: work chunks of the launch  */
   int32_t pairs_listed;  /* triplets: entries of the (1,2) pair list        */
+  int32_t batched_jobs;  /* pcf_wait: jobs that shared this job's launch    */
+  int32_t reserved;
+  double queue_ms;       /* pcf_wait: submit -> start (host clock)          */
+  double latency_ms;     /* pcf_wait: submit -> done (host clock)           */
 } pcf_stats;
 
 /* Sizes of a histogram: bins (2iso), bins^2 (2ani), bins^3 (3iso),
@@ -202,7 +206,14 @@ int pcf_analytic_3iso(pcf_ctx* ctx, pcf_catalog data, int32_t bins, float dmax, 
  * catalogs resident for reuse.  pcf_wait blocks until the job has finished
  * and copies its results out (counts/weights may be NULL). */
 typedef int64_t pcf_ticket;
+/* (Re)starts the engine: bytes <= 0 means 80% of the free HBM; max_inflight
+ * lanes (1..16), each with its own stream.  Waits for submitted jobs first. */
 int pcf_set_hbm_budget(pcf_ctx* ctx, int64_t bytes, int32_t max_inflight);
+/* Continuous batching: a lane taking a pair job (PCF_2ISO / PCF_2ANI, mixed
+ * precision) also takes up to max_batch - 1 queued compatible pair jobs (same
+ * type, bins, dmax, box and tuning knobs) into one persistent launch.
+ * 1 (the default) disables it. */
+int pcf_set_max_batch(pcf_ctx* ctx, int32_t max_batch);
 int pcf_submit(pcf_ctx* ctx, const pcf_job* job, pcf_ticket* ticket);
 int pcf_poll(pcf_ctx* ctx, pcf_ticket ticket, int32_t* done);
 int pcf_wait(pcf_ctx* ctx, pcf_ticket ticket, uint64_t* counts, double* weights,
