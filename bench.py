@@ -89,6 +89,47 @@ def run_triplet_step(e, cd, cr, precision, rebuild=True):
     return tot, evals, kms
 
 
+def triplet_ani_leg(e):
+    """3ani periodic: N=65,536 Neyman-Scott (2048 parents x 32, sigma 4) and
+    65,536 uniform, box 500, rmax 60, 15 bins, weights U[0.5,1.5]: DDD + DDR +
+    DRR + RRR, mixed path against the plain FP64 path."""
+    d = synth.neyman_scott(2048, 32, 4.0, 500.0, seed=41, weights="uniform")
+    r = synth.uniform(65536, 500.0, seed=42, weights="uniform")
+    cd, cr = e.upload(d), e.upload(r)
+    out = {}
+    for prec, name in ((da.PCF_PREC_MIXED, "mixed"), (da.PCF_PREC_FP64, "fp64_plain")):
+        tot = evals = 0
+        kms = 0.0
+        t0 = time.perf_counter()
+        for hist, x, y in ((da.PCF_XXX, cd, -1), (da.PCF_XXY, cd, cr), (da.PCF_XXY, cr, cd),
+                           (da.PCF_XXX, cr, -1)):
+            job = da.make_job(da.PCF_3ANI, hist, bins=15, dmax=60.0, cat_x=x, cat_y=y, bpc=True,
+                              box=500.0, precision=prec)
+            c, _ = e.histogram(job)
+            st = e.stats()
+            tot += int(c.sum())
+            evals += st["evaluations"]
+            kms += st["kernel_ms"]
+        t1 = time.perf_counter()
+        out[name] = {"ms": (t1 - t0) * 1e3, "kernel_ms": kms, "triplets": tot,
+                     "evaluations": evals}
+    e.release(cd)
+    e.release(cr)
+    if out["mixed"]["triplets"] != out["fp64_plain"]["triplets"]:
+        raise SystemExit("plain FP64 and mixed 3ani paths disagree on the count")
+    m = out["mixed"]
+    ach = m["evaluations"] * (FLOPS_PER_TRIPLET + 2) / (m["kernel_ms"] * 1e-3)
+    return {"workload": "3ani periodic box 500, N=65536 Neyman-Scott (2048x32, sigma 4) + 65536 "
+                        "uniform, weights U[0.5,1.5], rmax 60, 15 bins, DDD+DDR+DRR+RRR",
+            "value": m["triplets"] / (m["ms"] * 1e-3), "unit": "triplets/s",
+            "ms_per_step": m["ms"], "kernel_ms_per_step": m["kernel_ms"],
+            "evaluations_per_step": m["evaluations"],
+            "fp64_plain_ms_per_step": out["fp64_plain"]["ms"],
+            "speedup_vs_fp64_plain": out["fp64_plain"]["ms"] / m["ms"],
+            "roofline": {"bound": "valu", "achieved": ach / 1e12, "peak": PEAK_FP32 / 1e12,
+                         "unit": "TFLOP/s", "frac": ach / PEAK_FP32, "traffic": None}}
+
+
 def triplet_leg(e, steps):
     d = synth.neyman_scott(1024, 32, 5.0, 250.0, seed=11, weights="uniform")
     r = synth.uniform(32768, 250.0, seed=12, weights="uniform")
@@ -183,19 +224,20 @@ def batch_leg(e, steps, no_tune=False):
                          "unit": "TFLOP/s", "frac": ach / PEAK_FP32, "traffic": None}}
 
 
-def serving_leg(requests=400, rate=100.0, lanes=4, max_batch=8):
+def serving_leg(requests=400, rate=50.0, lanes=4, max_batch=8):
     """Open-loop serving: Poisson arrivals of a request mix over resident
     catalogs, answered by the engine (pcf_submit / pcf_wait) with `lanes`
     concurrent lanes and continuous batching.  Mix: 50% 2iso, 30% 2ani, 20%
     3iso; bins in {20, 30, 40}; periodic for half the requests; catalogs from a
-    pool of 24 (sizes log-uniform in [1024, 262144], uniform or Neyman-Scott,
+    pool of 24 (sizes log-uniform in [1024, 1048576], uniform or Neyman-Scott,
     box 250 or 1000); rmax U[40, 120] for pairs, U[20, 40] and N <= 16384 for
-    triplets.  Latency = submit -> result ready, per request."""
+    triplets (a 3iso request at N = 1e6 and rmax 120 in box 250 would be
+    ~1e17 triplets).  Latency = submit -> result ready, per request."""
     rng = np.random.default_rng(31)
     pool = []  # (box, n, xyzw)
     for k in range(24):
         box = 250.0 if k % 2 == 0 else 1000.0
-        n = int(round(np.exp(rng.uniform(np.log(1024), np.log(262144)))))
+        n = int(round(np.exp(rng.uniform(np.log(1024), np.log(1048576)))))
         if k % 3 == 0:
             xyzw = synth.neyman_scott(max(1, n // 32), 32, 5.0, box, seed=300 + k)
         else:
@@ -250,7 +292,7 @@ def serving_leg(requests=400, rate=100.0, lanes=4, max_batch=8):
         t1 = time.perf_counter()
     lat = np.array(lat)
     return {"workload": f"{requests} Poisson requests at {rate:g}/s: 50% 2iso, 30% 2ani, "
-                        "20% 3iso over 24 resident catalogs (N 1024..262144, box 250/1000, "
+                        "20% 3iso over 24 resident catalogs (N 1024..1048576, box 250/1000, "
                         "uniform or clustered), bins 20/30/40, rmax U[40,120] (3iso U[20,40], "
                         f"N <= 16384), 50% periodic; {lanes} lanes, batching up to {max_batch}",
             "requests": requests, "wall_s": t1 - t0, "requests_per_s": requests / (t1 - t0),
@@ -376,6 +418,7 @@ def main():
         if p64 // args.fp64_steps != pairs // args.steps:
             raise SystemExit("plain FP64 path and mixed path disagree on the pair count")
         trip = None if args.no_triplets else triplet_leg(e, args.triplet_steps)
+        trip_ani = None if args.no_triplets else triplet_ani_leg(e)
         ana = None if args.no_analytic else analytic_leg(e)
         bat = None if args.no_batch else batch_leg(e, args.batch_steps, args.no_tune)
 
@@ -411,6 +454,8 @@ def main():
         out["autotune"] = tuning
     if trip is not None:
         out["triplets"] = trip
+    if trip_ani is not None:
+        out["triplets_3ani"] = trip_ani
     if ana is not None:
         out["analytic_3iso"] = ana
     if bat is not None:
