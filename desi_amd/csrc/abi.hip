@@ -598,7 +598,7 @@ void pcf_destroy(pcf_ctx* c) {
   for (DevBuf* b : {&c->scratch_cells, &c->scratch_perm, &c->wl_counts, &c->wl_items,
                     &c->table_buf, &c->rules_buf, &c->hist_buf, &c->misc_buf, &c->stats_partials,
                     &c->scaled_x, &c->scaled_y, &c->trip_idx, &c->trip_pairs, &c->trip_hist,
-                    &c->wl_items2})
+                    &c->wl_items2, &c->trip_qidx, &c->trip_q})
     b->release();
   for (auto& ev : c->ev)
     if (ev) (void)hipEventDestroy(ev);

@@ -5,10 +5,12 @@
 // then linearises the non-empty nodes and copies them to the device
 // (cuda/main.cu:215-329).  Here the grid is a counting sort on the device:
 //   1. k_cell_index  : cell of every point + per-cell counts (global atomics)
-//   2. k_scan_exclusive: cell_start[] = exclusive prefix sum of the counts
+//   2. launch_scan   : cell_start[] = exclusive prefix sum of the counts
 //   3. k_scatter     : points copied to their cell's range, SoA-friendly float4
 // The resulting layout is cell-ordered (x-major, z fastest) so that every z-range
 // of a (cx,cy) column is one contiguous, coalesced point range.
+#include <algorithm>
+
 #include "pcf_internal.h"
 
 namespace pcf {
@@ -83,36 +85,6 @@ __global__ void k_cell_index(const float4* __restrict__ raw, int64_t n, GridGeom
   atomicAdd(&counts[c], 1);
 }
 
-// Exclusive scan of counts[0..m) into out[0..m] (out[m] = total), one
-// workgroup: each thread reduces a contiguous chunk, the chunk sums are scanned
-// in LDS, then each thread writes its chunk's prefix.  out may alias counts.
-constexpr int kScanBlock = 1024;
-__global__ __launch_bounds__(kScanBlock) void k_scan_exclusive(const int* counts, int64_t m,
-                                                              int* out) {
-  __shared__ long long part[kScanBlock];
-  const int t = threadIdx.x;
-  const int64_t chunk = (m + kScanBlock - 1) / kScanBlock;
-  const int64_t b = t * chunk;
-  const int64_t e = b + chunk < m ? b + chunk : m;
-  long long s = 0;
-  for (int64_t i = b; i < e; ++i) s += counts[i];
-  part[t] = s;
-  __syncthreads();
-  // Hillis-Steele inclusive scan of the chunk sums.
-  for (int off = 1; off < kScanBlock; off <<= 1) {
-    long long v = t >= off ? part[t - off] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  long long run = part[t] - s;  // exclusive prefix of this chunk
-  for (int64_t i = b; i < e; ++i) {
-    int c = counts[i];
-    out[i] = (int)run;
-    run += c;
-  }
-  if (t == kScanBlock - 1) out[m] = (int)part[t];
-}
 
 __global__ void k_scatter(const float4* __restrict__ raw, int64_t n, const int* __restrict__ cell_of,
                           const int* __restrict__ cell_start, int* __restrict__ cursor,
@@ -172,7 +144,10 @@ hipError_t build_grid(const float4* raw, int64_t n, const GridGeom& g, DevGrid* 
   }
   out->g = g;
   out->n = n;
-  if ((e = scratch_cells.ensure(sizeof(int) * (nc + 1))) != hipSuccess) return e;
+  const size_t cbytes = (sizeof(int) * (size_t)(nc + 1) + 15) & ~(size_t)15;
+  if ((e = scratch_cells.ensure(cbytes + sizeof(long long) * (size_t)scan_tiles(nc))) !=
+      hipSuccess)
+    return e;
   if ((e = scratch_perm.ensure(sizeof(int) * (n > 0 ? n : 1))) != hipSuccess) return e;
   int* counts = scratch_cells.as<int>();
   int* cell_of = scratch_perm.as<int>();
@@ -182,9 +157,10 @@ hipError_t build_grid(const float4* raw, int64_t n, const GridGeom& g, DevGrid* 
     hipLaunchKernelGGL(k_cell_index, dim3(blocks), dim3(256), 0, s, raw, n, g, cell_of, counts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(kScanBlock), 0, s, counts, nc,
-                     out->cell_start);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_scan(counts, nc, out->cell_start,
+                       reinterpret_cast<long long*>(scratch_cells.as<char>() + cbytes), s)) !=
+      hipSuccess)
+    return e;
   if ((e = launch_zero(counts, sizeof(int) * nc, s)) != hipSuccess) return e;
   if (n > 0) {
     hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, raw, n, cell_of,
@@ -205,6 +181,119 @@ hipError_t build_grid64(DevGrid* grid, hipStream_t s) {
     e = hipGetLastError();
   }
   return e;
+}
+
+// ---------------------------------------------------------------------------
+// Device-wide exclusive prefix sum of int counts: out[i] = sum(in[0..i)),
+// out[n] = total (totals below 2^31).  Three passes over tiles of 4096:
+// tile sums, a one-workgroup scan of the tile sums, then each tile scanned in
+// LDS with its offset.  out may alias in.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__global__ __launch_bounds__(kScanThreads) void k_tile_sums(const int* __restrict__ in, int64_t n,
+                                                            long long* __restrict__ tsum) {
+  __shared__ long long red[kScanThreads];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  long long s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + (int64_t)k * kScanThreads + threadIdx.x;
+    if (i < n) s += in[i];
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = kScanThreads / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tsum[blockIdx.x] = red[0];
+}
+
+constexpr int kSumsBlock = 1024;
+__global__ __launch_bounds__(kSumsBlock) void k_scan_tile_sums(long long* __restrict__ tsum,
+                                                               int64_t ntiles,
+                                                               int* __restrict__ total) {
+  __shared__ long long part[kSumsBlock];
+  const int t = threadIdx.x;
+  const int64_t chunk = (ntiles + kSumsBlock - 1) / kSumsBlock;
+  const int64_t b = t * chunk;
+  const int64_t e = b + chunk < ntiles ? b + chunk : ntiles;
+  long long s = 0;
+  for (int64_t i = b; i < e; ++i) s += tsum[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < kSumsBlock; off <<= 1) {
+    long long v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  long long run = part[t] - s;
+  for (int64_t i = b; i < e; ++i) {
+    const long long c = tsum[i];
+    tsum[i] = run;
+    run += c;
+  }
+  if (t == kSumsBlock - 1) *total = (int)part[t];
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_tile_scan(const int* in, int64_t n,
+                                                            const long long* __restrict__ tsum,
+                                                            int* out) {
+  __shared__ int tile[kScanTile];
+  __shared__ long long wsum[kScanThreads];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  for (int k = 0; k < kScanItems; ++k) {
+    const int j = k * kScanThreads + threadIdx.x;
+    const int64_t i = base + j;
+    tile[j] = i < n ? in[i] : 0;
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  long long s = 0;
+  for (int k = 0; k < kScanItems; ++k) s += tile[t * kScanItems + k];
+  wsum[t] = s;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    long long v = t >= off ? wsum[t - off] : 0;
+    __syncthreads();
+    wsum[t] += v;
+    __syncthreads();
+  }
+  long long run = tsum[blockIdx.x] + wsum[t] - s;
+  for (int k = 0; k < kScanItems; ++k) {
+    const int c = tile[t * kScanItems + k];
+    tile[t * kScanItems + k] = (int)run;
+    run += c;
+  }
+  __syncthreads();
+  for (int k = 0; k < kScanItems; ++k) {
+    const int j = k * kScanThreads + threadIdx.x;
+    const int64_t i = base + j;
+    if (i < n) out[i] = tile[j];
+  }
+}
+}  // namespace
+
+hipError_t launch_scan(const int* in, int64_t n, int* out, long long* tsum, hipStream_t s) {
+  hipError_t e;
+  const int64_t ntiles = (n + kScanTile - 1) / kScanTile;
+  if (ntiles > 0) {
+    hipLaunchKernelGGL(k_tile_sums, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, in, n,
+                       tsum);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(kSumsBlock), 0, s, tsum, ntiles, out + n);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (ntiles > 0) {
+    hipLaunchKernelGGL(k_tile_scan, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, in, n, tsum,
+                       out);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace pcf

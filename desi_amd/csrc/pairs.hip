@@ -46,33 +46,6 @@ __global__ void k_worklist_fill(WorkListArgs a, const int* __restrict__ offsets,
   enumerate_cell<true>(a, (int)cell, items + offsets[cell]);
 }
 
-// Same single-workgroup exclusive scan as the grid build (kept local so the
-// translation units stay independent).
-constexpr int kScanBlock = 1024;
-__global__ __launch_bounds__(kScanBlock) void k_scan_items(const int* counts, int64_t m, int* out) {
-  __shared__ long long part[kScanBlock];
-  const int t = threadIdx.x;
-  const int64_t chunk = (m + kScanBlock - 1) / kScanBlock;
-  const int64_t b = t * chunk;
-  const int64_t e = b + chunk < m ? b + chunk : m;
-  long long s = 0;
-  for (int64_t i = b; i < e; ++i) s += counts[i];
-  part[t] = s;
-  __syncthreads();
-  for (int off = 1; off < kScanBlock; off <<= 1) {
-    long long v = t >= off ? part[t - off] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  long long run = part[t] - s;
-  for (int64_t i = b; i < e; ++i) {
-    int c = counts[i];
-    out[i] = (int)run;
-    run += c;
-  }
-  if (t == kScanBlock - 1) out[m] = (int)part[t];
-}
 
 // ---------------------------------------------------------------------------
 // Counting kernels
@@ -475,14 +448,16 @@ hipError_t build_pair_worklist(const WorkListArgs& a, DevBuf& counts, DevBuf& it
                                hipStream_t s) {
   hipError_t e;
   const int64_t nc = a.g.ncell();
-  if ((e = counts.ensure(sizeof(int) * (2 * nc + 2))) != hipSuccess) return e;
+  const size_t ibytes = (sizeof(int) * (size_t)(2 * nc + 2) + 15) & ~(size_t)15;
+  if ((e = counts.ensure(ibytes + sizeof(long long) * (size_t)scan_tiles(nc))) != hipSuccess)
+    return e;
   int* cnt = counts.as<int>();
   int* off = cnt + nc + 1;
+  long long* tsum = reinterpret_cast<long long*>(counts.as<char>() + ibytes);
   const int blocks = (int)((nc + 127) / 128);
   hipLaunchKernelGGL(k_worklist_count, dim3(blocks), dim3(128), 0, s, a, cnt);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_scan_items, dim3(1), dim3(kScanBlock), 0, s, cnt, nc, off);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_scan(cnt, nc, off, tsum, s)) != hipSuccess) return e;
   int total = 0;
   if ((e = hipMemcpyAsync(&total, off + nc, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess)
     return e;
@@ -512,17 +487,19 @@ hipError_t build_batch_worklists(const std::vector<WorkListArgs>& args, DevBuf& 
   const int64_t nc = args[0].g.ncell();
   const int64_t m = nc * nj;
   if (m + 2 > INT32_MAX) return hipErrorInvalidValue;
-  if ((e = counts.ensure(sizeof(int) * (size_t)(2 * m + nj + 4))) != hipSuccess) return e;
+  const size_t ibytes = (sizeof(int) * (size_t)(2 * m + nj + 4) + 15) & ~(size_t)15;
+  if ((e = counts.ensure(ibytes + sizeof(long long) * (size_t)scan_tiles(m))) != hipSuccess)
+    return e;
   int* cnt = counts.as<int>();
   int* off = cnt + m + 1;
   int* jo = off + m + 1;
+  long long* tsum = reinterpret_cast<long long*>(counts.as<char>() + ibytes);
   const int blocks = (int)((nc + 127) / 128);
   for (int k = 0; k < nj; ++k) {
     hipLaunchKernelGGL(k_worklist_count, dim3(blocks), dim3(128), 0, s, args[k], cnt + k * nc);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_scan_items, dim3(1), dim3(kScanBlock), 0, s, cnt, m, off);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_scan(cnt, m, off, tsum, s)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_gather_offsets, dim3((nj + 256) / 256), dim3(256), 0, s, off, nc, nj, jo);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   job_offsets->assign(nj + 1, 0);

@@ -44,7 +44,8 @@ struct pcf_ctx {
   std::shared_ptr<pcf::CatalogStore> store;
   bool lane = false;  // an engine lane: shares the store of the context that made it
   pcf::DevBuf scratch_cells, scratch_perm, wl_counts, wl_items, table_buf, rules_buf, hist_buf,
-      misc_buf, stats_partials, scaled_x, scaled_y, trip_idx, trip_pairs, trip_hist, wl_items2;
+      misc_buf, stats_partials, scaled_x, scaled_y, trip_idx, trip_pairs, trip_hist, wl_items2,
+      trip_qidx, trip_q;
   pcf_stats last{};
   pcf::JobEngine* engine = nullptr;  // asynchronous serving engine (jobs.hip)
   pcf::BatchPlan* batch = nullptr;   // last batched launch plan + graph (batch.hip)

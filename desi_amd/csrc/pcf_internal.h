@@ -211,6 +211,10 @@ hipError_t launch_pairs(const PairLaunch& pl, hipStream_t s);
 hipError_t launch_scale_points(const float4* in, int64_t n, float sigma, float4* out,
                                hipStream_t s);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+// Exclusive prefix sum over the device: out[0..n) and the total in out[n]
+// (totals below 2^31); tsum holds scan_tiles(n) tile sums.  out may alias in.
+inline int64_t scan_tiles(int64_t n) { return (n + 4095) / 4096 + 1; }
+hipError_t launch_scan(const int* in, int64_t n, int* out, long long* tsum, hipStream_t s);
 
 struct WorkListArgs {
   const int* x_cell_start;  // i side (tiles)

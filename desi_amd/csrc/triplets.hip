@@ -13,13 +13,13 @@
 //      periodic image of point 2 relative to point 1 and the two points'
 //      coordinates pre-shifted into point 1's frame; records are grouped by
 //      point 1 in cell order (count, scan, scatter);
-//   2. triplet work list: tiles of 64 third points (one per lane) against the
-//      pair-list range of each neighbour column segment of point-1 cells;
-//   3. triplet kernel: lanes hold the third point, the pair records stream
-//      through scalar loads; both remaining sides are binned through the FP32
-//      d2 -> bin table, lanes in an ambiguous cell (or at a cosine-bin edge)
-//      take the exact FP64 path;
-//   4. 3iso: pairs are unordered (i < j) and the third point is any point; the
+//   2. triplet kernel: a wave owns one point 1; its third points are binned
+//      one per lane against point 1 (the 1-3 side) and the ones in range are
+//      listed in LDS, then the wave's lanes take point 1's pairs and walk the
+//      list (the 2-3 side).  Both sides go through the FP32 d2 -> bin table;
+//      lanes in an ambiguous cell (and 3ani's cosine bins) take the exact FP64
+//      path;
+//   3. 3iso: pairs are unordered (i < j) and the third point is any point; the
 //      ordered histogram is H[a][b][c] = V[a][b][c] + V[a][c][b] (exchanging
 //      points 1 and 2 swaps the d23 and d31 sides).  3ani: the cosine of the
 //      1-2 direction is not symmetric, so pairs are emitted in both orders.
@@ -60,6 +60,7 @@ struct EmitArgs {
   double R;
   double L;
   double ds_th;
+  int key_t;      // 3ani list: key = point1 * bins^2 + b12 * bins + t (else point1 * bins + b12)
 };
 
 __device__ __forceinline__ int img_neg(int img) { return 26 - img; }
@@ -145,7 +146,10 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_pairs(
           if (e.ani &&
               exact_pair(qo, po, img_neg(img), 1, e.bpc, sT, e.bins, e.R, e.L, e.ds_th, &t21) < 0)
             t21 = -1;
-          const int key_i = my_i * e.bins + b12, key_j = j * e.bins + b12;
+          const int key_i = e.key_t ? (my_i * e.bins + b12) * e.bins + (t12 < 0 ? 0 : t12)
+                                    : my_i * e.bins + b12;
+          const int key_j = e.key_t ? (j * e.bins + b12) * e.bins + (t21 < 0 ? 0 : t21)
+                                    : j * e.bins + b12;
           if (PASS == 0) {
             if (fwd) atomicAdd(&cnt[key_i], 1);
             if (e.ani && t21 >= 0) atomicAdd(&cnt[key_j], 1);
@@ -173,6 +177,97 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_pairs(
                 r.meta = img_neg(img) | (b12 << 8) | (t21 << 20);
                 out[slot] = r;
               }
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// Exact 1-3 side of a cross pair (point 3 from the other catalog): bin of d31
+// and the cosine bin p of the 1-3 direction, or -1.  XXY takes d31 = 0 (the
+// cosine is then NaN and lands in bin 0, 3ani_k.cuh:150-170 under the PTX
+// NaN -> 0 conversion).
+__device__ __forceinline__ int exact_cross(const float4 p1, const float4 p3, int img, int bpc,
+                                           const double* T, int bins, double R, double L,
+                                           double ds_th, int* p_out) {
+  const int f3 = image_flags(p1, p3, img, false);
+  if (f3 < 0) return -1;
+  const float a1[3] = {p1.x, p1.y, p1.z}, a3[3] = {p3.x, p3.y, p3.z};
+  double t[3];
+  for (int a = 0; a < 3; ++a) {
+    const double d = __dsub_rn((double)a3[a], (double)a1[a]);
+    t[a] = bpc ? proj_term(d, L, (f3 >> a) & 1) : d;
+  }
+  const double s = __dadd_rn(__dadd_rn(__dmul_rn(t[0], t[0]), __dmul_rn(t[1], t[1])),
+                             __dmul_rn(t[2], t[2]));
+  const int b = class_lds(s, T, bins, R, 0);
+  if (b < 0) return -1;
+  const double pp = __dmul_rn(__dadd_rn(1.0, t[2] / sqrt(s)), ds_th);
+  const int pb = isnan(pp) ? 0 : (int)pp;
+  if (pb > bins - 1) return -1;
+  *p_out = pb;
+  return b;
+}
+
+// The third-point list of 3ani XXY: every (point 1 of X, point 3 of K) in
+// range, keyed by point 1, in the PairRec layout (point 3 in point 1's frame,
+// meta = image | b31 << 8 | p << 20).
+template <int PASS>
+__global__ __launch_bounds__(kEmitBlock) void k_emit_cross(
+    const float4* __restrict__ xs, const float4* __restrict__ xo,
+    const float4* __restrict__ ys, const float4* __restrict__ yo,
+    const PairItem* __restrict__ items, const uint8_t* __restrict__ table,
+    const double* __restrict__ T, int* __restrict__ cnt, const int* __restrict__ off,
+    int* __restrict__ cursor, PairRec* __restrict__ out, EmitArgs e) {
+  __shared__ double sT[kMaxTripBins + 2];
+  for (int t = threadIdx.x; t <= e.bins; t += kEmitBlock) sT[t] = T[t];
+  __syncthreads();
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = (int)((blockIdx.x * (unsigned)kEmitBlock + threadIdx.x) >> 6);
+  const int nwaves = (int)((gridDim.x * (unsigned)kEmitBlock) >> 6);
+  const unsigned ntm1 = (unsigned)(e.nt - 1);
+  const unsigned bins = (unsigned)e.bins;
+  const unsigned amb_row = bins + 1;
+  for (int it = wave; it < e.nitems; it += nwaves) {
+    const int i0 = __builtin_amdgcn_readfirstlane(items[it].i0);
+    const int ni = __builtin_amdgcn_readfirstlane(items[it].ni);
+    const int j0 = __builtin_amdgcn_readfirstlane(items[it].j0);
+    const int nj = __builtin_amdgcn_readfirstlane(items[it].nj);
+    const int img = __builtin_amdgcn_readfirstlane(items[it].code) & 31;
+    const bool active = lane < ni;
+    const int my_i = i0 + lane;
+    float4 ps = make_float4(1e18f, 1e18f, 1e18f, 0.f), po = ps;
+    if (active) {
+      ps = xs[my_i];
+      po = xo[my_i];
+    }
+    const float kxL = (float)(img / 9 - 1) * e.Ls, kyL = (float)((img / 3) % 3 - 1) * e.Ls,
+                kzL = (float)(img % 3 - 1) * e.Ls;
+    const float sx = ps.x - kxL, sy = ps.y - kyL, sz = ps.z - kzL;
+    for (int j = j0; j < j0 + nj; ++j) {
+      const float4 q = ys[j];
+      const float dx = q.x - sx, dy = q.y - sy, dz = q.z - sz;
+      const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+      unsigned row = table[min(__float2uint_rz(d2), ntm1)];
+      if (!active) row = bins;
+      if (row < bins || row == amb_row) {
+        const float4 qo = yo[j];
+        int pb = 0;
+        const int b31 = exact_cross(po, qo, img, e.bpc, sT, e.bins, e.R, e.L, e.ds_th, &pb);
+        if (b31 >= 0) {
+          if (PASS == 0) {
+            atomicAdd(&cnt[my_i], 1);
+          } else {
+            const int slot = off[my_i] + atomicAdd(&cursor[my_i], 1);
+            if (slot < off[my_i + 1]) {
+              PairRec r;
+              r.xj = q.x + kxL; r.yj = q.y + kyL; r.zj = q.z + kzL;
+              r.wj = qo.w;
+              r.j = j;
+              r.meta = img | (b31 << 8) | (pb << 20);
+              out[slot] = r;
             }
           }
         }
@@ -276,12 +371,26 @@ __device__ __forceinline__ void for_each_segment(const float4 p, const TripArgs&
   }
 }
 
-// Lanes hold 64 pairs (1, 2) of one point 1 in a b12 slice; the third points
-// stream through scalar loads from point 1's neighbour segments.  The 1-3 side
-// is uniform across the wave: a third point out of point 1's range is skipped
-// by the whole wave.  ANI = 0: V histogram [b12 - lo][b23][b31] in LDS.
-// ANI = 1: the ordered 5-D histogram in global memory.  PLAIN = 1: every side
-// through the FP64 arithmetic (the plain FP64 path).
+// One wave per point-1 task, in two alternating phases.
+//   A (lanes = third points): point 1's neighbour segments are scanned 64
+//     third points at a time; each lane bins its 1-3 side (FP32 table, exact
+//     FP64 when ambiguous, always exact for 3ani's cosine) and the third points
+//     in range are appended to the wave's list in LDS (ballot compaction):
+//     image-shifted scaled coordinates and weight, index, b31 | p << 8 |
+//     image << 16.
+//   B (lanes = pairs): when the list is full (and at the end), the wave walks
+//     point 1's pairs in the b12 slice, 64 per pass, against every list entry
+//     (an LDS broadcast): the 2-3 side per lane, then the deposit.
+// So the 1-3 side costs one lane-evaluation per third point instead of one
+// wave-wide evaluation per third point and pass over the pairs.  ANI = 0: V
+// histogram [b12 - lo][b23][b31] in LDS.  ANI = 1: the ordered 5-D histogram
+// in global memory.  PLAIN = 1: every side through the FP64 arithmetic (the
+// plain FP64 path).
+template <int ANI>
+struct ListCap {
+  static constexpr int value = ANI ? 256 : 64;  // third-point entries per wave
+};
+
 template <int ANI, int WEIGHTED, int PLAIN>
 __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
     const float4* __restrict__ xs, const float4* __restrict__ xo,
@@ -291,6 +400,7 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
     const double* __restrict__ T, unsigned long long* __restrict__ hist,
     double* __restrict__ whist, unsigned long long* __restrict__ stats, TripArgs a) {
   constexpr int BLOCK = ANI ? kAniTripBlock : kTripBlock;
+  constexpr int CAP = ListCap<ANI>::value;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ uint8_t tab[kMixedTableMax];
   __shared__ double sT[kMaxTripBins + 2];
@@ -298,7 +408,15 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
   const int bins2 = bins * bins;
   const int nrows = ANI ? 0 : (a.hi - a.lo) * bins2;
   uint32_t* hc = reinterpret_cast<uint32_t*>(smem);
-  double* hw = reinterpret_cast<double*>(smem + ((nrows * 4 + 15) & ~15));
+  const size_t hc_bytes = ((size_t)nrows * 4 + 15) & ~(size_t)15;
+  double* hw = reinterpret_cast<double*>(smem + hc_bytes);
+  const size_t hist_bytes = (hc_bytes + (WEIGHTED ? (size_t)nrows * 8 : 0) + 15) & ~(size_t)15;
+  const int wave = threadIdx.x / kWave;
+  float4* Lq = reinterpret_cast<float4*>(smem + hist_bytes) + wave * CAP;
+  int* Lk = reinterpret_cast<int*>(smem + hist_bytes + (size_t)(BLOCK / kWave) * CAP * 16) +
+            wave * CAP;
+  int* Lm = reinterpret_cast<int*>(smem + hist_bytes + (size_t)(BLOCK / kWave) * CAP * 20) +
+            wave * CAP;
   if (!PLAIN)
     for (int t = threadIdx.x; t < a.nt; t += BLOCK) tab[t] = table[t];
   for (int t = threadIdx.x; t <= bins; t += BLOCK) sT[t] = T[t];
@@ -326,74 +444,49 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
     const float4 is = xs[i];   // point 1, scaled
     const float4 io = xo[i];   // point 1, original
     const double wi = (double)io.w;
+    int n = 0;  // list entries (uniform)
 
-    for (int pc = pbeg; pc < pend; pc += kWave) {
-      const int p = pc + lane;
-      const bool active = p < pend;
-      PairRec r;
-      r.xj = r.yj = r.zj = 1e18f;
-      r.wj = 0.f;
-      r.j = -1;
-      r.meta = 0;
-      if (active) r = pairs[p];
-      const int b12 = (r.meta >> 8) & 0xfff;
-      const int t12 = (r.meta >> 20) & 0xfff;
-      const int im2 = r.meta & 31;
-      const int f2 = ((im2 / 9 - 1) != 0 ? 1 : 0) | (((im2 / 3) % 3 - 1) != 0 ? 2 : 0) |
-                     ((im2 % 3 - 1) != 0 ? 4 : 0);
-      const double w12 = wi * (double)r.wj;
-      const unsigned nact = (unsigned)__popcll(__ballot(active));
-      // iso LDS row base of this pair; ani global index base
-      const unsigned rbase = (unsigned)(b12 - a.lo) * (unsigned)bins2;
-      const int64_t abase = ((int64_t)b12 * bins * bins * bins) * bins + (int64_t)t12 * bins;
-      float4 jo = make_float4(0.f, 0.f, 0.f, 0.f);
-      bool jo_loaded = false;
-
-      for_each_segment(io, a, k_cell_start, [&](int k0, int k1, int sx, int sy, int sz) {
-        const float sxL = (float)sx * a.Ls, syL = (float)sy * a.Ls, szL = (float)sz * a.Ls;
-        const int simg = (sx + 1) * 9 + (sy + 1) * 3 + (sz + 1);
-        for (int k = k0; k < k1; ++k) {
-          if (a.auto_ && k == i) continue;
-          const float4 kq = ks_all[k];
-          const float kx = kq.x + sxL, ky = kq.y + syL, kz = kq.z + szL;
-          // ---- side 1-3 (uniform across the wave)
-          const float ax = kx - is.x, ay = ky - is.y, az = kz - is.z;
-          const float d31 = __builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax));
-          unsigned c31 = PLAIN ? ubins + 1 : tab[min(__float2uint_rz(d31), ntm1)];
-          if (c31 == ubins) continue;  // certainly out of range of point 1
-          int b31 = (int)c31, pb = 0, f3 = 0;
-          float4 kco = make_float4(0.f, 0.f, 0.f, 0.f);
-          double t31[3] = {0.0, 0.0, 0.0};
-          const bool need31 = ANI || c31 > ubins;  // exact 1-3 side (uniform)
-          if (need31) {
-            kco = ko_all[k];
-            f3 = image_flags(io, kco, simg, false);
-            if (f3 < 0) continue;
-            const float a1[3] = {io.x, io.y, io.z}, a3[3] = {kco.x, kco.y, kco.z};
-            for (int q = 0; q < 3; ++q) {
-              const double d = __dsub_rn((double)a3[q], (double)a1[q]);
-              t31[q] = a.bpc ? proj_term(d, a.L, (f3 >> q) & 1) : d;
-            }
-            const double s31 = __dadd_rn(
-                __dadd_rn(__dmul_rn(t31[0], t31[0]), __dmul_rn(t31[1], t31[1])),
-                __dmul_rn(t31[2], t31[2]));
-            if (a.auto_ && !(s31 > 0.0)) continue;
-            b31 = class_lds(s31, sT, bins, a.R, 0);
-            if (b31 < 0) continue;
-            if (ANI) {
-              const double cth = __dadd_rn(1.0, t31[2] / sqrt(s31));
-              const double pp = __dmul_rn(cth, a.ds_th);
-              pb = isnan(pp) ? 0 : (int)pp;
-              if (pb > bins - 1) continue;
-            }
-          }
-          evals += nact;
-          const double wk = (double)kq.w;  // the scaled copy keeps the weight
-          // ---- side 2-3 (per lane)
-          const float cx = kx - r.xj, cy = ky - r.yj, cz = kz - r.zj;
+    // ---- phase B: the wave's pairs against the list
+    auto run_list = [&]() {
+      __builtin_amdgcn_wave_barrier();
+      // Lane l of chunk c takes pair l * nch + c: pairs are sorted by b12, so
+      // consecutive pairs share histogram rows; spreading them over the lanes
+      // keeps a wave's LDS atomics off one address.
+      const int npair = pend - pbeg;
+      const int nch = (npair + kWave - 1) / kWave;
+      for (int ch = 0; ch < nch; ++ch) {
+        const int pi = lane * nch + ch;
+        const int p = pbeg + pi;
+        const bool active = pi < npair;
+        PairRec r;
+        r.xj = r.yj = r.zj = 1e18f;
+        r.wj = 0.f;
+        r.j = -1;
+        r.meta = 0;
+        if (active) r = pairs[p];
+        const int b12 = (r.meta >> 8) & 0xfff;
+        const int t12 = (r.meta >> 20) & 0xfff;
+        const int im2 = r.meta & 31;
+        const int f2 = ((im2 / 9 - 1) != 0 ? 1 : 0) | (((im2 / 3) % 3 - 1) != 0 ? 2 : 0) |
+                       ((im2 % 3 - 1) != 0 ? 4 : 0);
+        const double w12 = wi * (double)r.wj;
+        const unsigned nact = (unsigned)__popcll(__ballot(active));
+        evals += (unsigned long long)nact * (unsigned long long)n;
+        const unsigned rbase = (unsigned)(b12 - a.lo) * (unsigned)bins2;
+        const int64_t abase = ((int64_t)b12 * bins * bins * bins) * bins + (int64_t)t12 * bins;
+        float4 jo = make_float4(0.f, 0.f, 0.f, 0.f);
+        bool jo_loaded = false;
+        for (int e = 0; e < n; ++e) {
+          const float4 kq = Lq[e];
+          const int k = Lk[e];
+          const int meta = Lm[e];
+          const int b31 = meta & 0xff;
+          const int pb = (meta >> 8) & 0xff;
+          const int simg = meta >> 16;
+          const float cx = kq.x - r.xj, cy = kq.y - r.yj, cz = kq.z - r.zj;
           const float d23 = __builtin_fmaf(cz, cz, __builtin_fmaf(cy, cy, cx * cx));
           unsigned c23 = PLAIN ? ubins + 1 : tab[min(__float2uint_rz(d23), ntm1)];
-          bool amb = active && c23 > ubins;
+          const bool amb = active && c23 > ubins;
           if (__ballot(amb)) {
             if (!PLAIN) nre += __popcll(__ballot(amb));
             if (amb) {
@@ -403,11 +496,8 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
                   jo = xo[r.j];
                   jo_loaded = true;
                 }
-                int g3 = f3;
-                if (!need31) {
-                  kco = ko_all[k];
-                  g3 = image_flags(io, kco, simg, false);
-                }
+                const float4 kco = ko_all[k];
+                const int g3 = image_flags(io, kco, simg, false);
                 if (g3 >= 0) {
                   const float a2[3] = {jo.x, jo.y, jo.z}, a3[3] = {kco.x, kco.y, kco.z};
                   double t23[3];
@@ -430,17 +520,77 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
             if (!ANI) {
               const unsigned row = rbase + c23 * ubins + (unsigned)b31;
               atomicAdd(&hc[row], 1u);
-              if (WEIGHTED) atomicAdd(&hw[row], __dmul_rn(w12, wk));
+              if (WEIGHTED) atomicAdd(&hw[row], __dmul_rn(w12, (double)kq.w));
             } else {
               const int64_t idx = abase + ((int64_t)b31 * bins * bins * bins +
                                            (int64_t)c23 * bins * bins) + pb;
               atomicAdd(&hist[idx], 1ull);
-              if (WEIGHTED) atomicAdd(&whist[idx], __dmul_rn(w12, wk));
+              if (WEIGHTED) atomicAdd(&whist[idx], __dmul_rn(w12, (double)kq.w));
             }
           }
         }
-      });
-    }
+      }
+      n = 0;
+      __builtin_amdgcn_wave_barrier();
+    };
+
+    // ---- phase A: point 1's third points, one per lane
+    for_each_segment(io, a, k_cell_start, [&](int k0, int k1, int sx, int sy, int sz) {
+      const float sxL = (float)sx * a.Ls, syL = (float)sy * a.Ls, szL = (float)sz * a.Ls;
+      const int simg = (sx + 1) * 9 + (sy + 1) * 3 + (sz + 1);
+      for (int kb = k0; kb < k1; kb += kWave) {
+        const int k = kb + lane;
+        bool v = k < k1 && !(a.auto_ && k == i);
+        float4 kq = make_float4(1e18f, 1e18f, 1e18f, 0.f);
+        if (v) kq = ks_all[k];
+        const float kx = kq.x + sxL, ky = kq.y + syL, kz = kq.z + szL;
+        const float ax = kx - is.x, ay = ky - is.y, az = kz - is.z;
+        const float d31 = __builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax));
+        unsigned c31 = PLAIN ? ubins + 1 : tab[min(__float2uint_rz(d31), ntm1)];
+        if (c31 == ubins) v = false;  // certainly out of range of point 1
+        int b31 = (int)c31, pb = 0;
+        if (v && (ANI || c31 > ubins)) {
+          // exact 1-3 side (and the 3ani cosine bin)
+          const float4 kco = ko_all[k];
+          const int f3 = image_flags(io, kco, simg, false);
+          if (f3 < 0) {
+            v = false;
+          } else {
+            const float a1[3] = {io.x, io.y, io.z}, a3[3] = {kco.x, kco.y, kco.z};
+            double t31[3];
+            for (int q = 0; q < 3; ++q) {
+              const double d = __dsub_rn((double)a3[q], (double)a1[q]);
+              t31[q] = a.bpc ? proj_term(d, a.L, (f3 >> q) & 1) : d;
+            }
+            const double s31 = __dadd_rn(
+                __dadd_rn(__dmul_rn(t31[0], t31[0]), __dmul_rn(t31[1], t31[1])),
+                __dmul_rn(t31[2], t31[2]));
+            if (a.auto_ && !(s31 > 0.0)) v = false;
+            b31 = class_lds(s31, sT, bins, a.R, 0);
+            if (b31 < 0) v = false;
+            if (ANI && v) {
+              const double cth = __dadd_rn(1.0, t31[2] / sqrt(s31));
+              const double pp = __dmul_rn(cth, a.ds_th);
+              pb = isnan(pp) ? 0 : (int)pp;
+              if (pb > bins - 1) v = false;
+            }
+          }
+        }
+        const unsigned long long m = __ballot(v);
+        const int cnt = __popcll(m);
+        if (cnt == 0) continue;
+        if (n + cnt > CAP) run_list();
+        const int pos = n + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        if (v) {
+          Lq[pos] = make_float4(kx, ky, kz, kq.w);
+          Lk[pos] = k;
+          Lm[pos] = b31 | (pb << 8) | (simg << 16);
+        }
+        n += cnt;
+      }
+    });
+    if (n > 0) run_list();
   }
   if (!ANI) {
     __syncthreads();
@@ -451,6 +601,221 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
         if (WEIGHTED) atomicAdd(&whist[base + t], hw[t]);
       }
     }
+  }
+  if (lane == 0) atomicAdd(&stats[0], evals);
+  if (lane == 0 && nre) atomicAdd(&stats[1], nre);
+}
+
+// 3ani with the histogram privatised in LDS.  A slice is one b12 row and a t
+// range [tlo, tlo + tn); a workgroup serves one slice (its LDS holds
+// [b31][b23][t - tlo][p], u32 counts and FP64 weights) and shares the slice's
+// point-1 tasks with the other workgroups of that slice; one launch runs all
+// slices, workgroups given to slices in proportion to their pairs.  A wave
+// owns point 1: its lanes take point 1's third points (the list q, 64 at a
+// time), and point 1's pairs in the slice (contiguous in the list p, which is
+// keyed by point 1, b12, t) stream through scalar loads, one per step, against
+// all lanes.  The third-point list of XXX is the pair list itself (point 3 is
+// any point in range of point 1); XXY's comes from k_emit_cross.
+struct AniSlice {
+  int b12, tlo, tn;
+  int qstride;  // key stride of the third-point list (bins^2 for XXX, 1 for XXY)
+};
+
+// Pairs per (b12, t) over all points 1 (the slices' work estimate).
+__global__ void k_key_mass(const int* __restrict__ off, int64_t nkeys, int bins2,
+                           unsigned long long* __restrict__ mass) {
+  __shared__ unsigned long long m[1600];
+  for (int t = threadIdx.x; t < bins2; t += blockDim.x) m[t] = 0ull;
+  __syncthreads();
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nkeys;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int c = off[k + 1] - off[k];
+    if (c) atomicAdd(&m[k % bins2], (unsigned long long)c);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < bins2; t += blockDim.x)
+    if (m[t]) atomicAdd(&mass[t], m[t]);
+}
+
+template <int WEIGHTED, int PLAIN>
+__global__ __launch_bounds__(kTripBlock) void k_trip_ani_lds(
+    const float4* __restrict__ xo, const float4* __restrict__ ko_all,
+    const PairRec* __restrict__ P, const int* __restrict__ poff,
+    const PairRec* __restrict__ Q, const int* __restrict__ qoff, int* __restrict__ counters,
+    const uint8_t* __restrict__ table, const double* __restrict__ T,
+    unsigned long long* __restrict__ hist, double* __restrict__ whist,
+    unsigned long long* __restrict__ stats, TripArgs a, const AniSlice* __restrict__ slices,
+    const int* __restrict__ wg_slice) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int si = wg_slice[blockIdx.x];
+  const AniSlice sl = slices[si];
+  int* const counter = counters + si;
+  __shared__ uint8_t tab[kMixedTableMax];
+  __shared__ double sT[kMaxTripBins + 2];
+  const int B = a.bins;
+  const int nrows = B * B * sl.tn * B;  // [b31][b23][t - tlo][p]
+  uint32_t* hc = reinterpret_cast<uint32_t*>(smem);
+  double* hw = reinterpret_cast<double*>(smem + (((size_t)nrows * 4 + 15) & ~(size_t)15));
+  if (!PLAIN)
+    for (int t = threadIdx.x; t < a.nt; t += kTripBlock) tab[t] = table[t];
+  for (int t = threadIdx.x; t <= B; t += kTripBlock) sT[t] = T[t];
+  for (int t = threadIdx.x; t < nrows; t += kTripBlock) {
+    hc[t] = 0u;
+    if (WEIGHTED) hw[t] = 0.0;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & (kWave - 1);
+  const unsigned ntm1 = (unsigned)(a.nt - 1);
+  const unsigned ubins = (unsigned)B;
+  const int B2 = B * B;
+  unsigned long long evals = 0, nre = 0;
+  for (;;) {
+    int task = 0;
+    if (lane == 0) task = atomicAdd(counter, 1);
+    const int i = __builtin_amdgcn_readfirstlane(task);
+    if (i >= a.nx) break;
+    const int kb = i * B2 + sl.b12 * B + sl.tlo;
+    const int pbeg = __builtin_amdgcn_readfirstlane(poff[kb]);
+    const int pend = __builtin_amdgcn_readfirstlane(poff[kb + sl.tn]);
+    if (pend <= pbeg) continue;
+    const int qbeg = __builtin_amdgcn_readfirstlane(qoff[i * sl.qstride]);
+    const int qend = __builtin_amdgcn_readfirstlane(qoff[(i + 1) * sl.qstride]);
+    if (qend <= qbeg) continue;
+    const float4 io = xo[i];
+    const double wi = (double)io.w;
+    // Lane l of chunk c takes entry l * nch + c: the list is sorted by (b31,
+    // p), so consecutive entries share histogram rows; spreading them over the
+    // lanes keeps a wave's LDS atomics off one address.
+    const int nq = qend - qbeg;
+    const int nch = (nq + kWave - 1) / kWave;
+    auto load_q = [&](int ch) {
+      PairRec kr;
+      kr.xj = kr.yj = kr.zj = 1e18f;
+      kr.wj = 0.f;
+      kr.j = -1;
+      kr.meta = 0;
+      const int qi = lane * nch + ch;
+      if (qi < nq) kr = Q[qbeg + qi];
+      return kr;
+    };
+    // Point 1's pairs in the slice, 64 at a time in VGPRs (one per lane; all
+    // of them at once for a typical slice), broadcast by readlane.
+    for (int pc = pbeg; pc < pend; pc += kWave) {
+      PairRec mp;
+      mp.xj = mp.yj = mp.zj = 1e18f;
+      mp.wj = 0.f;
+      mp.j = 0;
+      mp.meta = 0;
+      if (pc + lane < pend) mp = P[pc + lane];
+      const int np = min(kWave, pend - pc);
+      auto bcast = [&](int u) {
+        PairRec r;
+        r.xj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mp.xj), u));
+        r.yj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mp.yj), u));
+        r.zj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mp.zj), u));
+        r.wj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mp.wj), u));
+        r.j = __builtin_amdgcn_readlane(mp.j, u);
+        r.meta = __builtin_amdgcn_readlane(mp.meta, u);
+        return r;
+      };
+      PairRec kr_next = load_q(0);
+      for (int ch = 0; ch < nch; ++ch) {
+        const PairRec kr = kr_next;
+        if (ch + 1 < nch) kr_next = load_q(ch + 1);  // in flight while this chunk runs
+        const bool active = lane * nch + ch < nq;
+        const int k = kr.j;
+        const int img3 = kr.meta & 31;
+        const int b31 = (kr.meta >> 8) & 0xfff;
+        const int p31 = (kr.meta >> 20) & 0xfff;
+        const int g3 = ((img3 / 9 - 1) != 0 ? 1 : 0) | (((img3 / 3) % 3 - 1) != 0 ? 2 : 0) |
+                       ((img3 % 3 - 1) != 0 ? 4 : 0);
+        const double wk = (double)kr.wj;
+        const unsigned nact = (unsigned)__popcll(__ballot(active));
+        float4 kco = make_float4(0.f, 0.f, 0.f, 0.f);
+        bool kco_loaded = false;
+        const unsigned rb = ((unsigned)b31 * ubins) * (unsigned)sl.tn * ubins + (unsigned)p31;
+        // The 2-3 side of pair r against every lane's third point: the table
+        // class, or the exact FP64 bin for lanes in an ambiguous cell.
+        auto classify = [&](const PairRec& r, unsigned c23) -> unsigned {
+          const bool amb = active && c23 > ubins;
+          if (__ballot(amb)) {
+            if (!PLAIN) nre += __popcll(__ballot(amb));
+            if (amb) {
+              c23 = ubins;
+              if (!(a.auto_ && k == r.j)) {
+                if (!kco_loaded) {
+                  kco = ko_all[k];
+                  kco_loaded = true;
+                }
+                const float4 jo = xo[r.j];
+                const int im2 = r.meta & 31;
+                const int f2 = ((im2 / 9 - 1) != 0 ? 1 : 0) |
+                               (((im2 / 3) % 3 - 1) != 0 ? 2 : 0) | ((im2 % 3 - 1) != 0 ? 4 : 0);
+                const float a2[3] = {jo.x, jo.y, jo.z}, a3[3] = {kco.x, kco.y, kco.z};
+                double t23[3];
+                for (int qd = 0; qd < 3; ++qd) {
+                  const double d = __dsub_rn((double)a3[qd], (double)a2[qd]);
+                  t23[qd] = a.bpc ? proj_term(d, a.L, ((f2 >> qd) & 1) != ((g3 >> qd) & 1)) : d;
+                }
+                const double s23 = __dadd_rn(
+                    __dadd_rn(__dmul_rn(t23[0], t23[0]), __dmul_rn(t23[1], t23[1])),
+                    __dmul_rn(t23[2], t23[2]));
+                if (!(a.auto_ && !(s23 > 0.0))) {
+                  const int e23 = class_lds(s23, sT, B, a.R, 0);
+                  if (e23 >= 0) c23 = (unsigned)e23;
+                }
+              }
+            }
+          }
+          return c23;
+        };
+        auto lookup = [&](const PairRec& r) -> unsigned {
+          const float cx = kr.xj - r.xj, cy = kr.yj - r.yj, cz = kr.zj - r.zj;
+          const float d23 = __builtin_fmaf(cz, cz, __builtin_fmaf(cy, cy, cx * cx));
+          return PLAIN ? ubins + 1 : tab[min(__float2uint_rz(d23), ntm1)];
+        };
+        auto deposit = [&](const PairRec& r, unsigned c23) {
+          if (active && c23 < ubins) {
+            const int t12 = (r.meta >> 20) & 0xfff;
+            const unsigned row = rb + (c23 * (unsigned)sl.tn + (unsigned)(t12 - sl.tlo)) * ubins;
+            atomicAdd(&hc[row], 1u);
+            if (WEIGHTED) atomicAdd(&hw[row], __dmul_rn(wi * (double)r.wj, wk));
+          }
+        };
+        int u = 0;
+        for (; u + 4 <= np; u += 4) {
+          const PairRec r0 = bcast(u), r1 = bcast(u + 1), r2 = bcast(u + 2), r3 = bcast(u + 3);
+          unsigned c0 = lookup(r0), c1 = lookup(r1), c2 = lookup(r2), c3 = lookup(r3);
+          evals += 4ull * nact;
+          c0 = classify(r0, c0);
+          c1 = classify(r1, c1);
+          c2 = classify(r2, c2);
+          c3 = classify(r3, c3);
+          deposit(r0, c0);
+          deposit(r1, c1);
+          deposit(r2, c2);
+          deposit(r3, c3);
+        }
+        for (; u < np; ++u) {
+          const PairRec r = bcast(u);
+          evals += nact;
+          deposit(r, classify(r, lookup(r)));
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // LDS row (b31, b23, dt, p) -> global [b12][b31][b23][t][p]
+  for (int t = threadIdx.x; t < nrows; t += kTripBlock) {
+    if (!hc[t]) continue;
+    const int p = t % B;
+    const int dt = (t / B) % sl.tn;
+    const int b23 = (t / (B * sl.tn)) % B;
+    const int b31 = t / (B * sl.tn * B);
+    const int64_t idx =
+        ((((int64_t)sl.b12 * B + b31) * B + b23) * B + (sl.tlo + dt)) * B + p;
+    atomicAdd(&hist[idx], (unsigned long long)hc[t]);
+    if (WEIGHTED) atomicAdd(&whist[idx], hw[t]);
   }
   if (lane == 0) atomicAdd(&stats[0], evals);
   if (lane == 0 && nre) atomicAdd(&stats[1], nre);
@@ -473,32 +838,6 @@ __global__ void k_fill_int(int* p, int64_t n, int v) {
   if (t < n) p[t] = v;
 }
 
-// Single-workgroup exclusive scan (counts -> offsets, offsets[n] = total).
-constexpr int kScanBlock = 1024;
-__global__ __launch_bounds__(kScanBlock) void k_scan(const int* counts, int64_t m, int* out) {
-  __shared__ long long part[kScanBlock];
-  const int t = threadIdx.x;
-  const int64_t chunk = (m + kScanBlock - 1) / kScanBlock;
-  const int64_t b = t * chunk;
-  const int64_t e = b + chunk < m ? b + chunk : m;
-  long long s = 0;
-  for (int64_t i = b; i < e; ++i) s += counts[i];
-  part[t] = s;
-  __syncthreads();
-  for (int off = 1; off < kScanBlock; off <<= 1) {
-    long long v = t >= off ? part[t - off] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  long long run = part[t] - s;
-  for (int64_t i = b; i < e; ++i) {
-    int c = counts[i];
-    out[i] = (int)run;
-    run += c;
-  }
-  if (t == kScanBlock - 1) out[m] = (int)part[t];
-}
 
 }  // namespace
 
@@ -603,12 +942,20 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   int npitems = 0;
   HIPCHK(c, build_pair_worklist(wa, c->wl_counts, c->wl_items, &npitems, c->stream));
   const int64_t nx = gx->n;
-  const int64_t nkeys = nx * bins;
+  // 3ani goes through the LDS-privatised kernel when one t value of a b12 row
+  // ([b31][b23][p]) fits the LDS budget and the (point 1, b12, t) keys fit an
+  // int; its pair list is then keyed by t as well.
+  const size_t ani_budget = 160 * 1024 - kMixedTableMax - 8 * (kMaxTripBins + 2) - 1024;
+  const size_t ani_per_t = (size_t)bins * bins * bins * (4 + (weighted ? 8 : 0));
+  const bool ani_lds = ani && ani_per_t <= ani_budget && nx * b64 * b64 + 1 <= (int64_t)INT32_MAX;
+  const int64_t nkeys = ani_lds ? nx * b64 * b64 : nx * bins;
   if (nkeys + 1 > (int64_t)INT32_MAX) return fail(c, PCF_ERR_UNSUPPORTED, "too many pair keys");
-  HIPCHK(c, c->trip_idx.ensure(sizeof(int) * (size_t)(3 * (nkeys + 1) + 8)));
+  const size_t kbytes = (sizeof(int) * (size_t)(3 * (nkeys + 1) + 8) + 15) & ~(size_t)15;
+  HIPCHK(c, c->trip_idx.ensure(kbytes + sizeof(long long) * (size_t)scan_tiles(nkeys)));
   int* d_cnt = c->trip_idx.as<int>();
   int* d_off = d_cnt + (nkeys + 1);
   int* d_cur = d_off + (nkeys + 1);
+  long long* d_tsum = reinterpret_cast<long long*>(c->trip_idx.as<char>() + kbytes);
   HIPCHK(c, launch_zero(d_cnt, sizeof(int) * (size_t)(nkeys + 1), c->stream));
   HIPCHK(c, launch_zero(d_cur, sizeof(int) * (size_t)(nkeys + 1), c->stream));
   EmitArgs ea;
@@ -622,6 +969,7 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   ea.L = (double)L;
   float ds_th_f = (float)bins / 2;  // 3ani_k.cuh:42
   ea.ds_th = (double)ds_th_f;
+  ea.key_t = ani_lds ? 1 : 0;
   const int emit_blocks = std::max(1, c->ncu * 8);
   if (npitems > 0) {
     hipLaunchKernelGGL(k_emit_pairs<0>, dim3(emit_blocks), dim3(kEmitBlock), 0, c->stream, xs,
@@ -629,8 +977,7 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
                        (PairRec*)nullptr, ea);
     HIPCHK(c, hipGetLastError());
   }
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanBlock), 0, c->stream, d_cnt, nkeys, d_off);
-  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, launch_scan(d_cnt, nkeys, d_off, d_tsum, c->stream));
   int npairs = 0;
   HIPCHK(c, hipMemcpyAsync(&npairs, d_off + nkeys, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -676,14 +1023,17 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
     if (!ani) {
       // LDS budget of one 1024-thread workgroup per CU: the slice of b12 rows
       // whose counts (and FP64 weights) fit beside the table.
-      const size_t budget = 160 * 1024 - kMixedTableMax - 8 * (kMaxTripBins + 2) - 1024;
+      const size_t lists = (size_t)(kTripBlock / kWave) * ListCap<0>::value * 24;
+      const size_t budget = 160 * 1024 - kMixedTableMax - 8 * (kMaxTripBins + 2) - 1024 - lists;
       const size_t per_b12 = (size_t)bins * bins * (4 + (weighted ? 8 : 0));
       const int S = (int)std::max<size_t>(1, std::min<size_t>(bins, (budget - 16) / per_b12));
       for (int lo = 0; lo < bins; lo += S) {
         t.lo = lo;
         t.hi = std::min(bins, lo + S);
         const int nrows = (t.hi - t.lo) * bins * bins;
-        const size_t lds = ((size_t)nrows * 4 + 15) / 16 * 16 + (weighted ? (size_t)nrows * 8 : 0);
+        const size_t lds =
+            ((((size_t)nrows * 4 + 15) / 16 * 16 + (weighted ? (size_t)nrows * 8 : 0) + 15) &
+             ~(size_t)15) + lists;
         HIPCHK(c, launch_zero(d_counter, sizeof(int), c->stream));
 #define PCF_TRIP_ISO(W, P)                                                                     \
   do {                                                                                         \
@@ -700,13 +1050,124 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
 #undef PCF_TRIP_ISO
         HIPCHK(c, hipGetLastError());
       }
+    } else if (ani_lds) {
+      // Third-point list: the pair list itself (XXX) or the X x K cross list.
+      const PairRec* d_q = d_pairs;
+      const int* d_qoff = d_off;
+      int qstride = bins * bins;
+      if (!auto_) {
+        WorkListArgs wq = wa;
+        wq.y_cell_start = gk->cell_start;
+        wq.auto_ = 0;
+        int nqitems = 0;
+        HIPCHK(c, build_pair_worklist(wq, c->wl_counts, c->wl_items2, &nqitems, c->stream));
+        const size_t qbytes = (sizeof(int) * (size_t)(3 * (nx + 1) + 8) + 15) & ~(size_t)15;
+        HIPCHK(c, c->trip_qidx.ensure(qbytes + sizeof(long long) * (size_t)scan_tiles(nx)));
+        int* q_cnt = c->trip_qidx.as<int>();
+        int* q_off = q_cnt + (nx + 1);
+        int* q_cur = q_off + (nx + 1);
+        long long* q_tsum = reinterpret_cast<long long*>(c->trip_qidx.as<char>() + qbytes);
+        HIPCHK(c, launch_zero(q_cnt, sizeof(int) * (size_t)(nx + 1), c->stream));
+        HIPCHK(c, launch_zero(q_cur, sizeof(int) * (size_t)(nx + 1), c->stream));
+        EmitArgs eq = ea;
+        eq.nitems = nqitems;
+        eq.key_t = 0;
+        if (nqitems > 0) {
+          hipLaunchKernelGGL(k_emit_cross<0>, dim3(emit_blocks), dim3(kEmitBlock), 0, c->stream,
+                             xs, gx->pts, ksc, gk->pts, c->wl_items2.as<PairItem>(), d_ktab, d_T,
+                             q_cnt, q_off, q_cur, (PairRec*)nullptr, eq);
+          HIPCHK(c, hipGetLastError());
+        }
+        HIPCHK(c, launch_scan(q_cnt, nx, q_off, q_tsum, c->stream));
+        int nq = 0;
+        HIPCHK(c, hipMemcpyAsync(&nq, q_off + nx, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (nq < 0) return fail(c, PCF_ERR_UNSUPPORTED, "third-point list exceeds 2^31 records");
+        HIPCHK(c, c->trip_q.ensure(sizeof(PairRec) * (size_t)std::max(nq, 1)));
+        if (nqitems > 0 && nq > 0) {
+          hipLaunchKernelGGL(k_emit_cross<1>, dim3(emit_blocks), dim3(kEmitBlock), 0, c->stream,
+                             xs, gx->pts, ksc, gk->pts, c->wl_items2.as<PairItem>(), d_ktab, d_T,
+                             q_cnt, q_off, q_cur, c->trip_q.as<PairRec>(), eq);
+          HIPCHK(c, hipGetLastError());
+        }
+        d_q = c->trip_q.as<PairRec>();
+        d_qoff = q_off;
+        qstride = 1;
+      }
+      // Slices and their workgroups (proportional to the pairs in the slice).
+      const int tn = (int)std::max<size_t>(1, std::min<size_t>(bins, ani_budget / ani_per_t));
+      HIPCHK(c, c->misc_buf.ensure(sizeof(unsigned long long) * (size_t)bins * bins));
+      unsigned long long* d_mass = c->misc_buf.as<unsigned long long>();
+      HIPCHK(c, launch_zero(d_mass, sizeof(unsigned long long) * (size_t)bins * bins, c->stream));
+      hipLaunchKernelGGL(k_key_mass, dim3(c->ncu * 4), dim3(256), 0, c->stream, d_off, nkeys,
+                         bins * bins, d_mass);
+      HIPCHK(c, hipGetLastError());
+      std::vector<unsigned long long> mass((size_t)bins * bins);
+      HIPCHK(c, hipMemcpyAsync(mass.data(), d_mass, sizeof(unsigned long long) * mass.size(),
+                               hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      std::vector<AniSlice> slices;
+      std::vector<double> wslice;
+      double wtot = 0;
+      for (int b12 = 0; b12 < bins; ++b12)
+        for (int tlo = 0; tlo < bins; tlo += tn) {
+          AniSlice sl;
+          sl.b12 = b12;
+          sl.tlo = tlo;
+          sl.tn = std::min(tn, bins - tlo);
+          sl.qstride = qstride;
+          double w = 0;
+          for (int q = tlo; q < tlo + sl.tn; ++q) w += (double)mass[(size_t)b12 * bins + q];
+          if (w <= 0) continue;
+          slices.push_back(sl);
+          wslice.push_back(w);
+          wtot += w;
+        }
+      std::vector<int> wg;
+      const int G = c->ncu * 4;  // workgroups of the launch (about 4 rounds of one per CU)
+      for (size_t q = 0; q < slices.size(); ++q) {
+        const int nwg = std::max(1, (int)std::lround(G * wslice[q] / wtot));
+        for (int r = 0; r < nwg; ++r) wg.push_back((int)q);
+      }
+      const size_t slice_bytes = sizeof(AniSlice) * std::max<size_t>(1, slices.size());
+      const size_t sbytes = (slice_bytes + 15) & ~(size_t)15;
+      HIPCHK(c, c->rules_buf.ensure(sbytes + sizeof(int) * std::max<size_t>(1, wg.size()) +
+                                    sizeof(int) * (slices.size() + 1)));
+      AniSlice* d_slices = c->rules_buf.as<AniSlice>();
+      int* d_wg = reinterpret_cast<int*>(c->rules_buf.as<char>() + sbytes);
+      int* d_ctr = d_wg + wg.size();
+      if (!slices.empty()) {
+        HIPCHK(c, hipMemcpyAsync(d_slices, slices.data(), sizeof(AniSlice) * slices.size(),
+                                 hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(d_wg, wg.data(), sizeof(int) * wg.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, launch_zero(d_ctr, sizeof(int) * slices.size(), c->stream));
+        const size_t nrows = (size_t)bins * bins * tn * bins;
+        const size_t lds = ((nrows * 4 + 15) & ~(size_t)15) + (weighted ? nrows * 8 : 0);
+        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+#define PCF_TRIP_ANI_LDS(W, P)                                                                 \
+  do {                                                                                         \
+    HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_trip_ani_lds<W, P>),         \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));      \
+    hipLaunchKernelGGL((k_trip_ani_lds<W, P>), dim3((unsigned)wg.size()), dim3(kTripBlock),    \
+                       lds, c->stream, gx->pts, gk->pts, d_pairs, d_off, d_q, d_qoff, d_ctr,   \
+                       d_ktab, d_T, d_V, d_VW, d_stats, t, d_slices, d_wg);                    \
+  } while (0)
+        if (weighted && plain) PCF_TRIP_ANI_LDS(1, 1);
+        else if (weighted) PCF_TRIP_ANI_LDS(1, 0);
+        else if (plain) PCF_TRIP_ANI_LDS(0, 1);
+        else PCF_TRIP_ANI_LDS(0, 0);
+#undef PCF_TRIP_ANI_LDS
+        HIPCHK(c, hipGetLastError());
+      }
     } else {
       t.lo = 0;
       t.hi = bins;
       HIPCHK(c, launch_zero(d_counter, sizeof(int), c->stream));
       const int blocks = c->ncu * 4;
+      const size_t lds = (size_t)(kAniTripBlock / kWave) * ListCap<1>::value * 24;
 #define PCF_TRIP_ANI(W, P)                                                                     \
-  hipLaunchKernelGGL((k_triplets<1, W, P>), dim3(blocks), dim3(kAniTripBlock), 0, c->stream,   \
+  hipLaunchKernelGGL((k_triplets<1, W, P>), dim3(blocks), dim3(kAniTripBlock), lds, c->stream, \
                      xs, gx->pts, ksc, gk->pts, gk->cell_start, d_pairs, d_off, d_counter,     \
                      d_ktab, d_T, d_V, d_VW, d_stats, t)
       if (weighted && plain) PCF_TRIP_ANI(1, 1);

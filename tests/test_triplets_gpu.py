@@ -160,3 +160,12 @@ def test_cfg2_shape_symmetry(engine):
         assert np.all(ratio >= 0.125 - 1e-12) and np.all(ratio <= 3.375 + 1e-12)
     finally:
         engine.release(cd)
+
+
+@pytest.mark.parametrize("cross", [False, True])
+def test_3ani_many_bins_global_histogram(engine, cross):
+    """bins = 24 weighted: one [b31][b23][p] slice of the LDS kernel does not fit,
+    so the 5-D histogram goes through the global-atomic kernel."""
+    d = synth.neyman_scott(10, 30, 5.0, 250.0, seed=61, weights="uniform")
+    r = synth.uniform(300, 250.0, seed=62, weights="uniform")
+    _check(engine, d, r if cross else None, ani=True, bpc=True, box=250.0, bins=24, dmax=60.0)
