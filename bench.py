@@ -58,11 +58,14 @@ def run_step(e, cd, cr, precision, rebuild=True):
     evals = 0
     kms = 0.0
     rech = 0
-    for hist, cy in ((da.PCF_XX, -1), (da.PCF_XX, None), (da.PCF_XY, cr)):
+    # DD re-grids D, RR re-grids R; DR uses the two grids just built (the
+    # three share one geometry).
+    for hist, cy, rb in ((da.PCF_XX, -1, rebuild), (da.PCF_XX, None, rebuild),
+                         (da.PCF_XY, cr, False)):
         cx = cr if cy is None else cd
         job = da.make_job(da.PCF_2ISO, hist, bins=BINS, dmax=RMAX, cat_x=cx,
                           cat_y=-1 if cy is None else cy, precision=precision,
-                          rebuild_grid=rebuild)
+                          rebuild_grid=rb)
         c, _ = e.histogram(job)
         st = e.stats()
         tot_pairs += int(c.sum())

@@ -8,6 +8,8 @@
 #include <mutex>
 #include <string>
 #include <vector>
+#include <tuple>
+#include <map>
 
 #include "../../include/pcf_abi.h"
 #include "pcf_internal.h"
@@ -103,8 +105,56 @@ static double err_bound(double d2, double A) {
 // coordinates by sigma = sqrt(inv_h) (a power of two, exact in FP32) and
 // convert d2 straight to a table index.  *nt_out <= nt_max entries cover
 // [0, Vmax) where everything beyond Vmax is certainly out of range.
+static int build_table_uncached(const BinRules& r, double A, int nt_max,
+                                std::vector<uint8_t>* table, float* inv_h_out, int* nt_out);
+
+// The table depends only on (bins, ani, R, ds, le, zero, A, nt_max) and takes
+// ~1 ms of host time to build: built tables are kept (process-wide, bounded).
 int build_table(const BinRules& r, double A, int nt_max, std::vector<uint8_t>* table,
                 float* inv_h_out, int* nt_out) {
+  struct Key {
+    int bins, ani, le, zero, nt_max;
+    double R, ds, A;
+    bool operator<(const Key& o) const {
+      return std::tie(bins, ani, le, zero, nt_max, R, ds, A) <
+             std::tie(o.bins, o.ani, o.le, o.zero, o.nt_max, o.R, o.ds, o.A);
+    }
+  };
+  struct Val {
+    std::vector<uint8_t> table;
+    float inv_h;
+    int nt;
+    int rc;
+  };
+  static std::mutex mu;
+  static std::map<Key, Val> cache;
+  const Key key{r.bins, r.ani, r.le, r.zero, nt_max, r.R, r.ds, A};
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      *table = it->second.table;
+      *inv_h_out = it->second.inv_h;
+      *nt_out = it->second.nt;
+      return it->second.rc;
+    }
+  }
+  Val v;
+  v.inv_h = 1.f;
+  v.nt = 2;
+  v.rc = build_table_uncached(r, A, nt_max, &v.table, &v.inv_h, &v.nt);
+  *table = v.table;
+  *inv_h_out = v.inv_h;
+  *nt_out = v.nt;
+  const int rc = v.rc;
+  std::lock_guard<std::mutex> lk(mu);
+  if (cache.size() >= 64) cache.erase(cache.begin());
+  cache.emplace(key, std::move(v));
+  return rc;
+}
+
+static int build_table_uncached(const BinRules& r, double A, int nt_max,
+                                std::vector<uint8_t>* table, float* inv_h_out, int* nt_out) {
   const int bins = r.bins;
   const bool ani = r.ani != 0;
   double U = std::min(r.R, r.T[bins]);
@@ -237,11 +287,15 @@ int choose_geometry(const Catalog* X, const Catalog* Y, float dmax, int bpc, flo
         g->lo[a] = 0.f;
         g->cs[a] = (float)((double)box / P);
       } else {
-        double ext = hi[a] - lo[a];
+        // The origin snaps to a multiple of the cell edge, so catalogs with
+        // about the same bounds (DD, RR and DR of one survey) share the
+        // geometry and hence their resident grids.
+        const double lo_s = std::floor(lo[a] / cs_target) * cs_target;
+        double ext = hi[a] - lo_s;
         int P = (int)std::floor(ext / cs_target) + 1;
         if (P < 1) P = 1;
         g->P[a] = P;
-        g->lo[a] = (float)lo[a];
+        g->lo[a] = (float)lo_s;
         g->cs[a] = (float)cs_target;
       }
       g->inv_cs[a] = 1.0f / g->cs[a];
