@@ -332,20 +332,16 @@ DevGrid* get_grid(pcf_ctx* c, Catalog* cat, const GridGeom& g, bool force_rebuil
   std::lock_guard<std::mutex> lk(c->store->mu);
   for (auto& gr : cat->grids) {
     if (gr->valid && gr->g.same(g)) {
+      hipError_t e = hipSuccess;
       if (force_rebuild && gr->users == 0) {
-        hipError_t e = build_grid(cat->raw, cat->n, g, gr.get(), c->scratch_cells,
-                                  c->scratch_perm, c->stream);
-        if (e == hipSuccess) e = hipEventRecord(gr->ready, c->stream);
-        if (e != hipSuccess) {
-          *rc = hipfail(c, e, "build_grid");
-          return nullptr;
-        }
-      } else {
-        hipError_t e = hipStreamWaitEvent(c->stream, gr->ready, 0);
-        if (e != hipSuccess) {
-          *rc = hipfail(c, e, "grid wait");
-          return nullptr;
-        }
+        e = build_grid(cat->raw, cat->n, g, gr.get(), c->scratch_cells, c->scratch_perm,
+                       c->grid_stream);
+        if (e == hipSuccess) e = hipEventRecord(gr->ready, c->grid_stream);
+      }
+      if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, gr->ready, 0);
+      if (e != hipSuccess) {
+        *rc = hipfail(c, e, "build_grid");
+        return nullptr;
       }
       gr->users++;
       return gr.get();
@@ -366,8 +362,10 @@ DevGrid* get_grid(pcf_ctx* c, Catalog* cat, const GridGeom& g, bool force_rebuil
   auto gr = std::make_unique<DevGrid>();
   hipError_t e = hipEventCreateWithFlags(&gr->ready, hipEventDisableTiming);
   if (e == hipSuccess)
-    e = build_grid(cat->raw, cat->n, g, gr.get(), c->scratch_cells, c->scratch_perm, c->stream);
-  if (e == hipSuccess) e = hipEventRecord(gr->ready, c->stream);
+    e = build_grid(cat->raw, cat->n, g, gr.get(), c->scratch_cells, c->scratch_perm,
+                   c->grid_stream);
+  if (e == hipSuccess) e = hipEventRecord(gr->ready, c->grid_stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, gr->ready, 0);
   if (e != hipSuccess) {
     free_grid(gr.get());
     *rc = (e == hipErrorOutOfMemory) ? fail(c, PCF_ERR_NOMEM, "grid allocation")
@@ -389,9 +387,9 @@ hipError_t ensure_grid64(pcf_ctx* c, DevGrid* g) {
   std::lock_guard<std::mutex> lk(c->store->mu);
   hipError_t e;
   if (!g->pts64) {
-    if ((e = hipStreamWaitEvent(c->stream, g->ready, 0)) != hipSuccess) return e;
-    if ((e = build_grid64(g, c->stream)) != hipSuccess) return e;
-    return hipEventRecord(g->ready, c->stream);
+    if ((e = hipStreamWaitEvent(c->grid_stream, g->ready, 0)) != hipSuccess) return e;
+    if ((e = build_grid64(g, c->grid_stream)) != hipSuccess) return e;
+    if ((e = hipEventRecord(g->ready, c->grid_stream)) != hipSuccess) return e;
   }
   return hipStreamWaitEvent(c->stream, g->ready, 0);
 }
@@ -490,13 +488,15 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   // Histogram + counters.
   const size_t hist_bytes = sizeof(unsigned long long) * nflat;
   const size_t w_bytes = sizeof(double) * nflat;
-  HIPCHK(c, c->hist_buf.ensure(hist_bytes + w_bytes + 64));
+  // tail: 4 u64 statistics, then the work cursors (8 ranges x 64 bytes)
+  const size_t tail = 32 + 8 * 64;
+  HIPCHK(c, c->hist_buf.ensure(hist_bytes + w_bytes + tail));
   unsigned long long* d_hist = c->hist_buf.as<unsigned long long>();
   double* d_w = reinterpret_cast<double*>(c->hist_buf.as<char>() + hist_bytes);
   unsigned long long* d_stats =
       reinterpret_cast<unsigned long long*>(c->hist_buf.as<char>() + hist_bytes + w_bytes);
   int* d_counter = reinterpret_cast<int*>(d_stats + 4);
-  HIPCHK(c, launch_zero(c->hist_buf.p, hist_bytes + w_bytes + 64, c->stream));
+  HIPCHK(c, launch_zero(c->hist_buf.p, hist_bytes + w_bytes + tail, c->stream));
 
   PairLaunch pl{};
   pl.xp = gx->pts;
@@ -624,6 +624,8 @@ int pcf_create(pcf_ctx** out, int32_t device) {
   if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) goto fail;
   c->ncu = prop.multiProcessorCount;
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) goto fail;
+  if ((e = hipStreamCreateWithFlags(&c->grid_stream, hipStreamNonBlocking)) != hipSuccess)
+    goto fail;
   for (auto& ev : c->ev)
     if ((e = hipEventCreate(&ev)) != hipSuccess) goto fail;
   *out = c;
@@ -656,7 +658,9 @@ void pcf_destroy(pcf_ctx* c) {
     b->release();
   for (auto& ev : c->ev)
     if (ev) (void)hipEventDestroy(ev);
+  if (c->grid_stream) (void)hipStreamSynchronize(c->grid_stream);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->grid_stream) (void)hipStreamDestroy(c->grid_stream);
   delete c;
 }
 
@@ -769,6 +773,7 @@ int create_lane(pcf_ctx* parent, pcf_ctx** out) {
   c->lane = true;
   hipError_t e = hipSetDevice(c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->grid_stream, hipStreamNonBlocking);
   for (auto& ev : c->ev)
     if (e == hipSuccess) e = hipEventCreate(&ev);
   if (e != hipSuccess) {

@@ -281,12 +281,14 @@ int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p, GridPins
     p->grids.push_back(grid_of[id]);
   }
 
-  // Capture the run into a graph.
+  // Capture the run into a graph (serving lanes run their batches directly:
+  // their job sets rarely repeat).
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   p->exec = nullptr;
   p->graph = nullptr;
   HIPCHK(c, enqueue_plan(c, p, (size_t)njobs, true));
+  if (c->lane) return PCF_OK;
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   hipError_t e = enqueue_plan(c, p, (size_t)njobs, false);
   hipGraph_t graph = nullptr;
@@ -411,7 +413,8 @@ int run_pairs_batch(pcf_ctx* c, const pcf_job* jobs_in, int njobs, uint64_t* cou
   }
   BatchPlan* p = c->batch;
   HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-  HIPCHK(c, hipGraphLaunch(p->exec, c->stream));
+  if (p->exec) HIPCHK(c, hipGraphLaunch(p->exec, c->stream));
+  else HIPCHK(c, enqueue_plan(c, p, (size_t)njobs, false));
   HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
   const size_t nf = (size_t)p->nflat * njobs;
   std::vector<unsigned long long> hc(nf + 8);

@@ -83,15 +83,20 @@ __device__ __forceinline__ unsigned pair_row(const float4 q, float xs, float ys,
     const unsigned t = min(__float2uint_rz(d2), ntm1);
     return tab[t];
   } else {
+    // 2ani reads two table cells: lanes with a component beyond the table's
+    // last cell (out of range) skip both reads.
     const float dz2 = dz * dz;
     const float do2 = __builtin_fmaf(dy, dy, dx * dx);
-    const unsigned tz = min(__float2uint_rz(dz2), ntm1);
-    const unsigned to = min(__float2uint_rz(do2), ntm1);
-    const unsigned cz = tab[tz];
-    const unsigned co = tab[to];
-    const unsigned m = max(cz, co);
-    const unsigned r = cz * bins + co;
-    return m < bins ? r : (m == kAniOut8 ? out_row : amb_row);
+    const unsigned tz = __float2uint_rz(dz2);
+    const unsigned to = __float2uint_rz(do2);
+    unsigned r = out_row;
+    if (tz < ntm1 && to < ntm1) {
+      const unsigned cz = tab[tz];
+      const unsigned co = tab[to];
+      const unsigned m = max(cz, co);
+      r = m < bins ? cz * bins + co : (m == kAniOut8 ? out_row : amb_row);
+    }
+    return r;
   }
 }
 
@@ -122,6 +127,7 @@ __device__ __forceinline__ unsigned exact_row(const float4 pi, const float4 pj, 
 }
 
 constexpr int kBlockMixed = kMixedBlock;
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, each with its own L2
 constexpr int kMaxTable = kMixedTableMax;
 
 // xp/yp: points scaled by sigma (bulk arithmetic); xp_orig/yp_orig: the same
@@ -196,8 +202,12 @@ __global__ __launch_bounds__(kBlockMixed, 4) void k_pairs_mixed(
     const double wi = (double)pi.w;
 
     auto deposit = [&](unsigned row, float wj) {
-      atomicAdd(hc_lane + (row << CS), 1u);
-      if (WEIGHTED) atomicAdd(hw_lane + (row << CS), wi * (double)wj);
+      // 2iso deposits every lane (a branch costs more than the LDS op);
+      // 2ani, with most candidates out of range, skips excluded pairs.
+      if (!ANI || row < out_row) {
+        atomicAdd(hc_lane + (row << CS), 1u);
+        if (WEIGHTED) atomicAdd(hw_lane + (row << CS), wi * (double)wj);
+      }
     };
     auto fix = [&](unsigned& r, int j) {
       // Lanes whose table cell straddles an edge: exact FP64 bin.
@@ -270,15 +280,25 @@ __global__ __launch_bounds__(kBlockMixed, 4) void k_pairs_mixed(
   };
 
   if constexpr (!BATCH) {
-    // A wave claims k.claim consecutive items per atomic: the claim's round
-    // trip to L2 (the wave waits on it) is paid once per k.claim items.
-    for (;;) {
-      int base = 0;
-      if (lane == 0) base = atomicAdd(counter, k.claim);
-      base = __builtin_amdgcn_readfirstlane(base);
-      if (base >= k.nitems) break;
-      const int end = min(base + k.claim, k.nitems);
-      for (int it = base; it < end; ++it) run_item(it, xp, yp, xp_orig, yp_orig, k.zero);
+    // The item list (cell order) is cut into kXcds contiguous ranges, one per
+    // XCD: workgroups are dealt to the XCDs round robin, so the waves of
+    // workgroup b first drain range b % kXcds (neighbouring cells, one L2),
+    // then help the other ranges.  A wave claims k.claim consecutive items per
+    // atomic: the claim's round trip (the wave waits on it) is paid once per
+    // k.claim items.  counter[r * 16] is range r's cursor (own cache line).
+    const int home = blockIdx.x % kXcds;
+    for (int rr = 0; rr < kXcds; ++rr) {
+      const int r = (home + rr) % kXcds;
+      const int rb = (int)(((long long)k.nitems * r) / kXcds);
+      const int re = (int)(((long long)k.nitems * (r + 1)) / kXcds);
+      for (;;) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(counter + r * 16, k.claim);
+        base = rb + __builtin_amdgcn_readfirstlane(base);
+        if (base >= re) break;
+        const int end = min(base + k.claim, re);
+        for (int it = base; it < end; ++it) run_item(it, xp, yp, xp_orig, yp_orig, k.zero);
+      }
     }
     __syncthreads();
     flush(0, false);
@@ -359,15 +379,25 @@ __global__ __launch_bounds__(kBlock) void k_pairs_fp64(
   const int lane = threadIdx.x & (kWave - 1);
   const int copy = lane & (ncopy - 1);
   unsigned long long evals = 0;
+  // The same XCD-aware ranges and multi-item claims as the mixed kernel.
+  const int home = blockIdx.x % kXcds;
+  int rr = 0, r = home;
+  int rb = (int)(((long long)k.nitems * r) / kXcds);
+  int re = (int)(((long long)k.nitems * (r + 1)) / kXcds);
   int it = 0, claim_end = 0;
   for (;; ++it) {
-    if (it >= claim_end) {  // claim k.claim items per atomic, as the mixed kernel
+    while (it >= claim_end) {
       int base = 0;
-      if (lane == 0) base = atomicAdd(counter, k.claim);
-      it = __builtin_amdgcn_readfirstlane(base);
-      claim_end = min(it + k.claim, k.nitems);
+      if (lane == 0) base = atomicAdd(counter + r * 16, k.claim);
+      it = rb + __builtin_amdgcn_readfirstlane(base);
+      claim_end = min(it + k.claim, re);
+      if (it < re) break;
+      if (++rr == kXcds) break;  // every range drained
+      r = (home + rr) % kXcds;
+      rb = (int)(((long long)k.nitems * r) / kXcds);
+      re = (int)(((long long)k.nitems * (r + 1)) / kXcds);
     }
-    if (it >= k.nitems) break;
+    if (rr == kXcds) break;
     const int i0 = __builtin_amdgcn_readfirstlane(items[it].i0);
     const int ni = __builtin_amdgcn_readfirstlane(items[it].ni);
     const int j0 = __builtin_amdgcn_readfirstlane(items[it].j0);

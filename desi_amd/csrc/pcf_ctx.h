@@ -39,6 +39,7 @@ struct pcf_ctx {
   int device = 0;
   int ncu = 0;
   hipStream_t stream = nullptr;
+  hipStream_t grid_stream = nullptr;  // grid builds (never captured into a graph)
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   std::string err;
   std::shared_ptr<pcf::CatalogStore> store;
