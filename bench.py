@@ -445,7 +445,7 @@ def main():
                    "n_data": N, "n_random": N, "box": BOX, "rmax": RMAX, "bins": BINS},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
-                     "traffic": profiled_traffic(PAIR_PROFILE, "k_pairs_mixed<0, 0, 6>")},
+                     "traffic": profiled_traffic(PAIR_PROFILE, "k_pairs_mixed NS")},
         "pair_kernel_ms_per_step": kms / args.steps,
         "evaluations_per_step": evals // args.steps,
         "rechecks_per_step": rech // args.steps,
