@@ -464,13 +464,26 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   if (!plain && build_table(rules, A, nt_max, &table, &inv_h, &nt) != 0)
     return fail(c, PCF_ERR_UNSUPPORTED, "bin table");
   const float sigma = std::sqrt(inv_h);  // exact power of two
-  HIPCHK(c, c->rules_buf.ensure(sizeof(BinRules)));
-  HIPCHK(c, c->table_buf.ensure(nt));
-  HIPCHK(c, hipMemcpyAsync(c->rules_buf.p, &rules, sizeof(BinRules), hipMemcpyHostToDevice,
-                           c->stream));
-  if (!plain)
-    HIPCHK(c, hipMemcpyAsync(c->table_buf.p, table.data(), nt,
-                             hipMemcpyHostToDevice, c->stream));
+  // Rules and table in one device buffer, uploaded from page-locked staging
+  // (asynchronous) and only when they differ from the last upload.  The
+  // staging bytes are not touched again before this call's final sync.
+  const size_t rt_bytes = sizeof(BinRules) + (plain ? 0 : (size_t)nt);
+  const size_t hist_bytes = sizeof(unsigned long long) * nflat;
+  const size_t w_bytes = sizeof(double) * nflat;
+  const size_t rd_bytes = hist_bytes + w_bytes + 32;  // histogram, weights, statistics
+  const size_t rd_off = (rt_bytes + 255) & ~(size_t)255;
+  HIPCHK(c, c->pin.ensure(rd_off + rd_bytes));
+  HIPCHK(c, c->pair_rt.ensure(rt_bytes));
+  {
+    uint8_t* st = c->pin.as<uint8_t>();
+    std::memcpy(st, &rules, sizeof(BinRules));
+    if (!plain) std::memcpy(st + sizeof(BinRules), table.data(), nt);
+    if (c->pair_rt_host.size() != rt_bytes ||
+        std::memcmp(c->pair_rt_host.data(), st, rt_bytes) != 0) {
+      HIPCHK(c, hipMemcpyAsync(c->pair_rt.p, st, rt_bytes, hipMemcpyHostToDevice, c->stream));
+      c->pair_rt_host.assign(st, st + rt_bytes);
+    }
+  }
 
   // Work list.
   WorkListArgs wa;
@@ -482,12 +495,19 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   wa.auto_ = auto_ ? 1 : 0;
   wa.cylinder = ani ? 1 : 0;
   wa.map = nullptr;
+  // The list is sized by its bound and its length stays on the device (no
+  // host round trip), unless the bound is too large to allocate casually.
   int nitems = 0;
-  HIPCHK(c, build_pair_worklist(wa, c->wl_counts, c->wl_items, &nitems, c->stream));
+  const int* nitems_dev = nullptr;
+  const int64_t cap = worklist_bound(wa, gx->n);
+  if (cap <= ((int64_t)1 << 25)) {
+    HIPCHK(c, build_pair_worklist_async(wa, cap, c->wl_counts, c->wl_items, &nitems_dev,
+                                        c->stream));
+  } else {
+    HIPCHK(c, build_pair_worklist(wa, c->wl_counts, c->wl_items, &nitems, c->stream));
+  }
 
   // Histogram + counters.
-  const size_t hist_bytes = sizeof(unsigned long long) * nflat;
-  const size_t w_bytes = sizeof(double) * nflat;
   // tail: 4 u64 statistics, then the work cursors (8 ranges x 64 bytes)
   const size_t tail = 32 + 8 * 64;
   HIPCHK(c, c->hist_buf.ensure(hist_bytes + w_bytes + tail));
@@ -518,8 +538,10 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   pl.yp64 = gy->pts64;
   pl.items = c->wl_items.as<PairItem>();
   pl.nitems = nitems;
+  pl.nitems_dev = nitems_dev;
+  pl.nitems_cap = (int)std::min<int64_t>(cap, INT32_MAX);
   pl.counter = d_counter;
-  pl.table = c->table_buf.as<uint8_t>();
+  pl.table = c->pair_rt.as<uint8_t>() + sizeof(BinRules);
   pl.nt = nt;
   pl.inv_h = inv_h;
   pl.L = L * sigma;
@@ -531,7 +553,7 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   pl.hist = d_hist;
   pl.whist = d_w;
   pl.stats = d_stats;
-  pl.rules = c->rules_buf.as<BinRules>();
+  pl.rules = c->pair_rt.as<BinRules>();
   pl.ani = ani ? 1 : 0;
   pl.weighted = weighted ? 1 : 0;
   pl.plain64 = plain ? 1 : 0;
@@ -540,30 +562,36 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   pl.le = rules.le;
   pl.zero = rules.zero;
   pl.nblocks = std::max(1, c->ncu * sh.per_cu);
-  if (nitems > 0) {
+  const bool launched = nitems_dev != nullptr || nitems > 0;
+  if (launched) {
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     HIPCHK(c, launch_pairs(pl, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
   }
-  std::vector<unsigned long long> hc(nflat + 8);
-  std::vector<double> hw(nflat);
-  HIPCHK(c, hipMemcpyAsync(hc.data(), d_hist, hist_bytes, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(hw.data(), d_w, w_bytes, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(hc.data() + nflat, d_stats, 4 * sizeof(unsigned long long),
-                           hipMemcpyDeviceToHost, c->stream));
+  // Histogram, weights and statistics are contiguous: one read-back.
+  uint8_t* rd = c->pin.as<uint8_t>() + rd_off;
+  HIPCHK(c, hipMemcpyAsync(rd, d_hist, rd_bytes, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  const unsigned long long* hc = reinterpret_cast<const unsigned long long*>(rd);
+  const double* hw = reinterpret_cast<const double*>(rd + hist_bytes);
+  const unsigned long long* hs = reinterpret_cast<const unsigned long long*>(rd + hist_bytes + w_bytes);
+  if (nitems_dev && (int64_t)hs[2] > cap) {
+    // More items than the allocation: the bound did not hold and items were
+    // dropped.  Never expected; an error rather than a short histogram.
+    return fail(c, PCF_ERR_HIP, "work list exceeded its bound");
+  }
   const uint64_t mult = auto_ ? 2 : 1;
   for (int b = 0; b < nflat; ++b) {
     if (counts) counts[b] = (uint64_t)hc[b] * mult;
     if (weights) weights[b] = weighted ? hw[b] * (double)mult : (double)((uint64_t)hc[b] * mult);
   }
   pcf_stats st{};
-  st.evaluations = hc[nflat];
-  st.rechecks = hc[nflat + 1];
-  st.work_items = (uint64_t)nitems;
+  st.evaluations = hs[0];
+  st.rechecks = hs[1];
+  st.work_items = nitems_dev ? hs[2] : (uint64_t)nitems;
   float ms = 0.f;
-  if (nitems > 0) {
+  if (launched) {
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));
     st.kernel_ms = ms;
   }
@@ -654,8 +682,10 @@ void pcf_destroy(pcf_ctx* c) {
   for (DevBuf* b : {&c->scratch_cells, &c->scratch_perm, &c->wl_counts, &c->wl_items,
                     &c->table_buf, &c->rules_buf, &c->hist_buf, &c->misc_buf, &c->stats_partials,
                     &c->scaled_x, &c->scaled_y, &c->trip_idx, &c->trip_pairs, &c->trip_hist,
-                    &c->wl_items2, &c->trip_qidx, &c->trip_q})
+                    &c->wl_items2, &c->trip_qidx, &c->trip_q, &c->pair_rt})
     b->release();
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->pin.release();
   for (auto& ev : c->ev)
     if (ev) (void)hipEventDestroy(ev);
   if (c->grid_stream) (void)hipStreamSynchronize(c->grid_stream);

@@ -4,9 +4,10 @@
 // re-allocating the node's array on every insert (cuda/create_grid.cuh:81-138),
 // then linearises the non-empty nodes and copies them to the device
 // (cuda/main.cu:215-329).  Here the grid is a counting sort on the device:
-//   1. k_cell_index  : cell of every point + per-cell counts (global atomics)
+//   1. k_cell_index  : cell of every point, per-cell counts (global atomics)
+//                      and the point's rank in its cell (the atomic's return)
 //   2. launch_scan   : cell_start[] = exclusive prefix sum of the counts
-//   3. k_scatter     : points copied to their cell's range, SoA-friendly float4
+//   3. k_scatter     : point i copied to cell_start[cell] + rank, float4
 // The resulting layout is cell-ordered (x-major, z fastest) so that every z-range
 // of a (cx,cy) column is one contiguous, coalesced point range.
 #include <algorithm>
@@ -72,8 +73,11 @@ __device__ __forceinline__ int cell_axis(float v, float lo, float inv_cs, int P)
   return c >= P ? P - 1 : c;
 }
 
+// Cell of every point and its rank among its cell's points (the counting
+// atomic's return value), so the scatter needs no second atomic pass.
 __global__ void k_cell_index(const float4* __restrict__ raw, int64_t n, GridGeom g,
-                             int* __restrict__ cell_of, int* __restrict__ counts) {
+                             int* __restrict__ cell_of, int* __restrict__ rank,
+                             int* __restrict__ counts) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float4 p = raw[i];
@@ -82,18 +86,16 @@ __global__ void k_cell_index(const float4* __restrict__ raw, int64_t n, GridGeom
   int cz = cell_axis(p.z, g.lo[2], g.inv_cs[2], g.P[2]);
   int c = (cx * g.P[1] + cy) * g.P[2] + cz;
   cell_of[i] = c;
-  atomicAdd(&counts[c], 1);
+  rank[i] = atomicAdd(&counts[c], 1);
 }
 
 
 __global__ void k_scatter(const float4* __restrict__ raw, int64_t n, const int* __restrict__ cell_of,
-                          const int* __restrict__ cell_start, int* __restrict__ cursor,
+                          const int* __restrict__ rank, const int* __restrict__ cell_start,
                           float4* __restrict__ out) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  int c = cell_of[i];
-  int pos = cell_start[c] + atomicAdd(&cursor[c], 1);
-  out[pos] = raw[i];
+  out[cell_start[cell_of[i]] + rank[i]] = raw[i];
 }
 
 __global__ void k_to_double(const float4* __restrict__ in, int64_t n, double4* __restrict__ out) {
@@ -148,23 +150,24 @@ hipError_t build_grid(const float4* raw, int64_t n, const GridGeom& g, DevGrid* 
   if ((e = scratch_cells.ensure(cbytes + sizeof(long long) * (size_t)scan_tiles(nc))) !=
       hipSuccess)
     return e;
-  if ((e = scratch_perm.ensure(sizeof(int) * (n > 0 ? n : 1))) != hipSuccess) return e;
+  if ((e = scratch_perm.ensure(2 * sizeof(int) * (n > 0 ? n : 1))) != hipSuccess) return e;
   int* counts = scratch_cells.as<int>();
   int* cell_of = scratch_perm.as<int>();
+  int* rank = cell_of + (n > 0 ? n : 1);
   if ((e = launch_zero(counts, sizeof(int) * (nc + 1), s)) != hipSuccess) return e;
   const int blocks = (int)((n + 255) / 256);
   if (n > 0) {
-    hipLaunchKernelGGL(k_cell_index, dim3(blocks), dim3(256), 0, s, raw, n, g, cell_of, counts);
+    hipLaunchKernelGGL(k_cell_index, dim3(blocks), dim3(256), 0, s, raw, n, g, cell_of, rank,
+                       counts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   if ((e = launch_scan(counts, nc, out->cell_start,
                        reinterpret_cast<long long*>(scratch_cells.as<char>() + cbytes), s)) !=
       hipSuccess)
     return e;
-  if ((e = launch_zero(counts, sizeof(int) * nc, s)) != hipSuccess) return e;
   if (n > 0) {
-    hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, raw, n, cell_of,
-                       out->cell_start, counts, out->pts);
+    hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, raw, n, cell_of, rank,
+                       out->cell_start, out->pts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   out->valid = true;

@@ -40,9 +40,10 @@ __global__ void k_worklist_count(WorkListArgs a, int* __restrict__ counts) {
 }
 
 __global__ void k_worklist_fill(WorkListArgs a, const int* __restrict__ offsets,
-                                PairItem* __restrict__ items) {
+                                PairItem* __restrict__ items, int64_t cap) {
   const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= a.g.ncell()) return;
+  if ((int64_t)offsets[cell + 1] > cap) return;  // beyond the list's allocation
   enumerate_cell<true>(a, (int)cell, items + offsets[cell]);
 }
 
@@ -62,8 +63,26 @@ struct KArgs {
   double Lx;    // box side, unscaled
   int le;
   int zero;
-  int claim;    // work items a wave claims per atomic (mixed kernel)
+  int claim;    // work items a wave claims per atomic (0: derived from the count)
+  const int* nitems_dev;  // item count in device memory (nullptr: nitems)
+  int cap;                // items allocated (the count is clamped to it)
 };
+
+// Item count of a launch (from device memory when the list was built without
+// a host round trip) and the claim granularity: up to 8 items per atomic
+// while every wave still gets about 8 claims (tail balance).
+__device__ __forceinline__ void launch_items(const KArgs& k, int* nitems, int* claim) {
+  int n = k.nitems;
+  if (k.nitems_dev) n = min(__builtin_amdgcn_readfirstlane(*k.nitems_dev), k.cap);
+  *nitems = n;
+  int cl = k.claim;
+  if (cl <= 0) {
+    const long long waves = (long long)gridDim.x * (blockDim.x / kWave);
+    const long long c = (long long)n / (waves * 8);
+    cl = (int)(c < 1 ? 1 : (c > 8 ? 8 : c));
+  }
+  *claim = cl;
+}
 
 // FP32 bulk + FP64 edge re-evaluation.
 //
@@ -283,20 +302,24 @@ __global__ __launch_bounds__(kBlockMixed, 4) void k_pairs_mixed(
     // The item list (cell order) is cut into kXcds contiguous ranges, one per
     // XCD: workgroups are dealt to the XCDs round robin, so the waves of
     // workgroup b first drain range b % kXcds (neighbouring cells, one L2),
-    // then help the other ranges.  A wave claims k.claim consecutive items per
+    // then help the other ranges.  A wave claims `claim` consecutive items per
     // atomic: the claim's round trip (the wave waits on it) is paid once per
-    // k.claim items.  counter[r * 16] is range r's cursor (own cache line).
+    // `claim` items.  counter[r * 16] is range r's cursor (own cache line).
+    int nitems, claim;
+    launch_items(k, &nitems, &claim);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+      stats[2] = (unsigned long long)(k.nitems_dev ? *k.nitems_dev : k.nitems);
     const int home = blockIdx.x % kXcds;
     for (int rr = 0; rr < kXcds; ++rr) {
       const int r = (home + rr) % kXcds;
-      const int rb = (int)(((long long)k.nitems * r) / kXcds);
-      const int re = (int)(((long long)k.nitems * (r + 1)) / kXcds);
+      const int rb = (int)(((long long)nitems * r) / kXcds);
+      const int re = (int)(((long long)nitems * (r + 1)) / kXcds);
       for (;;) {
         int base = 0;
-        if (lane == 0) base = atomicAdd(counter + r * 16, k.claim);
+        if (lane == 0) base = atomicAdd(counter + r * 16, claim);
         base = rb + __builtin_amdgcn_readfirstlane(base);
         if (base >= re) break;
-        const int end = min(base + k.claim, re);
+        const int end = min(base + claim, re);
         for (int it = base; it < end; ++it) run_item(it, xp, yp, xp_orig, yp_orig, k.zero);
       }
     }
@@ -380,22 +403,26 @@ __global__ __launch_bounds__(kBlock) void k_pairs_fp64(
   const int copy = lane & (ncopy - 1);
   unsigned long long evals = 0;
   // The same XCD-aware ranges and multi-item claims as the mixed kernel.
+  int nitems, claim;
+  launch_items(k, &nitems, &claim);
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+      stats[2] = (unsigned long long)(k.nitems_dev ? *k.nitems_dev : k.nitems);
   const int home = blockIdx.x % kXcds;
   int rr = 0, r = home;
-  int rb = (int)(((long long)k.nitems * r) / kXcds);
-  int re = (int)(((long long)k.nitems * (r + 1)) / kXcds);
+  int rb = (int)(((long long)nitems * r) / kXcds);
+  int re = (int)(((long long)nitems * (r + 1)) / kXcds);
   int it = 0, claim_end = 0;
   for (;; ++it) {
     while (it >= claim_end) {
       int base = 0;
-      if (lane == 0) base = atomicAdd(counter + r * 16, k.claim);
+      if (lane == 0) base = atomicAdd(counter + r * 16, claim);
       it = rb + __builtin_amdgcn_readfirstlane(base);
-      claim_end = min(it + k.claim, re);
+      claim_end = min(it + claim, re);
       if (it < re) break;
       if (++rr == kXcds) break;  // every range drained
       r = (home + rr) % kXcds;
-      rb = (int)(((long long)k.nitems * r) / kXcds);
-      re = (int)(((long long)k.nitems * (r + 1)) / kXcds);
+      rb = (int)(((long long)nitems * r) / kXcds);
+      re = (int)(((long long)nitems * (r + 1)) / kXcds);
     }
     if (rr == kXcds) break;
     const int i0 = __builtin_amdgcn_readfirstlane(items[it].i0);
@@ -496,7 +523,35 @@ hipError_t build_pair_worklist(const WorkListArgs& a, DevBuf& counts, DevBuf& it
   if (total == 0) return hipSuccess;
   if ((e = items.ensure(sizeof(PairItem) * (size_t)total)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_worklist_fill, dim3(blocks), dim3(128), 0, s, a, off,
-                     items.as<PairItem>());
+                     items.as<PairItem>(), (int64_t)total);
+  return hipGetLastError();
+}
+
+int64_t worklist_bound(const WorkListArgs& a, int64_t n_tile_points) {
+  const int64_t tiles = (n_tile_points + kTile - 1) / kTile + (int64_t)a.g.P[0] * a.g.P[1];
+  const int64_t cols = (int64_t)(2 * a.K[0] + 1) * (2 * a.K[1] + 1);
+  return tiles * cols * (a.g.bpc ? 3 : 1);
+}
+
+hipError_t build_pair_worklist_async(const WorkListArgs& a, int64_t cap, DevBuf& counts,
+                                     DevBuf& items, const int** total_dev, hipStream_t s) {
+  hipError_t e;
+  const int64_t nc = a.g.ncell();
+  const size_t ibytes = (sizeof(int) * (size_t)(2 * nc + 2) + 15) & ~(size_t)15;
+  if ((e = counts.ensure(ibytes + sizeof(long long) * (size_t)scan_tiles(nc))) != hipSuccess)
+    return e;
+  if ((e = items.ensure(sizeof(PairItem) * (size_t)std::max<int64_t>(cap, 1))) != hipSuccess)
+    return e;
+  int* cnt = counts.as<int>();
+  int* off = cnt + nc + 1;
+  long long* tsum = reinterpret_cast<long long*>(counts.as<char>() + ibytes);
+  const int blocks = (int)((nc + 127) / 128);
+  hipLaunchKernelGGL(k_worklist_count, dim3(blocks), dim3(128), 0, s, a, cnt);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_scan(cnt, nc, off, tsum, s)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_worklist_fill, dim3(blocks), dim3(128), 0, s, a, off,
+                     items.as<PairItem>(), cap);
+  *total_dev = off + nc;
   return hipGetLastError();
 }
 
@@ -542,7 +597,7 @@ hipError_t build_batch_worklists(const std::vector<WorkListArgs>& args, DevBuf& 
   if ((e = items.ensure(sizeof(PairItem) * (size_t)total)) != hipSuccess) return e;
   for (int k = 0; k < nj; ++k) {
     hipLaunchKernelGGL(k_worklist_fill, dim3(blocks), dim3(128), 0, s, args[k], off + k * nc,
-                       items.as<PairItem>());
+                       items.as<PairItem>(), (int64_t)total);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   return hipSuccess;
@@ -561,13 +616,10 @@ hipError_t launch_pairs(const PairLaunch& pl, hipStream_t s) {
   k.Lx = pl.Lx;
   k.le = pl.le;
   k.zero = pl.zero;
-  // Claim granularity: up to 8 items per atomic while every wave still gets
-  // about 8 claims (tail balance).
-  {
-    const long long waves = (long long)pl.nblocks * (kBlockMixed / kWave);
-    long long cl = waves > 0 ? (long long)pl.nitems / (waves * 8) : 1;
-    k.claim = (int)(cl < 1 ? 1 : (cl > 8 ? 8 : cl));
-  }
+  // Claim granularity: derived in the kernel from the item count.
+  k.claim = 0;
+  k.nitems_dev = pl.batch ? nullptr : pl.nitems_dev;
+  k.cap = pl.nitems_cap;
   const int ncopy = 1 << pl.cshift;
   dim3 grid(pl.nblocks), block(kBlock);
   hipError_t e;

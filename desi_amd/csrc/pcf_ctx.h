@@ -47,6 +47,11 @@ struct pcf_ctx {
   pcf::DevBuf scratch_cells, scratch_perm, wl_counts, wl_items, table_buf, rules_buf, hist_buf,
       misc_buf, stats_partials, scaled_x, scaled_y, trip_idx, trip_pairs, trip_hist, wl_items2,
       trip_qidx, trip_q;
+  // Pair path: BinRules + d2 -> bin table on the device, and the host bytes
+  // last uploaded there (an unchanged job skips the upload).
+  pcf::DevBuf pair_rt;
+  std::vector<uint8_t> pair_rt_host;
+  pcf::PinnedBuf pin;  // staging for uploads and result read-backs
   pcf_stats last{};
   pcf::JobEngine* engine = nullptr;  // asynchronous serving engine (jobs.hip)
   pcf::BatchPlan* batch = nullptr;   // last batched launch plan + graph (batch.hip)

@@ -111,6 +111,30 @@ struct DevBuf {
   T* as() const { return reinterpret_cast<T*>(p); }
 };
 
+// Page-locked host staging memory (small uploads and result read-backs that
+// stay asynchronous on the stream).
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t need) {
+    if (need <= bytes) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    size_t b = need + need / 4 + 256;
+    hipError_t e = hipHostMalloc(&p, b, hipHostMallocDefault);
+    if (e == hipSuccess) bytes = b;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const { return reinterpret_cast<T*>(p); }
+};
+
 // ---- device entry points (defined in grid.hip / pairs.hip) -------------
 hipError_t launch_catalog_stats(const float4* raw, int64_t n, float* partials, int nblocks,
                                 hipStream_t s);
@@ -141,6 +165,8 @@ struct PairLaunch {
   const double4* yp64;
   const PairItem* items;
   int nitems;
+  const int* nitems_dev = nullptr;  // item count in device memory (set: nitems unused)
+  int nitems_cap = 0;               // items allocated when nitems_dev is set
   int* counter;
   const uint8_t* table;
   int nt;
@@ -151,7 +177,7 @@ struct PairLaunch {
   int cshift;     // log2 of histogram copies per workgroup
   unsigned long long* hist;
   double* whist;
-  unsigned long long* stats;  // [0] evaluations, [1] rechecks
+  unsigned long long* stats;  // [0] evaluations, [1] rechecks, [2] work items
   const BinRules* rules;      // device copy
   int ani;
   int weighted;
@@ -230,5 +256,15 @@ struct WorkListArgs {
 // count through *nitems (a synchronous read of one integer).
 hipError_t build_pair_worklist(const WorkListArgs& a, DevBuf& counts, DevBuf& items,
                                int* nitems, hipStream_t s);
+// Upper bound of the work list's length: one item per (tile, neighbour column,
+// z segment), tiles = ceil(column points / 64) summed over the columns.
+int64_t worklist_bound(const WorkListArgs& a, int64_t n_tile_points);
+// The same work list without the host round trip: `items` is sized for `cap`
+// items (cap >= the count, e.g. worklist_bound) and the count stays on the
+// device at *total_dev for the counting kernel to read.  The fill skips any
+// cell whose items would end beyond cap, so memory stays in bounds even if
+// cap were short; the kernels clamp the count to cap and report it.
+hipError_t build_pair_worklist_async(const WorkListArgs& a, int64_t cap, DevBuf& counts,
+                                     DevBuf& items, const int** total_dev, hipStream_t s);
 
 }  // namespace pcf
