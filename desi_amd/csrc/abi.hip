@@ -431,7 +431,9 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   }
   const int nflat = ani ? bins * bins : bins;
   const bool weighted = !(X->all_w1 && Y->all_w1);
-  const bool plain = job->precision == PCF_PREC_FP64;
+  // The mixed kernel's recheck queue indexes j-side points with 27 bits.
+  const bool plain = job->precision == PCF_PREC_FP64 || X->n >= kRecheckMaxPoints ||
+                     Y->n >= kRecheckMaxPoints;
 
   GridGeom g;
   int K[3];

@@ -363,6 +363,9 @@ int run_pairs_batch(pcf_ctx* c, const pcf_job* jobs_in, int njobs, uint64_t* cou
   // The plain FP64 path and out-of-range arguments go through the single-job
   // driver (same kernels, one launch per job).
   bool single = j0.precision == PCF_PREC_FP64;
+  for (int k = 0; k < njobs && !single; ++k)
+    for (pcf_catalog id : {jobs[k].cat_x, jobs[k].hist == PCF_XY ? jobs[k].cat_y : jobs[k].cat_x})
+      if (get_cat(c, id)->n >= kRecheckMaxPoints) single = true;
   if (!single) {
     // Argument checks of the single-job driver (range, box) on each job.
     const int bins = j0.bins;

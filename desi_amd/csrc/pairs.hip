@@ -175,6 +175,9 @@ __global__ __launch_bounds__(kBlockMixed, 4) void k_pairs_mixed(
   const int hc_bytes = (nrows * ncopy * 4 + 15) & ~15;
   double* sT = reinterpret_cast<double*>(smem + hc_bytes);
   double* hw = sT + ((k.bins + 2) & ~1);
+  // Per-wave queue of the pairs whose table cell straddles an edge.
+  int2* const wq =
+      reinterpret_cast<int2*>(hw + (WEIGHTED ? nrows * ncopy : 0)) + (threadIdx.x >> 6) * kRecheckCap;
   for (int t = threadIdx.x; t < k.nt; t += kBlockMixed) tab[t] = table[t];
   for (int t = threadIdx.x; t < nrows * ncopy; t += kBlockMixed) {
     hc[t] = 0u;
@@ -192,6 +195,27 @@ __global__ __launch_bounds__(kBlockMixed, 4) void k_pairs_mixed(
   const unsigned bins = (unsigned)k.bins;
   unsigned long long evals = 0;
   unsigned long long nre = 0;
+  int qn = 0;  // entries in this wave's recheck queue (wave-uniform)
+
+  // Exact FP64 bins of the newest `nd` queued pairs, one pair per lane (the
+  // oracle's arithmetic on the unscaled inputs), deposited like the bulk.
+  auto drain = [&](const int nd, const float4* __restrict__ xo,
+                   const float4* __restrict__ yo, const int zero) {
+    __builtin_amdgcn_wave_barrier();  // the queue was written by other lanes
+    if (lane < nd) {
+      const int2 e = wq[qn - nd + lane];
+      const float4 pa = xo[e.x];
+      const float4 pb = yo[e.y & kRecheckIndexMask];
+      const unsigned r = exact_row<ANI>(pa, pb, (int)((unsigned)e.y >> kRecheckImgShift), sT,
+                                        k.bins, k.R, k.le, zero, k.Lx, out_row);
+      if (r < out_row) {
+        atomicAdd(hc_lane + (r << CS), 1u);
+        if (WEIGHTED) atomicAdd(hw_lane + (r << CS), (double)pa.w * (double)pb.w);
+      }
+    }
+    qn -= nd;
+    __builtin_amdgcn_wave_barrier();
+  };
 
   auto run_item = [&](const int it, const float4* __restrict__ xp,
                       const float4* __restrict__ yp, const float4* __restrict__ xp_orig,
@@ -229,12 +253,21 @@ __global__ __launch_bounds__(kBlockMixed, 4) void k_pairs_mixed(
       }
     };
     auto fix = [&](unsigned& r, int j) {
-      // Lanes whose table cell straddles an edge: exact FP64 bin.
+      // Lanes whose table cell straddles an edge: the pair goes to the wave's
+      // queue (counted out here) and is binned exactly when 64 are queued,
+      // one per lane, instead of running the FP64 path for a lane or two.
       const unsigned long long amb = __ballot(r == amb_row);
       if (amb) {
-        nre += __popcll(amb);
-        if (r == amb_row)
-          r = exact_row<ANI>(pi, yp_orig[j], img, sT, k.bins, k.R, k.le, zero, k.Lx, out_row);
+        const int na = __popcll(amb);
+        nre += na;
+        if (r == amb_row) {
+          const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi(
+                                   (unsigned)(amb >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)amb, 0u));
+          wq[pos] = make_int2(my_i, j | (img << kRecheckImgShift));
+          r = out_row;
+        }
+        qn += na;
+        if (qn >= kWave) drain(kWave, xp_orig, yp_orig, zero);
       }
     };
     auto quad = [&](int j, const float4 q0, const float4 q1, const float4 q2, const float4 q3) {
@@ -323,6 +356,7 @@ __global__ __launch_bounds__(kBlockMixed, 4) void k_pairs_mixed(
         for (int it = base; it < end; ++it) run_item(it, xp, yp, xp_orig, yp_orig, k.zero);
       }
     }
+    if (qn > 0) drain(qn, xp_orig, yp_orig, k.zero);
     __syncthreads();
     flush(0, false);
   } else {
@@ -344,6 +378,7 @@ __global__ __launch_bounds__(kBlockMixed, 4) void k_pairs_mixed(
         if (it >= ce) break;
         run_item(it, jd.xs, jd.ys, jd.xo, jd.yo, jd.zero);
       }
+      if (qn > 0) drain(qn, jd.xo, jd.yo, jd.zero);
       __syncthreads();
       flush(jd.hoff, true);
       __syncthreads();
@@ -628,7 +663,7 @@ hipError_t launch_pairs(const PairLaunch& pl, hipStream_t s) {
     const size_t hc_bytes = (size_t)(((pl.nflat + 2) * ncopy * 4 + 15) & ~15);
     const size_t t_bytes = (size_t)((pl.bins + 2) & ~1) * 8;
     const size_t hw_bytes = pl.weighted ? (size_t)(pl.nflat + 2) * ncopy * 8 : 0;
-    const size_t lds = hc_bytes + t_bytes + hw_bytes;
+    const size_t lds = hc_bytes + t_bytes + hw_bytes + kRecheckQueueBytes;
     const dim3 mblock(kBlockMixed);
 #define PCF_LAUNCH_MIXED1(A, W, C, B)                                                          \
   do {                                                                                         \

@@ -195,6 +195,14 @@ struct PairLaunch {
   int nchunks = 0;
 };
 constexpr int kMixedBlock = 512;       // threads per workgroup of the mixed kernel
+// Recheck queue of the mixed pair kernel: per wave up to 2 x 64 entries of
+// (i index, j index | image << 27), so j-side catalogs need n < 2^27 (larger
+// ones take the plain FP64 kernel, which gives the same results).
+constexpr int kRecheckCap = 128;
+constexpr int kRecheckImgShift = 27;
+constexpr int kRecheckIndexMask = (1 << kRecheckImgShift) - 1;
+constexpr int64_t kRecheckMaxPoints = (int64_t)1 << kRecheckImgShift;
+constexpr size_t kRecheckQueueBytes = (size_t)(kMixedBlock / 64) * kRecheckCap * 8;
 constexpr int kMixedTableMax = 28672;  // static LDS bytes of its d2 -> bin table
 
 // Launch shape of the pair kernels: 2^cshift LDS copies of each histogram row
@@ -223,8 +231,8 @@ inline PairShape pair_shape(int nflat, int bins, bool weighted, bool plain, int 
   }
   PairShape sh;
   sh.cshift = cs;
-  sh.lds_bytes = (plain ? 0 : (size_t)kMixedTableMax) + (size_t)rows * ((size_t)1 << cs) * per +
-                 (size_t)(bins + 2) * 8;
+  sh.lds_bytes = (plain ? 0 : (size_t)kMixedTableMax + kRecheckQueueBytes) +
+                 (size_t)rows * ((size_t)1 << cs) * per + (size_t)(bins + 2) * 8;
   const int threads = plain ? 256 : kMixedBlock;
   int per_cu = (int)((160 * 1024) / (sh.lds_bytes + 512));
   if (per_cu < 1) per_cu = 1;
