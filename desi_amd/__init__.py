@@ -21,6 +21,7 @@ TYPES = {"2iso": PCF_2ISO, "2ani": PCF_2ANI, "3iso": PCF_3ISO, "3ani": PCF_3ANI}
 PCF_XX, PCF_XY, PCF_XXX, PCF_XXY = 0, 1, 2, 3
 # Arithmetic.
 PCF_PREC_MIXED, PCF_PREC_FP64 = 0, 1
+PCF_PAIR_AUTO, PCF_PAIR_DIRECT, PCF_PAIR_GRAM = 0, 1, 2
 
 ERRORS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "out of device memory",
           -4: "coordinates out of range", -5: "unsupported", -6: "busy"}
@@ -41,13 +42,14 @@ class PcfJob(ctypes.Structure):
                 ("cat_y", ctypes.c_int32), ("partitions", ctypes.c_int32),
                 ("cells_per_rmax", ctypes.c_int32), ("rebuild_grid", ctypes.c_int32),
                 ("lds_copies_log2", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 2)]
+                ("pair_kernel", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class PcfTuning(ctypes.Structure):
     _fields_ = [("cells_per_rmax", ctypes.c_int32), ("lds_copies_log2", ctypes.c_int32),
                 ("blocks_per_cu", ctypes.c_int32), ("trials", ctypes.c_int32),
-                ("best_ms", ctypes.c_double), ("default_ms", ctypes.c_double)]
+                ("best_ms", ctypes.c_double), ("default_ms", ctypes.c_double),
+                ("pair_kernel", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -59,7 +61,7 @@ class PcfStats(ctypes.Structure):
                 ("total_ms", ctypes.c_double), ("cells_per_rmax", ctypes.c_int32),
                 ("graph_replay", ctypes.c_int32), ("chunks", ctypes.c_int32),
                 ("pairs_listed", ctypes.c_int32), ("batched_jobs", ctypes.c_int32),
-                ("reserved", ctypes.c_int32), ("queue_ms", ctypes.c_double),
+                ("direct_items", ctypes.c_int32), ("queue_ms", ctypes.c_double),
                 ("latency_ms", ctypes.c_double)]
 
     def as_dict(self):
@@ -121,7 +123,7 @@ def hist_size(type_, bins):
 
 def make_job(type_, hist, *, bins, dmax, cat_x, cat_y=-1, bpc=False, box=0.0,
              precision=PCF_PREC_MIXED, cells_per_rmax=0, partitions=35, rebuild_grid=False,
-             lds_copies_log2=0, blocks_per_cu=0):
+             lds_copies_log2=0, blocks_per_cu=0, pair_kernel=0):
     j = PcfJob()
     j.type = int(type_)
     j.hist = int(hist)
@@ -137,6 +139,7 @@ def make_job(type_, hist, *, bins, dmax, cat_x, cat_y=-1, bpc=False, box=0.0,
     j.rebuild_grid = int(bool(rebuild_grid))
     j.lds_copies_log2 = int(lds_copies_log2)
     j.blocks_per_cu = int(blocks_per_cu)
+    j.pair_kernel = int(pair_kernel)
     return j
 
 

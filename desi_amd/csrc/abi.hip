@@ -101,23 +101,41 @@ static double err_bound(double d2, double A) {
   return 2.0 * (5 * u * d2 + 3.5 * u * A * std::sqrt(d2) + 6 * u * u * A * A) + 1e-300;
 }
 
+// Bound on |d2_fp32 - d2_exact| for the Gram form of the 2iso kernel (matrix
+// cores), for tiles whose coordinates relative to the tile's box centre are at
+// most rho in length.  With d the true separation, |x_i'| <= rho and
+// |x_j'| <= rho + d:
+//   |x_i'|^2 and |x_j'|^2 (mul, fma, fma):          3u rho^2 + 3u (rho + d)^2
+//   the four chained fmas of v_mfma_f32_16x16x4f32: 4u (2 rho + d)^2
+//   rounding of the shifted coordinates (image shift, origin), per axis at
+//   most u (A + 2 rho + d), moving d2 by 2 sqrt(3) d u (A + 2 rho + d)
+// plus second-order terms, doubled for margin.
+static double err_bound_gram(double d2, double A, double rho) {
+  const double u = std::ldexp(1.0, -24);
+  const double d = std::sqrt(d2);
+  const double e = 3 * rho * rho + 3 * (rho + d) * (rho + d) + 4 * (2 * rho + d) * (2 * rho + d) +
+                   2 * std::sqrt(3.0) * d * (A + 2 * rho + d);
+  const double w = A + 2 * rho + d;
+  return 2.0 * (u * e + 12 * u * u * w * w) + 1e-300;
+}
+
 // The table scale inv_h is a power of four, so the bulk path can pre-scale
 // coordinates by sigma = sqrt(inv_h) (a power of two, exact in FP32) and
 // convert d2 straight to a table index.  *nt_out <= nt_max entries cover
 // [0, Vmax) where everything beyond Vmax is certainly out of range.
-static int build_table_uncached(const BinRules& r, double A, int nt_max,
+static int build_table_uncached(const BinRules& r, double A, double rho, int nt_max,
                                 std::vector<uint8_t>* table, float* inv_h_out, int* nt_out);
 
 // The table depends only on (bins, ani, R, ds, le, zero, A, nt_max) and takes
 // ~1 ms of host time to build: built tables are kept (process-wide, bounded).
 int build_table(const BinRules& r, double A, int nt_max, std::vector<uint8_t>* table,
-                float* inv_h_out, int* nt_out) {
+                float* inv_h_out, int* nt_out, double rho) {
   struct Key {
     int bins, ani, le, zero, nt_max;
-    double R, ds, A;
+    double R, ds, A, rho;
     bool operator<(const Key& o) const {
-      return std::tie(bins, ani, le, zero, nt_max, R, ds, A) <
-             std::tie(o.bins, o.ani, o.le, o.zero, o.nt_max, o.R, o.ds, o.A);
+      return std::tie(bins, ani, le, zero, nt_max, R, ds, A, rho) <
+             std::tie(o.bins, o.ani, o.le, o.zero, o.nt_max, o.R, o.ds, o.A, o.rho);
     }
   };
   struct Val {
@@ -128,7 +146,7 @@ int build_table(const BinRules& r, double A, int nt_max, std::vector<uint8_t>* t
   };
   static std::mutex mu;
   static std::map<Key, Val> cache;
-  const Key key{r.bins, r.ani, r.le, r.zero, nt_max, r.R, r.ds, A};
+  const Key key{r.bins, r.ani, r.le, r.zero, nt_max, r.R, r.ds, A, rho};
   {
     std::lock_guard<std::mutex> lk(mu);
     auto it = cache.find(key);
@@ -142,7 +160,7 @@ int build_table(const BinRules& r, double A, int nt_max, std::vector<uint8_t>* t
   Val v;
   v.inv_h = 1.f;
   v.nt = 2;
-  v.rc = build_table_uncached(r, A, nt_max, &v.table, &v.inv_h, &v.nt);
+  v.rc = build_table_uncached(r, A, rho, nt_max, &v.table, &v.inv_h, &v.nt);
   *table = v.table;
   *inv_h_out = v.inv_h;
   *nt_out = v.nt;
@@ -153,12 +171,18 @@ int build_table(const BinRules& r, double A, int nt_max, std::vector<uint8_t>* t
   return rc;
 }
 
-static int build_table_uncached(const BinRules& r, double A, int nt_max,
+static int build_table_uncached(const BinRules& r, double A, double rho, int nt_max,
                                 std::vector<uint8_t>* table, float* inv_h_out, int* nt_out) {
   const int bins = r.bins;
   const bool ani = r.ani != 0;
+  // Error margin of the paths sharing the table: the direct FP32 difference
+  // form, and with rho >= 0 also the Gram form.
+  auto err = [&](double d2) {
+    const double e = err_bound(d2, A);
+    return rho >= 0 ? std::max(e, err_bound_gram(d2, A, rho)) : e;
+  };
   double U = std::min(r.R, r.T[bins]);
-  double vmax = U * (1.0 + 1e-3) + 4.0 * err_bound(2 * U + 1, A) + 1e-30;
+  double vmax = U * (1.0 + 1e-3) + 4.0 * err(2 * U + 1) + 1e-30;
   int e2 = (int)std::floor(std::log2((double)(nt_max - 2) / vmax) / 2.0);
   if (e2 > 60) e2 = 60;
   if (e2 < -60) e2 = -60;
@@ -176,8 +200,8 @@ static int build_table_uncached(const BinRules& r, double A, int nt_max,
     double v_lo = (double)t * (1.0 - 4 * u) / ih;
     double v_hi = (t == nt - 1) ? std::numeric_limits<double>::infinity()
                                 : (double)(t + 1) * (1.0 + 4 * u) / ih;
-    double d_lo = v_lo - 2.0 * err_bound(2 * v_lo + 1, A);
-    double d_hi = std::isinf(v_hi) ? v_hi : v_hi + 2.0 * err_bound(2 * v_hi + 1, A);
+    double d_lo = v_lo - 2.0 * err(2 * v_lo + 1);
+    double d_hi = std::isinf(v_hi) ? v_hi : v_hi + 2.0 * err(2 * v_hi + 1);
     uint8_t code;
     if (r.zero != 0 && d_lo <= 0.0) {
       code = amb_code;  // exact zero is excluded, anything above is bin 0
@@ -239,7 +263,9 @@ int finish_upload(pcf_ctx* c, Catalog* cat) {
 namespace pcf {
 
 int choose_geometry(const Catalog* X, const Catalog* Y, float dmax, int bpc, float box,
-                    int cells_per_rmax, GridGeom* g, int K[3], float* reach_out, int* m_out) {
+                    int cells_per_rmax, GridGeom* g, int K[3], float* reach_out, int* m_out,
+                    int zsub) {
+  if (zsub < 1) zsub = 1;
   const double u = std::ldexp(1.0, -24);
   double lo[3], hi[3];
   for (int a = 0; a < 3; ++a) {
@@ -280,8 +306,9 @@ int choose_geometry(const Catalog* X, const Catalog* Y, float dmax, int bpc, flo
   for (int iter = 0; iter < 64; ++iter) {
     int64_t ncell = 1;
     for (int a = 0; a < 3; ++a) {
+      const double cst = a == 2 ? cs_target / zsub : cs_target;
       if (bpc) {
-        int P = (int)std::floor((double)box / cs_target);
+        int P = (int)std::floor((double)box / cst);
         if (P < 3) P = 3;
         g->P[a] = P;
         g->lo[a] = 0.f;
@@ -290,13 +317,13 @@ int choose_geometry(const Catalog* X, const Catalog* Y, float dmax, int bpc, flo
         // The origin snaps to a multiple of the cell edge, so catalogs with
         // about the same bounds (DD, RR and DR of one survey) share the
         // geometry and hence their resident grids.
-        const double lo_s = std::floor(lo[a] / cs_target) * cs_target;
+        const double lo_s = std::floor(lo[a] / cst) * cst;
         double ext = hi[a] - lo_s;
-        int P = (int)std::floor(ext / cs_target) + 1;
+        int P = (int)std::floor(ext / cst) + 1;
         if (P < 1) P = 1;
         g->P[a] = P;
         g->lo[a] = (float)lo_s;
-        g->cs[a] = (float)cs_target;
+        g->cs[a] = (float)cst;
       }
       g->inv_cs[a] = 1.0f / g->cs[a];
       ncell *= g->P[a];
@@ -408,6 +435,22 @@ void free_catalog(Catalog* cat) {
 // ---------------------------------------------------------------------------
 namespace {
 
+int64_t gx_points(const Catalog* X) { return X->n; }
+
+// Largest tile box half-diagonal the Gram path's error bound is built for:
+// a 64-point tile spans about 64 / (points per cell) cells of its column
+// (points inside a cell are in no particular z order, so the end cells count
+// whole), plus one cell either way; tiles beyond it take the direct path.
+double gram_rho(const GridGeom& g, int64_t n) {
+  const double pc = (double)std::max<int64_t>(n, 1) / (double)g.ncell();
+  const double zsub = std::max(1.0, std::round((double)g.cs[0] / (double)g.cs[2]));
+  double cells = std::ceil(64.0 / std::max(pc, 1e-9)) + 2.0;
+  cells = std::min(8.0 * zsub, std::max(3.0, cells));
+  const double cz = cells * g.cs[2];
+  const double r2 = 0.25 * ((double)g.cs[0] * g.cs[0] + (double)g.cs[1] * g.cs[1] + cz * cz);
+  return std::sqrt(r2) * 1.02;
+}
+
 int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights) {
   const bool auto_ = job->hist == PCF_XX;
   Catalog* X = get_cat(c, job->cat_x);
@@ -434,13 +477,17 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   // The mixed kernel's recheck queue indexes j-side points with 27 bits.
   const bool plain = job->precision == PCF_PREC_FP64 || X->n >= kRecheckMaxPoints ||
                      Y->n >= kRecheckMaxPoints;
+  // Gram-form path (matrix cores): 2iso, unweighted, mixed precision.  Its
+  // cells are cut kGramZsub times thinner along z: the j range of a tile then
+  // ends within a thin cell of what the tile can reach.
+  const bool gram = !plain && !ani && !weighted && job->pair_kernel != PCF_PAIR_DIRECT;
 
   GridGeom g;
   int K[3];
   float reach;
   int m = 0;
   if (choose_geometry(X, auto_ ? nullptr : Y, job->dmax, bpc, L, job->cells_per_rmax, &g, K,
-                      &reach, &m) != 0)
+                      &reach, &m, gram ? kGramZsub : 1) != 0)
     return fail(c, PCF_ERR_UNSUPPORTED, "geometry");
   const bool rebuild = job->rebuild_grid != 0;
   int rc = PCF_OK;
@@ -460,10 +507,13 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   make_rules(job->type, bpc, auto_, bins, job->dmax, L, &rules);
   double A = bpc ? (double)std::max(X->maxabs, Y->maxabs) + (double)L : 0.0;
   const int nt_max = kMixedTableMax;
+  // The Gram path's error bound covers tiles whose bounding box has
+  // half-diagonal <= rho.
+  const double rho = gram ? gram_rho(g, gx_points(X)) : -1.0;
   static thread_local std::vector<uint8_t> table;
   float inv_h = 1.f;
   int nt = 2;
-  if (!plain && build_table(rules, A, nt_max, &table, &inv_h, &nt) != 0)
+  if (!plain && build_table(rules, A, nt_max, &table, &inv_h, &nt, rho) != 0)
     return fail(c, PCF_ERR_UNSUPPORTED, "bin table");
   const float sigma = std::sqrt(inv_h);  // exact power of two
   // Rules and table in one device buffer, uploaded from page-locked staging
@@ -564,6 +614,9 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   pl.le = rules.le;
   pl.zero = rules.zero;
   pl.nblocks = std::max(1, c->ncu * sh.per_cu);
+  pl.mfma = gram ? 1 : 0;
+  // The kernel's own test on the tile box, 2% inside the table's margin.
+  pl.rho2 = gram ? (float)(rho * rho / (1.02 * 1.02) * (double)inv_h) : 0.f;
   const bool launched = nitems_dev != nullptr || nitems > 0;
   if (launched) {
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
@@ -592,6 +645,7 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   st.evaluations = hs[0];
   st.rechecks = hs[1];
   st.work_items = nitems_dev ? hs[2] : (uint64_t)nitems;
+  st.direct_items = (int32_t)std::min<uint64_t>(hs[3], INT32_MAX);
   float ms = 0.f;
   if (launched) {
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));

@@ -23,11 +23,16 @@
 //   * counts go to a per-lane-column LDS histogram (no bank conflicts among the
 //     lanes of a wave), merged into the global uint64 / FP64 histogram once per
 //     workgroup of a persistent grid that pulls items from an atomic counter.
+#include <type_traits>
 #include <vector>
 
 #include "pcf_internal.h"
 #include "worklist.h"
 #include "exact.h"
+
+// Wave-wide min / max (ROCm device library, DPP-based).
+extern "C" __device__ __attribute__((const)) float __ockl_wfred_min_f32(float);
+extern "C" __device__ __attribute__((const)) float __ockl_wfred_max_f32(float);
 
 namespace pcf {
 
@@ -66,6 +71,8 @@ struct KArgs {
   int claim;    // work items a wave claims per atomic (0: derived from the count)
   const int* nitems_dev;  // item count in device memory (nullptr: nitems)
   int cap;                // items allocated (the count is clamped to it)
+  float rho2;             // Gram path: largest squared half-diagonal of a tile's
+                          // bounding box (scaled units) its error bound covers
 };
 
 // Item count of a launch (from device memory when the list was built without
@@ -158,8 +165,19 @@ constexpr int kMaxTable = kMixedTableMax;
 // items (LDS counter), and at the end of the chunk the workgroup flushes its
 // LDS histogram into that job's rows and clears it.  Point arrays and the zero
 // rule come from the chunk's job descriptor; xp..yp_orig are unused.
-template <int ANI, int WEIGHTED, int CS, int BATCH>
-__global__ __launch_bounds__(kBlockMixed, 4) void k_pairs_mixed(
+//
+// MF = 1 (2iso, unweighted): squared distances of an item come from the
+// matrix cores in Gram form, d2 = |xi|^2 + |xj|^2 - 2 xi.xj, 16 j x 16 i per
+// v_mfma_f32_16x16x4f32 (rows = j: A operand (x, y, z, |x|^2) of j;
+// columns = i: B operand (-2x, -2y, -2z, 1) of i; C = |x_i|^2 broadcast down
+// the column).  Coordinates are taken relative to the centre of the i-tile's
+// bounding box, so the FP32 error is bounded by the box size; that bound is
+// part of the bin table's ambiguity margin.  An item whose tile box exceeds
+// the launch's bound (k.rho2) takes the direct per-lane path instead.  The
+// instruction is exactly a chain of four fmaf in k order on gfx950 (measured
+// bit for bit), which is what the bound assumes, with a factor 2 of margin.
+template <int ANI, int WEIGHTED, int CS, int BATCH, int MF>
+__global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_pairs_mixed(
     const float4* __restrict__ xp, const float4* __restrict__ yp,
     const float4* __restrict__ xp_orig, const float4* __restrict__ yp_orig,
     const PairItem* __restrict__ items, int* __restrict__ counter,
@@ -187,6 +205,11 @@ __global__ __launch_bounds__(kBlockMixed, 4) void k_pairs_mixed(
   __syncthreads();
 
   const int lane = threadIdx.x & (kWave - 1);
+  // Gram path: lane group grp = lane / 16 carries component grp of its point
+  // in the matrix operands; col = lane % 16 is its point within a block of 16.
+  const int grp = lane >> 4, col = lane & 15;
+  const float sel_x = grp == 0 ? 1.0f : 0.0f, sel_y = grp == 1 ? 1.0f : 0.0f;
+  const float sel_z = grp == 2 ? 1.0f : 0.0f, sel_w = grp == 3 ? 1.0f : 0.0f;
   uint32_t* const hc_lane = hc + (lane & (ncopy - 1));
   double* const hw_lane = hw + (lane & (ncopy - 1));
   const unsigned ntm1 = (unsigned)(k.nt - 1);
@@ -195,6 +218,7 @@ __global__ __launch_bounds__(kBlockMixed, 4) void k_pairs_mixed(
   const unsigned bins = (unsigned)k.bins;
   unsigned long long evals = 0;
   unsigned long long nre = 0;
+  unsigned long long ndirect = 0;  // Gram path: items run by the direct path
   int qn = 0;  // entries in this wave's recheck queue (wave-uniform)
 
   // Exact FP64 bins of the newest `nd` queued pairs, one pair per lane (the
@@ -310,6 +334,141 @@ __global__ __launch_bounds__(kBlockMixed, 4) void k_pairs_mixed(
       deposit(r, q.w);
     }
   };
+
+  // Gram-form item on the matrix cores (MF = 1).  Returns false, having done
+  // nothing, when the tile's bounding box is larger than the error bound of
+  // the table covers; the caller then runs the direct path.
+  auto run_item_mf = [&](const int it, const float4* __restrict__ xp,
+                         const float4* __restrict__ yp, const float4* __restrict__ xp_orig,
+                         const float4* __restrict__ yp_orig, const int zero) -> bool {
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    const int i0 = __builtin_amdgcn_readfirstlane(items[it].i0);
+    const int ni = __builtin_amdgcn_readfirstlane(items[it].ni);
+    const int j0 = __builtin_amdgcn_readfirstlane(items[it].j0);
+    const int nj = __builtin_amdgcn_readfirstlane(items[it].nj);
+    const int code = __builtin_amdgcn_readfirstlane(items[it].code);
+    const int img = code & 31;
+    const bool masked = (code & kCodeMask) != 0;
+    // Image offset on the i side, the same FP32 operation as the direct path.
+    const float shx = (float)(img / 9 - 1) * k.L;
+    const float shy = (float)((img / 3) % 3 - 1) * k.L;
+    const float shz = (float)(img % 3 - 1) * k.L;
+    auto uni = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
+    const float4 pl = xp[i0 + min(lane, ni - 1)];
+    const float tx = pl.x - shx, ty = pl.y - shy, tz = pl.z - shz;
+    const float lox = uni(__ockl_wfred_min_f32(tx)), hix = uni(__ockl_wfred_max_f32(tx));
+    const float loy = uni(__ockl_wfred_min_f32(ty)), hiy = uni(__ockl_wfred_max_f32(ty));
+    const float loz = uni(__ockl_wfred_min_f32(tz)), hiz = uni(__ockl_wfred_max_f32(tz));
+    const float ex = hix - lox, ey = hiy - loy, ez = hiz - loz;
+    if (!(0.25f * (ex * ex + ey * ey + ez * ez) <= k.rho2)) return false;
+    const float ox = 0.5f * (lox + hix), oy = 0.5f * (loy + hiy), oz = 0.5f * (loz + hiz);
+    evals += (unsigned long long)ni * (unsigned long long)nj;
+
+    const int nb = (ni + 15) >> 4;
+    // i side: B operand (component grp of point i = 16b + col, times -2;
+    // grp 3: 1) and the C column |x_i|^2; lanes past the tile get a huge
+    // |x_i|^2 (always out).
+    float bop[4], cin[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int ir = 16 * b + col;
+      const float4 p = xp[i0 + min(ir, ni - 1)];
+      const float x = (p.x - shx) - ox, y = (p.y - shy) - oy, z = (p.z - shz) - oz;
+      // Exact select by 0/1 coefficients (products by 0 and 1 are exact).
+      bop[b] = __builtin_fmaf(sel_x, -2.0f * x,
+                              __builtin_fmaf(sel_y, -2.0f * y, __builtin_fmaf(sel_z, -2.0f * z, sel_w)));
+      cin[b] = ir < ni ? __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x)) : 1e30f;
+    }
+    const int jend = j0 + nj;
+    auto fix2 = [&](unsigned& r, int ii, int jj) {
+      const unsigned long long amb = __ballot(r == amb_row);
+      if (amb) {
+        const int na = __popcll(amb);
+        nre += na;
+        if (r == amb_row) {
+          const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi(
+                                   (unsigned)(amb >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)amb, 0u));
+          wq[pos] = make_int2(ii, jj | (img << kRecheckImgShift));
+          r = out_row;
+        }
+        qn += na;
+        if (qn >= kWave) drain(kWave, xp_orig, yp_orig, zero);
+      }
+    };
+    // The j loop for a tile of NB i-blocks (NB*16 >= ni), 16 j per step.
+    auto jloop = [&](auto nbc) {
+      constexpr int NB = decltype(nbc)::value;
+      // The next step's j point is loaded one step ahead (its latency hides
+      // behind this step's matrix, LDS and histogram work).
+      float4 qn4 = yp[min(j0 + col, jend - 1)];
+      for (int jb = j0; jb < jend; jb += 16) {
+        // j side: A operand (component grp of point j = jb + col; grp 3:
+        // |x_j|^2, huge past the run's end).
+        const int jj = jb + col;
+        const float4 q = qn4;
+        qn4 = yp[min(jj + 16, jend - 1)];
+        const float x = q.x - ox, y = q.y - oy, z = q.z - oz;
+        const float bj = jj < jend ? __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x)) : 1e30f;
+        const float aop =
+            __builtin_fmaf(sel_x, x, __builtin_fmaf(sel_y, y, __builtin_fmaf(sel_z, z, sel_w * bj)));
+        // Two blocks at a time (fewer live registers, so more waves fit).
+#pragma unroll
+        for (int h = 0; h < NB; h += 2) {
+          constexpr int H = 2;
+          const int nh = (NB - h) < H ? (NB - h) : H;
+          f4v d[H];
+#pragma unroll
+          for (int u = 0; u < H; ++u) {
+            if (u < nh) {
+              const int b = h + u;
+              const f4v cv = {cin[b], cin[b], cin[b], cin[b]};
+              d[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop, bop[b], cv, 0, 0, 0);
+            }
+          }
+          unsigned r[4 * H];
+#pragma unroll
+          for (int u = 0; u < H; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+              r[4 * u + v] = u < nh ? tab[min(__float2uint_rz(d[u][v]), ntm1)] : 0u;
+          // Element (b, v) of this lane: i = i0 + 16b + col, j = jb + 4*grp + v.
+          if (masked) {
+#pragma unroll
+            for (int u = 0; u < H; ++u)
+#pragma unroll
+              for (int v = 0; v < 4; ++v)
+                r[4 * u + v] =
+                    (jb + 4 * grp + v <= i0 + 16 * (h + u) + col) ? out_row : r[4 * u + v];
+          }
+          unsigned m = r[0];
+#pragma unroll
+          for (int e = 1; e < 4 * H; ++e) m = max(m, r[e]);
+          if (__builtin_expect(__ballot(m == amb_row) != 0, 0)) {
+#pragma unroll
+            for (int u = 0; u < H; ++u)
+              if (u < nh) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                  fix2(r[4 * u + v], i0 + 16 * (h + u) + col, jb + 4 * grp + v);
+              }
+          }
+#pragma unroll
+          for (int u = 0; u < H; ++u)
+            if (u < nh) {
+#pragma unroll
+              for (int v = 0; v < 4; ++v) atomicAdd(hc_lane + (r[4 * u + v] << CS), 1u);
+            }
+        }
+      }
+    };
+    switch (nb) {
+      case 4: jloop(std::integral_constant<int, 4>{}); break;
+      case 3: jloop(std::integral_constant<int, 3>{}); break;
+      case 2: jloop(std::integral_constant<int, 2>{}); break;
+      default: jloop(std::integral_constant<int, 1>{}); break;
+    }
+    return true;
+  };
   // Sum of the LDS copies of each row into rows [hoff, hoff + nflat) of the
   // global histogram; `clear` zeroes the LDS rows for the next chunk.
   auto flush = [&](int hoff, bool clear) {
@@ -353,7 +512,13 @@ __global__ __launch_bounds__(kBlockMixed, 4) void k_pairs_mixed(
         base = rb + __builtin_amdgcn_readfirstlane(base);
         if (base >= re) break;
         const int end = min(base + claim, re);
-        for (int it = base; it < end; ++it) run_item(it, xp, yp, xp_orig, yp_orig, k.zero);
+        for (int it = base; it < end; ++it) {
+          if constexpr (MF) {
+            if (run_item_mf(it, xp, yp, xp_orig, yp_orig, k.zero)) continue;
+            ++ndirect;
+          }
+          run_item(it, xp, yp, xp_orig, yp_orig, k.zero);
+        }
       }
     }
     if (qn > 0) drain(qn, xp_orig, yp_orig, k.zero);
@@ -386,6 +551,7 @@ __global__ __launch_bounds__(kBlockMixed, 4) void k_pairs_mixed(
   }
   if (lane == 0) atomicAdd(&stats[0], evals);
   if (lane == 0 && nre) atomicAdd(&stats[1], nre);
+  if (MF && lane == 0 && ndirect) atomicAdd(&stats[3], ndirect);
 }
 
 // Scaled copy of a point array: x, y, z multiplied by sigma (a power of two,
@@ -655,6 +821,7 @@ hipError_t launch_pairs(const PairLaunch& pl, hipStream_t s) {
   k.claim = 0;
   k.nitems_dev = pl.batch ? nullptr : pl.nitems_dev;
   k.cap = pl.nitems_cap;
+  k.rho2 = pl.rho2;
   const int ncopy = 1 << pl.cshift;
   dim3 grid(pl.nblocks), block(kBlock);
   hipError_t e;
@@ -665,18 +832,19 @@ hipError_t launch_pairs(const PairLaunch& pl, hipStream_t s) {
     const size_t hw_bytes = pl.weighted ? (size_t)(pl.nflat + 2) * ncopy * 8 : 0;
     const size_t lds = hc_bytes + t_bytes + hw_bytes + kRecheckQueueBytes;
     const dim3 mblock(kBlockMixed);
-#define PCF_LAUNCH_MIXED1(A, W, C, B)                                                          \
-  do {                                                                                         \
-    if ((e = set_lds(k_pairs_mixed<A, W, C, B>, lds)) != hipSuccess) return e;                 \
-    if (pl.prepare_only) return hipSuccess;                                                    \
-    hipLaunchKernelGGL((k_pairs_mixed<A, W, C, B>), grid, mblock, lds, s, pl.xs, pl.ys, pl.xp, \
-                       pl.yp, pl.items, pl.counter, pl.table, pl.rules, pl.hist, pl.whist,     \
-                       pl.stats, k, pl.jobs, pl.chunks);                                       \
+#define PCF_LAUNCH_MIXED2(A, W, C, B, M)                                                          \
+  do {                                                                                            \
+    if ((e = set_lds(k_pairs_mixed<A, W, C, B, M>, lds)) != hipSuccess) return e;                 \
+    if (pl.prepare_only) return hipSuccess;                                                       \
+    hipLaunchKernelGGL((k_pairs_mixed<A, W, C, B, M>), grid, mblock, lds, s, pl.xs, pl.ys, pl.xp, \
+                       pl.yp, pl.items, pl.counter, pl.table, pl.rules, pl.hist, pl.whist,        \
+                       pl.stats, k, pl.jobs, pl.chunks);                                          \
   } while (0)
 #define PCF_LAUNCH_MIXED(A, W, C)                                  \
   do {                                                             \
-    if (pl.batch) PCF_LAUNCH_MIXED1(A, W, C, 1);                   \
-    else PCF_LAUNCH_MIXED1(A, W, C, 0);                            \
+    if (pl.batch) PCF_LAUNCH_MIXED2(A, W, C, 1, 0);                \
+    else if (A == 0 && W == 0 && pl.mfma) PCF_LAUNCH_MIXED2(0, 0, C, 0, 1); \
+    else PCF_LAUNCH_MIXED2(A, W, C, 0, 0);                         \
   } while (0)
 #define PCF_LAUNCH_CS(A, W)                                   \
   switch (pl.cshift) {                                        \
@@ -694,7 +862,7 @@ hipError_t launch_pairs(const PairLaunch& pl, hipStream_t s) {
     else { PCF_LAUNCH_CS(1, 1) }
 #undef PCF_LAUNCH_CS
 #undef PCF_LAUNCH_MIXED
-#undef PCF_LAUNCH_MIXED1
+#undef PCF_LAUNCH_MIXED2
   } else {
     const size_t hc_bytes = (size_t)(((pl.nflat + 1) * ncopy * 4 + 15) & ~15);
     const size_t hw_bytes = pl.weighted ? (size_t)(pl.nflat + 1) * ncopy * 8 : 0;

@@ -109,10 +109,16 @@ hipError_t ensure_grid64(pcf_ctx* c, DevGrid* g);
 // Defined in abi.hip.
 double ds_floored(int bins, float dmax);
 void bin_thresholds(int bins, double ds, double* T);
+// rho >= 0: the table also covers the Gram-form pair path for tiles of box
+// half-diagonal <= rho (unscaled units).
 int build_table(const BinRules& r, double A, int nt_max, std::vector<uint8_t>* table, float* inv_h,
-                int* nt_out);
+                int* nt_out, double rho = -1.0);
+// zsub > 1 cuts every cell into zsub thinner cells along z (the line of the
+// grid's columns), so the j range of a tile (whole z cells) overshoots what
+// the tile can reach by at most a thin cell at either end.
 int choose_geometry(const Catalog* X, const Catalog* Y, float dmax, int bpc, float box,
-                    int cells_per_rmax, GridGeom* g, int K[3], float* reach_out, int* m_out);
+                    int cells_per_rmax, GridGeom* g, int K[3], float* reach_out, int* m_out,
+                    int zsub = 1);
 DevGrid* get_grid(pcf_ctx* c, Catalog* cat, const GridGeom& g, bool force_rebuild, int* rc);
 void free_catalog(Catalog* cat);
 

@@ -189,6 +189,8 @@ struct PairLaunch {
   int zero;
   // batched launch (mixed path only): chunks of the concatenated work list
   int batch = 0;
+  int mfma = 0;       // 2iso unweighted: Gram-form distances on the matrix cores
+  float rho2 = 0.f;   // ... for tiles whose box half-diagonal^2 (scaled) is <= rho2
   int prepare_only = 0;  // only set the kernel's LDS attribute (before graph capture)
   const BatchJob* jobs = nullptr;
   const BatchChunk* chunks = nullptr;
@@ -204,6 +206,7 @@ constexpr int kRecheckIndexMask = (1 << kRecheckImgShift) - 1;
 constexpr int64_t kRecheckMaxPoints = (int64_t)1 << kRecheckImgShift;
 constexpr size_t kRecheckQueueBytes = (size_t)(kMixedBlock / 64) * kRecheckCap * 8;
 constexpr int kMixedTableMax = 28672;  // static LDS bytes of its d2 -> bin table
+constexpr int kGramZsub = 4;  // Gram pair path: thin z cells per lateral cell edge
 
 // Launch shape of the pair kernels: 2^cshift LDS copies of each histogram row
 // (lanes spread their atomics over the copies) and the resident workgroups per

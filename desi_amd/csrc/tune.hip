@@ -1,7 +1,9 @@
 // tune.hip -- autotuner of the pair kernels (pcf_autotune).
 //
-// The knobs are the cell edge (rmax / m: candidates per useful pair against
-// work items per point), the number of LDS copies of each histogram row
+// The knobs are the pair kernel (single unweighted 2iso jobs: the direct
+// per-lane form or the Gram form on the matrix cores), the cell edge (rmax / m:
+// candidates per useful pair against work items per point), the number of LDS
+// copies of each histogram row
 // (fewer bank conflicts on the deposit against LDS per workgroup) and the
 // resident workgroups per CU.  The workgroup size stays 512 (8 waves share one
 // d2 -> bin table in LDS).  Tuning runs the real histogram(s) on the caller's
@@ -42,7 +44,9 @@ std::string tune_key(pcf_ctx* c, const pcf_job* jobs, int njobs) {
 // Apply a cached tuning to jobs whose knobs are all at their defaults.
 void apply_tuning(pcf_ctx* c, pcf_job* jobs, int njobs) {
   for (int i = 0; i < njobs; ++i)
-    if (jobs[i].cells_per_rmax || jobs[i].lds_copies_log2 || jobs[i].blocks_per_cu) return;
+    if (jobs[i].cells_per_rmax || jobs[i].lds_copies_log2 || jobs[i].blocks_per_cu ||
+        jobs[i].pair_kernel)
+      return;
   const std::string key = tune_key(c, jobs, njobs);
   std::lock_guard<std::mutex> lk(c->store->mu);
   auto it = c->store->tuned.find(key);
@@ -51,13 +55,14 @@ void apply_tuning(pcf_ctx* c, pcf_job* jobs, int njobs) {
     jobs[i].cells_per_rmax = it->second.cells_per_rmax;
     jobs[i].lds_copies_log2 = it->second.lds_copies_log2;
     jobs[i].blocks_per_cu = it->second.blocks_per_cu;
+    jobs[i].pair_kernel = it->second.pair_kernel;
   }
 }
 
 namespace {
 
 struct Trial {
-  int m, lds, per_cu;
+  int m, lds, per_cu, kernel;
 };
 
 // Runs the jobs once with the given knobs; returns PCF_OK and the device time
@@ -70,6 +75,7 @@ int run_trial(pcf_ctx* c, const std::vector<pcf_job>& base, const Trial& t,
     j.cells_per_rmax = t.m;
     j.lds_copies_log2 = t.lds;
     j.blocks_per_cu = t.per_cu;
+    j.pair_kernel = t.kernel;
   }
   const int64_t hs = pcf_hist_size(jobs[0].type, jobs[0].bins);
   counts->assign((size_t)hs * jobs.size(), 0);
@@ -107,6 +113,7 @@ extern "C" int pcf_autotune(pcf_ctx* c, const pcf_job* jobs, int32_t njobs, pcf_
     j.cells_per_rmax = 0;
     j.lds_copies_log2 = 0;
     j.blocks_per_cu = 0;
+    j.pair_kernel = PCF_PAIR_AUTO;
     j.rebuild_grid = 0;
   }
   const std::string key = tune_key(c, base.data(), njobs);
@@ -119,7 +126,7 @@ extern "C" int pcf_autotune(pcf_ctx* c, const pcf_job* jobs, int32_t njobs, pcf_
   // also builds the grids).
   std::vector<uint64_t> ref, got;
   double t_def = 0, t = 0;
-  const Trial def{0, 0, 0};
+  const Trial def{0, 0, 0, PCF_PAIR_AUTO};
   int rc = run_trial(c, base, def, &ref, &t);
   if (rc != PCF_OK) return rc;
   if ((rc = run_trial(c, base, def, &ref, &t_def)) != PCF_OK) return rc;
@@ -140,18 +147,34 @@ extern "C" int pcf_autotune(pcf_ctx* c, const pcf_job* jobs, int32_t njobs, pcf_
     return PCF_OK;
   };
 
-  Trial best{m_def, 0, 0};
+  // 0. pair kernel: single unweighted 2iso jobs run the Gram form by default;
+  // the direct form competes for them.
+  const bool gram_ok = njobs == 1 && type == PCF_2ISO && !weighted;
+  Trial best{m_def, 0, 0, gram_ok ? PCF_PAIR_GRAM : PCF_PAIR_AUTO};
   double t_best = t_def;
-  // 1. cell edge
-  for (int m = 1; m <= 6; ++m) {
-    if (m == m_def) continue;
+  if (gram_ok) {
     double tm;
-    rc = timed(Trial{m, 0, 0}, &tm);
-    if (rc == PCF_ERR_UNSUPPORTED) continue;
-    if (rc != PCF_OK) return rc;
+    const Trial tr{m_def, 0, 0, PCF_PAIR_DIRECT};
+    if ((rc = timed(tr, &tm)) != PCF_OK) return rc;
     if (tm < t_best) {
       t_best = tm;
-      best = Trial{m, 0, 0};
+      best = tr;
+    }
+  }
+  // 1. cell edge (for each kernel candidate)
+  const int nk = gram_ok ? 2 : 1;
+  for (int kk = 0; kk < nk; ++kk) {
+    const int kern = gram_ok ? (kk == 0 ? PCF_PAIR_GRAM : PCF_PAIR_DIRECT) : PCF_PAIR_AUTO;
+    for (int m = 1; m <= 6; ++m) {
+      if (m == m_def) continue;
+      double tm;
+      rc = timed(Trial{m, 0, 0, kern}, &tm);
+      if (rc == PCF_ERR_UNSUPPORTED) continue;
+      if (rc != PCF_OK) return rc;
+      if (tm < t_best) {
+        t_best = tm;
+        best = Trial{m, 0, 0, kern};
+      }
     }
   }
   // 2. LDS copies of the histogram rows (log2 + 1 in the job)
@@ -161,7 +184,7 @@ extern "C" int pcf_autotune(pcf_ctx* c, const pcf_job* jobs, int32_t njobs, pcf_
     for (int cs = std::max(0, cs_def - 3); cs <= std::min(6, cs_def + 1); ++cs) {
       if (cs == cs_def) continue;
       double tm;
-      Trial tr{best.m, cs + 1, 0};
+      Trial tr{best.m, cs + 1, 0, best.kernel};
       if ((rc = timed(tr, &tm)) != PCF_OK) return rc;
       if (tm < t_best) {
         t_best = tm;
@@ -177,7 +200,7 @@ extern "C" int pcf_autotune(pcf_ctx* c, const pcf_job* jobs, int32_t njobs, pcf_
     const int pc_def = pair_shape(nflat, jobs[0].bins, weighted, false, cs + 1, 0).per_cu;
     for (int pc = 1; pc < pc_def; ++pc) {
       double tm;
-      Trial tr{best.m, best.lds, pc};
+      Trial tr{best.m, best.lds, pc, best.kernel};
       if ((rc = timed(tr, &tm)) != PCF_OK) return rc;
       if (tm < t_best) {
         t_best = tm;
@@ -189,6 +212,7 @@ extern "C" int pcf_autotune(pcf_ctx* c, const pcf_job* jobs, int32_t njobs, pcf_
   res.cells_per_rmax = best.m;
   res.lds_copies_log2 = best.lds;
   res.blocks_per_cu = best.per_cu;
+  res.pair_kernel = best.kernel;
   res.trials = trials;
   res.best_ms = t_best;
   res.default_ms = t_def;

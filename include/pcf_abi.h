@@ -90,8 +90,20 @@ typedef struct pcf_job {
   int32_t rebuild_grid;   /* 1: re-grid the catalogs even if resident      */
   int32_t lds_copies_log2; /* LDS histogram copies per row, log2 + 1; 0 = auto */
   int32_t blocks_per_cu;  /* resident workgroups per CU; 0 = auto          */
-  int32_t reserved[2];
+  int32_t pair_kernel;    /* mixed-precision pair path: PCF_PAIR_AUTO,      */
+                          /* PCF_PAIR_DIRECT or PCF_PAIR_GRAM               */
+  int32_t reserved;
 } pcf_job;
+
+/* ---- pair kernel variants (pcf_job.pair_kernel) ---------------------- *
+ * Both give the same counts; they differ in where d2 is formed.  DIRECT:
+ * per lane, (xi - xj)^2 summed in FP32 on the vector units.  GRAM: 16 x 16
+ * pairs at a time on the matrix cores as |xi|^2 + |xj|^2 - 2 xi.xj in
+ * tile-local coordinates (2iso, unweighted; other jobs run DIRECT).  AUTO is
+ * GRAM where it applies. */
+#define PCF_PAIR_AUTO 0
+#define PCF_PAIR_DIRECT 1
+#define PCF_PAIR_GRAM 2
 
 /* Per-call statistics of the last completed histogram. */
 typedef struct pcf_stats {
@@ -105,7 +117,8 @@ typedef struct pcf_stats {
   int32_t chunks;        /* pcf_histogram_batch: work chunks of the launch  */
   int32_t pairs_listed;  /* triplets: entries of the (1,2) pair list        */
   int32_t batched_jobs;  /* pcf_wait: jobs that shared this job's launch    */
-  int32_t reserved;
+  int32_t direct_items;  /* Gram pair path: work items whose tile was too  */
+                         /* large for its error bound (run on the direct path) */
   double queue_ms;       /* pcf_wait: submit -> start (host clock)          */
   double latency_ms;     /* pcf_wait: submit -> done (host clock)           */
 } pcf_stats;
@@ -159,8 +172,9 @@ int pcf_histogram_batch(pcf_ctx* ctx, const pcf_job* jobs, int32_t njobs, uint64
  * launch shape is fixed: 8x8 threads per block, one thread per node pair,
  * cuda/pcf2iso.cuh:74-80, and whose -p partitions flag sets the node size,
  * cuda/main.cu:86).  Runs the given job, or batch of pair jobs (njobs > 1, as
- * for pcf_histogram_batch), on its own catalogs under candidate cell sizes,
- * LDS histogram layouts and occupancies, verifies that every candidate gives
+ * for pcf_histogram_batch), on its own catalogs under candidate pair kernels
+ * (single 2iso jobs: direct or Gram), cell sizes, LDS histogram layouts and
+ * occupancies, verifies that every candidate gives
  * the default configuration's counts, and caches the fastest on the context:
  * later pcf_histogram / pcf_histogram_batch calls on the same workload (same
  * type, bins, dmax, box, precision and catalog sizes) whose knobs are 0 use it.
@@ -172,6 +186,9 @@ typedef struct pcf_tuning {
   int32_t trials;          /* histogram runs spent                           */
   double best_ms;          /* counting-kernel time of the choice             */
   double default_ms;       /* same, default configuration                    */
+  int32_t pair_kernel;     /* PCF_PAIR_DIRECT or PCF_PAIR_GRAM (2iso,        */
+                           /* unweighted); PCF_PAIR_AUTO otherwise           */
+  int32_t reserved;
 } pcf_tuning;
 int pcf_autotune(pcf_ctx* ctx, const pcf_job* jobs, int32_t njobs, pcf_tuning* out);
 

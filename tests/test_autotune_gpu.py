@@ -41,6 +41,29 @@ def test_cross_config_equality(engine, ani, bpc):
         engine.release(cr)
 
 
+@pytest.mark.parametrize("bpc", [False, True])
+def test_cross_config_equality_pair_kernels(engine, bpc):
+    # Unweighted 2iso: the Gram path (matrix cores, thin z cells) and the
+    # direct path under every cell size and LDS layout give the same counts.
+    d = synth.neyman_scott(60, 40, 6.0, 300.0, seed=103)
+    r = synth.uniform(3000, 300.0, seed=104)
+    cd, cr = engine.upload(d), engine.upload(r)
+    print(f"case pair kernels bpc={bpc}", flush=True)
+    try:
+        for hist, y in ((da.PCF_XX, -1), (da.PCF_XY, cr)):
+            co, _ = O.pairs(d, None if y < 0 else r, bpc=bpc, box=300.0, bins=25, dmax=45.0)
+            for kern, m, lds in itertools.product((da.PCF_PAIR_GRAM, da.PCF_PAIR_DIRECT),
+                                                  (1, 2, 3, 5), (0, 2, 7)):
+                job = da.make_job(da.PCF_2ISO, hist, bins=25, dmax=45.0, cat_x=cd, cat_y=y,
+                                  bpc=bpc, box=300.0, cells_per_rmax=m, lds_copies_log2=lds,
+                                  pair_kernel=kern)
+                c, _ = engine.histogram(job)
+                assert np.array_equal(c, co), (kern, m, lds)
+    finally:
+        engine.release(cd)
+        engine.release(cr)
+
+
 def test_autotune_single_job_is_cached_and_used(engine):
     d = synth.uniform(60000, 500.0, seed=111)
     cd = engine.upload(d)
@@ -50,6 +73,7 @@ def test_autotune_single_job_is_cached_and_used(engine):
         t = engine.autotune(job)
         print("tuning", t, flush=True)
         assert 1 <= t["cells_per_rmax"] <= 6 and t["trials"] >= 6
+        assert t["pair_kernel"] in (da.PCF_PAIR_DIRECT, da.PCF_PAIR_GRAM)
         assert t["best_ms"] <= t["default_ms"] * 1.0001
         c, _ = engine.histogram(job)  # knobs 0: the cached choice applies
         assert engine.stats()["cells_per_rmax"] == t["cells_per_rmax"]
