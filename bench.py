@@ -9,7 +9,9 @@ DD, RR and DR histograms in the FP32 + FP64-edge-recheck form.  Inputs are
 resident in HBM before the timed region (catalog upload is not timed).
 
 value = useful pairs per second (sum of the three unweighted histograms divided
-by the wall time of the step), the reference's "pairs/s".
+by the wall time of the step), the reference's "pairs/s".  The pair kernel is
+the autotuned choice: for this unweighted 2iso workload, normally the Gram form
+(d2 from the matrix cores) over the direct form (d2 on the vector units).
 Also reported: the same step on the plain FP64 path, the roofline of the pair
 kernel (8 FP32 flops per candidate-pair evaluation against the 157.3 TFLOP/s
 FP32 vector peak), and the CPU oracle timed on a bounded sample.
@@ -28,8 +30,8 @@ import numpy as np  # noqa: E402
 import desi_amd as da  # noqa: E402
 from desi_amd import synth  # noqa: E402
 
-PEAK_FP32 = 157.3e12  # MI355X FP32 vector (packed) peak, FLOP/s
-FLOPS_PER_PAIR = 8    # 3 sub + 3 mul + 2 add (FMA = 2 flops)
+PEAK_FP32 = 157.3e12  # MI355X FP32 peak, FLOP/s (packed vector and dense matrix alike)
+FLOPS_PER_PAIR = 8    # 3 sub + 3 mul + 2 add, or 4 chained FMAs in Gram form (FMA = 2 flops)
 FLOPS_PER_TRIPLET = 16  # two squared distances per third-point evaluation
 N = 1 << 20
 BOX = 1000.0
@@ -37,7 +39,10 @@ RMAX = 80.0
 BINS = 30
 
 
-PAIR_PROFILE = os.path.join(ROOT, "profiles", "k_pairs_mixed_2iso_N1M_box1000_rmax80.md")
+# rocprofv3 summaries of the NS pair kernel: the Gram form (matrix cores) and
+# the direct form (vector units); the tuned choice's summary gives `traffic`.
+PAIR_PROFILE = {2: os.path.join(ROOT, "profiles", "k_pairs_gram_2iso_N1M_box1000_rmax80.md"),
+                1: os.path.join(ROOT, "profiles", "k_pairs_mixed_2iso_N1M_box1000_rmax80.md")}
 
 
 def profiled_traffic(path, kernel):
@@ -427,6 +432,15 @@ def main():
 
     serving = None if args.no_serving else serving_leg(args.serving_requests)
     achieved = evals * FLOPS_PER_PAIR / (kms * 1e-3)
+    # The dominant kernel: the Gram form forms d2 with four chained FMAs on the
+    # matrix cores (FP32 MFMA peak 157.3 TFLOP/s, the same figure as the packed
+    # FP32 vector peak); the direct form runs on the vector units.
+    kern = None
+    if tuning is not None:
+        ks = {t.get("pair_kernel") for t in tuning.values()}
+        kern = ks.pop() if len(ks) == 1 else None
+    bound = "mfma" if kern == da.PCF_PAIR_GRAM else "valu"
+    prof = PAIR_PROFILE.get(kern if kern in PAIR_PROFILE else da.PCF_PAIR_GRAM)
     out = {
         "metric": "pairs/s (2iso DD+RR+DR, useful pairs)",
         "value": value,
@@ -443,9 +457,9 @@ def main():
         "config": {"workload": "2iso N=1048576 data + 1048576 random, box 1000, rmax 80, "
                                "30 bins, DD+RR+DR, no BPC",
                    "n_data": N, "n_random": N, "box": BOX, "rmax": RMAX, "bins": BINS},
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+        "roofline": {"bound": bound, "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
-                     "traffic": profiled_traffic(PAIR_PROFILE, "k_pairs_mixed NS")},
+                     "traffic": profiled_traffic(prof, "k_pairs_mixed NS")},
         "pair_kernel_ms_per_step": kms / args.steps,
         "evaluations_per_step": evals // args.steps,
         "rechecks_per_step": rech // args.steps,
