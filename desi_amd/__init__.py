@@ -22,6 +22,7 @@ PCF_XX, PCF_XY, PCF_XXX, PCF_XXY = 0, 1, 2, 3
 # Arithmetic.
 PCF_PREC_MIXED, PCF_PREC_FP64 = 0, 1
 PCF_PAIR_AUTO, PCF_PAIR_DIRECT, PCF_PAIR_GRAM = 0, 1, 2
+PCF_EST_NORMALIZE = 1
 
 ERRORS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "out of device memory",
           -4: "coordinates out of range", -5: "unsupported", -6: "busy"}
@@ -31,7 +32,7 @@ EXPORTED_SYMBOLS = (
     "pcf_catalog_upload", "pcf_catalog_upload_device", "pcf_catalog_release",
     "pcf_catalog_size", "pcf_histogram", "pcf_last_stats", "pcf_analytic_rr",
     "pcf_analytic_3iso", "pcf_histogram_batch", "pcf_autotune", "pcf_set_max_batch",
-    "pcf_set_hbm_budget", "pcf_submit", "pcf_poll", "pcf_wait",
+    "pcf_set_hbm_budget", "pcf_submit", "pcf_poll", "pcf_wait", "pcf_estimator",
 )
 
 
@@ -107,6 +108,8 @@ def load_library(path=LIB_PATH):
     L.pcf_autotune.argtypes = [vp, ctypes.POINTER(PcfJob), c_i32, ctypes.POINTER(PcfTuning)]
     L.pcf_analytic_3iso.argtypes = [vp, c_i32, c_i32, ctypes.c_float, ctypes.c_float, c_i64,
                                     c_i32, c_i32, vp, vp, vp, vp]
+    L.pcf_estimator.argtypes = [vp, c_i32, c_i64, vp, vp, vp, vp, ctypes.c_double,
+                                ctypes.c_double, c_i32, vp]
     L.pcf_set_hbm_budget.argtypes = [vp, c_i64, c_i32]
     L.pcf_set_max_batch.argtypes = [vp, c_i32]
     L.pcf_submit.argtypes = [vp, ctypes.POINTER(PcfJob), ctypes.POINTER(c_i64)]
@@ -203,6 +206,22 @@ class Engine:
         rr = np.zeros(bins, np.float64)
         self._check(self.lib.pcf_analytic_rr(self.ctx, bins, np_, dmax, box, rr.ctypes.data))
         return rr
+
+    def estimator(self, hists, n_data=0.0, n_rand=0.0, normalize=False):
+        """Landy-Szalay estimator on the device (pcf_estimator): hists is
+        (DD, DR, RR) or (DDD, DDR, DRR, RRR), equal-length float64 arrays."""
+        hs = [np.ascontiguousarray(h, dtype=np.float64).ravel() for h in hists]
+        if len(hs) not in (3, 4) or any(len(h) != len(hs[0]) for h in hs):
+            raise ValueError("estimator takes 3 (2PCF) or 4 (3PCF) equal-length histograms")
+        order = 2 if len(hs) == 3 else 3
+        xi = np.zeros(len(hs[0]), np.float64)
+        h3 = hs[3].ctypes.data if order == 3 else None
+        self._check(self.lib.pcf_estimator(self.ctx, order, len(xi), hs[0].ctypes.data,
+                                           hs[1].ctypes.data, hs[2].ctypes.data, h3,
+                                           float(n_data), float(n_rand),
+                                           PCF_EST_NORMALIZE if normalize else 0,
+                                           xi.ctypes.data))
+        return xi
 
     def histogram_batch(self, jobs):
         """Several pair histograms in one launch (pcf_histogram_batch): a list

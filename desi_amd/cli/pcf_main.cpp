@@ -21,6 +21,13 @@
 //     full bins^5 histogram with the reference's save_histogram5D format.
 // Environment: PCF_DATA_DIR / PCF_RESULTS_DIR override ../data/ and
 // ../results/; PCF_PRECISION=fp64 selects the plain FP64 path.
+// Extension: -e also writes the Landy-Szalay estimator of each random file,
+// formed on the device by pcf_estimator as the reference's analysis scripts
+// form it (cpp/2PCFisotropic/normal/graph_2DPF.py:8-14,
+// cpp/3PCFisotropic/normal/graph_3PCF.py:5-6): LS<tag><random> for 2PCF,
+// LS3<tag><random> for 3PCF (analytic modes: ...BPCanalytic_<data>), in the
+// histogram's own file format; PCF_EST_NORMALIZE=1 divides the histograms by
+// their pair (triplet) counts first.
 #include <dirent.h>
 
 #include <algorithm>
@@ -69,13 +76,16 @@ void show_help() {
           "  [-a]:                (optional) analytic random histograms (isotropic types).\n"
           "-p box_partitions:     (optional) grid partitions per dimension; accepted for\n"
           "                       compatibility, the engine sizes its own grid.\n"
-          "-s size_box:           (optional) side of the box holding the points.\n\n";
+          "-s size_box:           (optional) side of the box holding the points.\n"
+          "-e:                    (optional) also write the Landy-Szalay estimator per random\n"
+          "                       file (LS... for 2PCF, LS3... for 3PCF).\n\n";
 }
 
 struct Opts {
   string type;
   int np = 0, bn = 0, partitions = 35;
   bool bpc = false, analytic = false, rand_dir = false, size_box_provided = false;
+  bool estimator = false;
   float size_box = 0, dmax = 0;
   string data_name, rand_name;
 };
@@ -124,6 +134,20 @@ struct Run {
     out->assign((size_t)n, 0.0);
     pcf_job j = job(type, h, x, y);
     return pcf_histogram(ctx, &j, nullptr, out->data());
+  }
+
+  // Landy-Szalay estimator of finished histograms (2PCF: DD, DR, RR; 3PCF:
+  // DDD, DDR, DRR, RRR), written like a histogram of this type.
+  int estimate(int type, const std::vector<const std::vector<double>*>& hs, const string& name) {
+    const int order = (type == PCF_2ISO || type == PCF_2ANI) ? 2 : 3;
+    const size_t n = hs[0]->size();
+    std::vector<double> xi(n);
+    const int flags = env_or("PCF_EST_NORMALIZE", "0") == "1" ? PCF_EST_NORMALIZE : 0;
+    int r = pcf_estimator(ctx, order, (int64_t)n, hs[0]->data(), hs[1]->data(), hs[2]->data(),
+                          order == 3 ? hs[3]->data() : nullptr, (double)o.np, (double)o.np, flags,
+                          xi.data());
+    if (r != PCF_OK) return r;
+    return save(type, name, xi) ? PCF_OK : PCF_ERR_ARG;
   }
 
   bool save(int type, const string& name, const std::vector<double>& h) {
@@ -175,6 +199,7 @@ int main(int argc, char** argv) {
       else if (a == "-d") o.dmax = std::stof(next());
       else if (a == "-bpc") o.bpc = true;
       else if (a == "-a") o.analytic = true;
+      else if (a == "-e") o.estimator = true;
       else if (a == "-p") o.partitions = (int)std::stof(next());
       else if (a == "-s") {
         o.size_box = std::stof(next());
@@ -278,18 +303,21 @@ int main(int argc, char** argv) {
   const string& dn = histo_names[0];
   std::vector<double> h;
   auto t0 = std::chrono::steady_clock::now();
-  auto H = [&](int type, int hist, pcf_catalog x, pcf_catalog y, const string& name) -> int {
+  auto H = [&](int type, int hist, pcf_catalog x, pcf_catalog y, const string& name,
+               std::vector<double>* keep = nullptr) -> int {
     int r = run.hist(type, hist, x, y, &h);
     if (r != PCF_OK) return r;
+    if (keep) *keep = h;
     return run.save(type, name, h) ? PCF_OK : PCF_ERR_ARG;
   };
+  std::vector<double> h0, h1, h2, h3;  // kept for the estimator (-e)
 
   if (o.type == "2iso" || o.type == "2ani") {
     const int type = (o.type == "2iso") ? PCF_2ISO : PCF_2ANI;
     const string tag = (type == PCF_2ISO ? "iso_" : "ani_") + string(o.bpc ? "BPC_" : "");
     if (type == PCF_2ISO && o.bpc && o.analytic) {
       // cuda/pcf2isoBPC.cuh:166-289: DD with periodic images, RR analytic, DR = RR.
-      if ((rc = H(type, PCF_XX, cd, -1, "DDiso_BPCanalytic_" + dn)) != PCF_OK)
+      if ((rc = H(type, PCF_XX, cd, -1, "DDiso_BPCanalytic_" + dn, &h0)) != PCF_OK)
         return die(run.ctx, rc, "DD");
       std::vector<double> rr(o.bn);
       if ((rc = pcf_analytic_rr(run.ctx, o.bn, o.np, o.dmax, run.o.size_box, rr.data())) !=
@@ -297,13 +325,20 @@ int main(int argc, char** argv) {
         return die(run.ctx, rc, "analytic RR");
       run.save(type, "RRiso_BPCanalytic_" + dn, rr);
       run.save(type, "DRiso_BPCanalytic_" + dn, rr);
+      if (o.estimator && (rc = run.estimate(type, {&h0, &rr, &rr}, "LSiso_BPCanalytic_" + dn)) !=
+                             PCF_OK)
+        return die(run.ctx, rc, "estimator");
     } else {
-      if ((rc = H(type, PCF_XX, cd, -1, "DD" + tag + dn)) != PCF_OK) return die(run.ctx, rc, "DD");
+      if ((rc = H(type, PCF_XX, cd, -1, "DD" + tag + dn, &h0)) != PCF_OK)
+        return die(run.ctx, rc, "DD");
       for (size_t i = 0; i < cr.size(); ++i) {
-        if ((rc = H(type, PCF_XX, cr[i], -1, "RR" + tag + histo_names[i + 1])) != PCF_OK)
+        if ((rc = H(type, PCF_XX, cr[i], -1, "RR" + tag + histo_names[i + 1], &h2)) != PCF_OK)
           return die(run.ctx, rc, "RR");
-        if ((rc = H(type, PCF_XY, cd, cr[i], "DR" + tag + histo_names[i + 1])) != PCF_OK)
+        if ((rc = H(type, PCF_XY, cd, cr[i], "DR" + tag + histo_names[i + 1], &h1)) != PCF_OK)
           return die(run.ctx, rc, "DR");
+        if (o.estimator &&
+            (rc = run.estimate(type, {&h0, &h1, &h2}, "LS" + tag + histo_names[i + 1])) != PCF_OK)
+          return die(run.ctx, rc, "estimator");
       }
     }
   } else {
@@ -311,7 +346,7 @@ int main(int argc, char** argv) {
     const string tag = (type == PCF_3ISO ? "iso_" : "ani_") + string(o.bpc ? "BPC_" : "");
     if (type == PCF_3ISO && o.bpc && o.analytic) {
       // cuda/pcf3isoBPC.cuh:177-375: periodic DDD, analytic RRR and DDR, DRR = DDR.
-      if ((rc = H(type, PCF_XXX, cd, -1, "DDDiso_BPCanalytic_" + dn)) != PCF_OK)
+      if ((rc = H(type, PCF_XXX, cd, -1, "DDDiso_BPCanalytic_" + dn, &h0)) != PCF_OK)
         return die(run.ctx, rc, "DDD");
       const size_t b3 = (size_t)o.bn * o.bn * o.bn;
       std::vector<double> ddr(b3), rrr(b3);
@@ -321,6 +356,9 @@ int main(int argc, char** argv) {
       run.save(type, "DDRiso_BPCanalytic_" + dn, ddr);
       run.save(type, "DRRiso_BPCanalytic_" + dn, ddr);
       run.save(type, "RRRiso_BPCanalytic_" + dn, rrr);
+      if (o.estimator &&
+          (rc = run.estimate(type, {&h0, &ddr, &ddr, &rrr}, "LS3iso_BPCanalytic_" + dn)) != PCF_OK)
+        return die(run.ctx, rc, "estimator");
       auto t1 = std::chrono::steady_clock::now();
       cout << "Spent " << std::chrono::duration<double, std::milli>(t1 - t0).count()
            << " miliseconds to compute and save all the histograms." << endl;
@@ -328,16 +366,19 @@ int main(int argc, char** argv) {
       cout << "Program terminated..." << endl;
       return 0;
     }
-    if ((rc = H(type, PCF_XXX, cd, -1, "DDD" + tag + dn)) != PCF_OK)
+    if ((rc = H(type, PCF_XXX, cd, -1, "DDD" + tag + dn, &h0)) != PCF_OK)
       return die(run.ctx, rc, "DDD");
     for (size_t i = 0; i < cr.size(); ++i) {
       const string& rn = histo_names[i + 1];
-      if ((rc = H(type, PCF_XXX, cr[i], -1, "RRR" + tag + rn)) != PCF_OK)
+      if ((rc = H(type, PCF_XXX, cr[i], -1, "RRR" + tag + rn, &h3)) != PCF_OK)
         return die(run.ctx, rc, "RRR");
-      if ((rc = H(type, PCF_XXY, cd, cr[i], "DDR" + tag + rn)) != PCF_OK)
+      if ((rc = H(type, PCF_XXY, cd, cr[i], "DDR" + tag + rn, &h1)) != PCF_OK)
         return die(run.ctx, rc, "DDR");
-      if ((rc = H(type, PCF_XXY, cr[i], cd, "DRR" + tag + rn)) != PCF_OK)
+      if ((rc = H(type, PCF_XXY, cr[i], cd, "DRR" + tag + rn, &h2)) != PCF_OK)
         return die(run.ctx, rc, "DRR");
+      if (o.estimator && (rc = run.estimate(type, {&h0, &h1, &h2, &h3}, "LS3" + tag + rn)) !=
+                             PCF_OK)
+        return die(run.ctx, rc, "estimator");
     }
   }
   auto t1 = std::chrono::steady_clock::now();

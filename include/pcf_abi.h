@@ -216,6 +216,22 @@ int pcf_analytic_3iso(pcf_ctx* ctx, pcf_catalog data, int32_t bins, float dmax, 
                       int64_t np, int32_t ptt, int32_t bn_ref, double* ddr, double* rrr,
                       double* ff_av, double* ff_av_ref);
 
+/* Correlation-function estimator from finished histograms, on the device
+ * (no counterpart in the reference's C++/CUDA programs: it restates the
+ * analysis scripts cpp/2PCFisotropic/normal/graph_2DPF.py:8-14 and
+ * cpp/3PCFisotropic/normal/graph_3PCF.py:5-6,21-22).  n values per histogram.
+ *   order 2: h0 = DD, h1 = DR, h2 = RR (h3 unused);
+ *            xi = (DD - 2 DR + RR) / RR, every zero of DD, DR, RR taken as 1e-11;
+ *   order 3: h0 = DDD, h1 = DDR, h2 = DRR, h3 = RRR;
+ *            xi = (DDD - 3 DDR + 3 DRR - RRR) / RRR, NaN or infinite -> 0.
+ * flags & PCF_EST_NORMALIZE divides each histogram by its number of ordered
+ * pairs (triplets) of distinct points first (n_data, n_rand: catalog sizes);
+ * without it the raw counts are used, as the scripts do. */
+#define PCF_EST_NORMALIZE 1
+int pcf_estimator(pcf_ctx* ctx, int32_t order, int64_t n, const double* h0, const double* h1,
+                  const double* h2, const double* h3, double n_data, double n_rand,
+                  int32_t flags, double* xi);
+
 /* ---- asynchronous form (serving) ------------------------------------- *
  * pcf_submit enqueues a histogram job; the engine admits it when enough HBM
  * (the budget set by pcf_set_hbm_budget) and a stream slot are free, runs

@@ -151,3 +151,33 @@ def triplets(a, b=None, *, ani=False, bpc=False, box=0.0, bins=30, dmax=80.0, th
     if rc != 0:
         raise ValueError(f"oracle triplets rc={rc}")
     return c, w
+
+
+def estimator(hists, n_data=0.0, n_rand=0.0, normalize=False):
+    """Landy-Szalay estimators as the reference's analysis scripts form them,
+    elementwise in FP64 (numpy).  3 histograms (DD, DR, RR):
+    (DD - 2 DR + RR) / RR with zeros of all three replaced by 1e-11
+    (cpp/2PCFisotropic/normal/graph_2DPF.py:8-14).  4 histograms (DDD, DDR,
+    DRR, RRR): (DDD - 3 DDR + 3 DRR - RRR) / RRR with NaN and inf set to 0
+    (cpp/3PCFisotropic/normal/graph_3PCF.py:5-6,21-22).  normalize divides each
+    histogram by its number of ordered pairs (triplets) of distinct points."""
+    h = [np.array(x, dtype=np.float64, copy=True).ravel() for x in hists]
+    nd, nr = float(n_data), float(n_rand)
+    if len(h) == 3:
+        if normalize:
+            for x, s in zip(h, (nd * (nd - 1), nd * nr, nr * (nr - 1))):
+                x /= s
+        dd, dr, rr = h
+        for x in (dd, rr, dr):
+            x[x == 0] = 0.00000000001
+        return (dd - 2 * dr + rr) / rr
+    if normalize:
+        for x, s in zip(h, (nd * (nd - 1) * (nd - 2), nd * (nd - 1) * nr, nd * nr * (nr - 1),
+                            nr * (nr - 1) * (nr - 2))):
+            x /= s
+    ddd, ddr, drr, rrr = h
+    with np.errstate(divide="ignore", invalid="ignore"):
+        xi = (ddd - 3 * ddr + 3 * drr - rrr) / rrr
+    xi[np.isnan(xi)] = 0.0
+    xi[np.isinf(xi)] = 0.0
+    return xi

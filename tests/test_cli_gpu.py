@@ -100,3 +100,43 @@ def test_3iso_analytic_files(tmp_path, engine):
     np.testing.assert_allclose(ddr, g_ddr, rtol=1e-11, atol=0)
     np.testing.assert_allclose(rrr, g_rrr, rtol=1e-11, atol=0)
     assert np.array_equal(_load(res / "DDDiso_BPCanalytic_data.dat"), ddd.astype(np.float64))
+
+
+def test_estimator_files(tmp_path):
+    """-e writes the Landy-Szalay estimator beside the histograms, equal (to
+    the 12 printed digits) to the scripts' formula on the oracle's counts."""
+    res = _run(tmp_path, "2iso", "-n", "1000", "-f", "data_1K.dat", "-r", "rand0_1K.dat",
+               "-b", "30", "-d", "80", "-bpc", "-s", "250", "-e")
+    d, r = catalog("data_1K.dat"), catalog("rand0_1K.dat")
+    hs = [O.pairs(a, b, bpc=True, box=250.0, bins=30, dmax=80.0)[0].astype(np.float64)
+          for a, b in ((d, None), (d, r), (r, None))]
+    xi = O.estimator(hs)
+    np.testing.assert_allclose(_load(res / "LSiso_BPC_rand0_1K.dat"), xi, rtol=1e-11, atol=1e-12)
+    res = _run(tmp_path, "3iso", "-n", "500", "-f", "data_500.dat", "-r", "rand0_500.dat",
+               "-b", "8", "-d", "40", "-e")
+    d, r = catalog("data_500.dat"), catalog("rand0_500.dat")
+    box = float(np.ceil(max(d[:, :3].max(), r[:, :3].max()) + 1))
+    hs = [O.triplets(a, b, box=box, bins=8, dmax=40.0)[0].astype(np.float64)
+          for a, b in ((d, None), (d, r), (r, d), (r, None))]
+    xi3 = O.estimator(hs)
+    np.testing.assert_allclose(_load(res / "LS3iso_rand0_500.dat"), xi3, rtol=1e-11, atol=1e-12)
+
+
+def test_3ani_full_5d_file(tmp_path):
+    """PCF_SAVE_5D=1 writes the whole bins^5 histogram in save_histogram5D's
+    layout (cuda/create_grid.cuh:230-266); by default the bins^3 slab."""
+    args = ("3ani", "-n", "300", "-f", "data_500.dat", "-r", "rand0_500.dat", "-b", "5", "-d",
+            "40", "-bpc", "-s", "250")
+    res = _run(tmp_path, *args)
+    slab = _load(res / "DDDani_BPC_data_500.dat")
+    res5 = tmp_path / "r5"
+    res5.mkdir()
+    env = dict(os.environ, PCF_DATA_DIR=CATS, PCF_RESULTS_DIR=str(res5), PCF_SAVE_5D="1")
+    p = subprocess.run([da.CLI_PATH, *args], capture_output=True, text=True, env=env,
+                       timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    full = _load(res5 / "DDDani_BPC_data_500.dat")
+    d = catalog("data_500.dat")[:300]
+    c, _ = O.triplets(d, None, ani=True, bpc=True, box=250.0, bins=5, dmax=40.0)
+    assert np.array_equal(full, c.astype(np.float64))
+    assert np.array_equal(slab, full[:5 ** 3])
