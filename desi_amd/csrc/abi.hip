@@ -433,9 +433,7 @@ void free_catalog(Catalog* cat) {
 // ---------------------------------------------------------------------------
 // Pair histograms (2iso / 2ani)
 // ---------------------------------------------------------------------------
-namespace {
-
-int64_t gx_points(const Catalog* X) { return X->n; }
+namespace pcf {
 
 // Largest tile box half-diagonal the Gram path's error bound is built for:
 // a 64-point tile spans about 64 / (points per cell) cells of its column
@@ -450,6 +448,12 @@ double gram_rho(const GridGeom& g, int64_t n) {
   const double r2 = 0.25 * ((double)g.cs[0] * g.cs[0] + (double)g.cs[1] * g.cs[1] + cz * cz);
   return std::sqrt(r2) * 1.02;
 }
+
+}  // namespace pcf
+
+namespace {
+
+int64_t gx_points(const Catalog* X) { return X->n; }
 
 int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights) {
   const bool auto_ = job->hist == PCF_XX;
@@ -477,10 +481,10 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   // The mixed kernel's recheck queue indexes j-side points with 27 bits.
   const bool plain = job->precision == PCF_PREC_FP64 || X->n >= kRecheckMaxPoints ||
                      Y->n >= kRecheckMaxPoints;
-  // Gram-form path (matrix cores): 2iso, unweighted, mixed precision.  Its
-  // cells are cut kGramZsub times thinner along z: the j range of a tile then
-  // ends within a thin cell of what the tile can reach.
-  const bool gram = !plain && !ani && !weighted && job->pair_kernel != PCF_PAIR_DIRECT;
+  // Gram-form path (matrix cores): unweighted, mixed precision.  Its cells
+  // are cut kGramZsub times thinner along z: the j range of a tile then ends
+  // within a thin cell of what the tile can reach.
+  const bool gram = !plain && !weighted && job->pair_kernel != PCF_PAIR_DIRECT;
 
   GridGeom g;
   int K[3];

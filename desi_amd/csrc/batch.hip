@@ -36,7 +36,8 @@ struct BatchPlan {
   // geometry / rules
   GridGeom g{};
   int nflat = 0, bins = 0, ani = 0, weighted = 0, cshift = 0, nt = 0, nblocks = 0, m = 0;
-  float inv_h = 1.f, sigma = 1.f;
+  bool gram = false;  // Gram form of the pair kernel (matrix cores)
+  float inv_h = 1.f, sigma = 1.f, rho2 = 0.f;
   BinRules rules{};
   int total_items = 0;
   std::vector<int> job_offsets;
@@ -67,7 +68,8 @@ bool same_job(const pcf_job& a, const pcf_job& b) {
   return a.type == b.type && a.hist == b.hist && a.bpc == b.bpc && a.bins == b.bins &&
          a.dmax == b.dmax && a.box == b.box && a.precision == b.precision &&
          a.cat_x == b.cat_x && a.cat_y == b.cat_y && a.cells_per_rmax == b.cells_per_rmax &&
-         a.lds_copies_log2 == b.lds_copies_log2 && a.blocks_per_cu == b.blocks_per_cu;
+         a.lds_copies_log2 == b.lds_copies_log2 && a.blocks_per_cu == b.blocks_per_cu &&
+         a.pair_kernel == b.pair_kernel;
 }
 
 // Graph body: scaled copies, histogram + counter clear, the batched kernel.
@@ -111,6 +113,8 @@ hipError_t enqueue_plan(pcf_ctx* c, BatchPlan* p, size_t njobs, bool prepare) {
   pl.le = p->rules.le;
   pl.zero = p->rules.zero;
   pl.batch = 1;
+  pl.mfma = p->gram ? 1 : 0;
+  pl.rho2 = p->rho2;
   pl.jobs = p->jobs_d.as<BatchJob>();
   pl.chunks = p->chunks_d.as<BatchChunk>();
   pl.nchunks = (int)p->chunks.size();
@@ -153,10 +157,13 @@ int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p, GridPins
     any = true;
   }
   U.n = any ? (int64_t)std::llround(nsum / (double)ids.size()) : 0;
+  // Gram form (matrix cores) for unweighted mixed-precision batches, on thin
+  // z cells; PCF_PAIR_DIRECT in the first job selects the direct form.
+  p->gram = all_w1 && j0.pair_kernel != PCF_PAIR_DIRECT;
   int K[3];
   float reach;
-  if (choose_geometry(&U, nullptr, j0.dmax, bpc, L, j0.cells_per_rmax, &p->g, K, &reach, &p->m) !=
-      0)
+  if (choose_geometry(&U, nullptr, j0.dmax, bpc, L, j0.cells_per_rmax, &p->g, K, &reach, &p->m,
+                      p->gram ? kGramZsub : 1) != 0)
     return fail(c, PCF_ERR_UNSUPPORTED, "geometry");
   // Grids (resident, cached per catalog and geometry).
   int rc = PCF_OK;
@@ -184,8 +191,10 @@ int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p, GridPins
   make_rules(j0.type, bpc, 1, bins, j0.dmax, L, &p->rules);
   const double A = bpc ? (double)maxabs + (double)L : 0.0;
   std::vector<uint8_t> table;
-  if (build_table(tab_rules, A, kMixedTableMax, &table, &p->inv_h, &p->nt) != 0)
+  const double rho = p->gram ? gram_rho(p->g, U.n) : -1.0;
+  if (build_table(tab_rules, A, kMixedTableMax, &table, &p->inv_h, &p->nt, rho) != 0)
     return fail(c, PCF_ERR_UNSUPPORTED, "bin table");
+  p->rho2 = p->gram ? (float)(rho * rho / (1.02 * 1.02) * (double)p->inv_h) : 0.f;
   p->sigma = std::sqrt(p->inv_h);
   p->bins = bins;
   p->ani = ani ? 1 : 0;

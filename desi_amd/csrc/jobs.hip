@@ -61,7 +61,7 @@ bool compatible(const pcf_job& a, const pcf_job& b) {
   return batchable(a) && batchable(b) && a.type == b.type && a.bins == b.bins &&
          a.dmax == b.dmax && a.bpc == b.bpc && (!a.bpc || a.box == b.box) &&
          a.cells_per_rmax == b.cells_per_rmax && a.lds_copies_log2 == b.lds_copies_log2 &&
-         a.blocks_per_cu == b.blocks_per_cu;
+         a.blocks_per_cu == b.blocks_per_cu && a.pair_kernel == b.pair_kernel;
 }
 
 }  // namespace

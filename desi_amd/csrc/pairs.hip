@@ -166,8 +166,8 @@ constexpr int kMaxTable = kMixedTableMax;
 // LDS histogram into that job's rows and clears it.  Point arrays and the zero
 // rule come from the chunk's job descriptor; xp..yp_orig are unused.
 //
-// MF = 1 (2iso, unweighted): squared distances of an item come from the
-// matrix cores in Gram form, d2 = |xi|^2 + |xj|^2 - 2 xi.xj, 16 j x 16 i per
+// MF = 1 (unweighted; 2ani: see run_item_mf): squared distances of an item
+// come from the matrix cores in Gram form, d2 = |xi|^2 + |xj|^2 - 2 xi.xj, 16 j x 16 i per
 // v_mfma_f32_16x16x4f32 (rows = j: A operand (x, y, z, |x|^2) of j;
 // columns = i: B operand (-2x, -2y, -2z, 1) of i; C = |x_i|^2 broadcast down
 // the column).  Coordinates are taken relative to the centre of the i-tile's
@@ -365,27 +365,13 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
     evals += (unsigned long long)ni * (unsigned long long)nj;
 
     const int nb = (ni + 15) >> 4;
-    // i side: B operand (component grp of point i = 16b + col, times -2;
-    // grp 3: 1) and the C column |x_i|^2; lanes past the tile get a huge
-    // |x_i|^2 (always out).
-    float bop[4], cin[4];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int ir = 16 * b + col;
-      const float4 p = xp[i0 + min(ir, ni - 1)];
-      const float x = (p.x - shx) - ox, y = (p.y - shy) - oy, z = (p.z - shz) - oz;
-      // Exact select by 0/1 coefficients (products by 0 and 1 are exact).
-      bop[b] = __builtin_fmaf(sel_x, -2.0f * x,
-                              __builtin_fmaf(sel_y, -2.0f * y, __builtin_fmaf(sel_z, -2.0f * z, sel_w)));
-      cin[b] = ir < ni ? __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x)) : 1e30f;
-    }
     const int jend = j0 + nj;
-    auto fix2 = [&](unsigned& r, int ii, int jj) {
-      const unsigned long long amb = __ballot(r == amb_row);
+    auto fix2 = [&](unsigned& r, bool is_amb, int ii, int jj) {
+      const unsigned long long amb = __ballot(is_amb);
       if (amb) {
         const int na = __popcll(amb);
         nre += na;
-        if (r == amb_row) {
+        if (is_amb) {
           const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi(
                                    (unsigned)(amb >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)amb, 0u));
           wq[pos] = make_int2(ii, jj | (img << kRecheckImgShift));
@@ -395,77 +381,201 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
         if (qn >= kWave) drain(kWave, xp_orig, yp_orig, zero);
       }
     };
-    // The j loop for a tile of NB i-blocks (NB*16 >= ni), 16 j per step.
-    auto jloop = [&](auto nbc) {
-      constexpr int NB = decltype(nbc)::value;
-      // The next step's j point is loaded one step ahead (its latency hides
-      // behind this step's matrix, LDS and histogram work).
-      float4 qn4 = yp[min(j0 + col, jend - 1)];
-      for (int jb = j0; jb < jend; jb += 16) {
-        // j side: A operand (component grp of point j = jb + col; grp 3:
-        // |x_j|^2, huge past the run's end).
-        const int jj = jb + col;
-        const float4 q = qn4;
-        qn4 = yp[min(jj + 16, jend - 1)];
-        const float x = q.x - ox, y = q.y - oy, z = q.z - oz;
-        const float bj = jj < jend ? __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x)) : 1e30f;
-        const float aop =
-            __builtin_fmaf(sel_x, x, __builtin_fmaf(sel_y, y, __builtin_fmaf(sel_z, z, sel_w * bj)));
-        // Two blocks at a time (fewer live registers, so more waves fit).
+    if constexpr (!ANI) {
+      // i side: B operand (component grp of point i = 16b + col, times -2;
+      // grp 3: 1) and the C column |x_i|^2; lanes past the tile get a huge
+      // |x_i|^2 (always out).
+      float bop[4], cin[4];
 #pragma unroll
-        for (int h = 0; h < NB; h += 2) {
-          constexpr int H = 2;
-          const int nh = (NB - h) < H ? (NB - h) : H;
-          f4v d[H];
+      for (int b = 0; b < 4; ++b) {
+        const int ir = 16 * b + col;
+        const float4 p = xp[i0 + min(ir, ni - 1)];
+        const float x = (p.x - shx) - ox, y = (p.y - shy) - oy, z = (p.z - shz) - oz;
+        // Exact select by 0/1 coefficients (products by 0 and 1 are exact).
+        bop[b] = __builtin_fmaf(sel_x, -2.0f * x,
+                                __builtin_fmaf(sel_y, -2.0f * y, __builtin_fmaf(sel_z, -2.0f * z, sel_w)));
+        cin[b] = ir < ni ? __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x)) : 1e30f;
+      }
+      // The j loop for a tile of NB i-blocks (NB*16 >= ni), 16 j per step.
+      auto jloop = [&](auto nbc) {
+        constexpr int NB = decltype(nbc)::value;
+        // The next step's j point is loaded one step ahead (its latency hides
+        // behind this step's matrix, LDS and histogram work).
+        float4 qn4 = yp[min(j0 + col, jend - 1)];
+        for (int jb = j0; jb < jend; jb += 16) {
+          // j side: A operand (component grp of point j = jb + col; grp 3:
+          // |x_j|^2, huge past the run's end).
+          const int jj = jb + col;
+          const float4 q = qn4;
+          qn4 = yp[min(jj + 16, jend - 1)];
+          const float x = q.x - ox, y = q.y - oy, z = q.z - oz;
+          const float bj = jj < jend ? __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x)) : 1e30f;
+          const float aop =
+              __builtin_fmaf(sel_x, x, __builtin_fmaf(sel_y, y, __builtin_fmaf(sel_z, z, sel_w * bj)));
+          // Two blocks at a time (fewer live registers, so more waves fit).
 #pragma unroll
-          for (int u = 0; u < H; ++u) {
-            if (u < nh) {
-              const int b = h + u;
-              const f4v cv = {cin[b], cin[b], cin[b], cin[b]};
-              d[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop, bop[b], cv, 0, 0, 0);
+          for (int h = 0; h < NB; h += 2) {
+            constexpr int H = 2;
+            const int nh = (NB - h) < H ? (NB - h) : H;
+            f4v d[H];
+#pragma unroll
+            for (int u = 0; u < H; ++u) {
+              if (u < nh) {
+                const int b = h + u;
+                const f4v cv = {cin[b], cin[b], cin[b], cin[b]};
+                d[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop, bop[b], cv, 0, 0, 0);
+              }
             }
-          }
-          unsigned r[4 * H];
-#pragma unroll
-          for (int u = 0; u < H; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v)
-              r[4 * u + v] = u < nh ? tab[min(__float2uint_rz(d[u][v]), ntm1)] : 0u;
-          // Element (b, v) of this lane: i = i0 + 16b + col, j = jb + 4*grp + v.
-          if (masked) {
+            unsigned r[4 * H];
 #pragma unroll
             for (int u = 0; u < H; ++u)
 #pragma unroll
               for (int v = 0; v < 4; ++v)
-                r[4 * u + v] =
-                    (jb + 4 * grp + v <= i0 + 16 * (h + u) + col) ? out_row : r[4 * u + v];
-          }
-          unsigned m = r[0];
+                r[4 * u + v] = u < nh ? tab[min(__float2uint_rz(d[u][v]), ntm1)] : 0u;
+            // Element (b, v) of this lane: i = i0 + 16b + col, j = jb + 4*grp + v.
+            if (masked) {
 #pragma unroll
-          for (int e = 1; e < 4 * H; ++e) m = max(m, r[e]);
-          if (__builtin_expect(__ballot(m == amb_row) != 0, 0)) {
+              for (int u = 0; u < H; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                  r[4 * u + v] =
+                      (jb + 4 * grp + v <= i0 + 16 * (h + u) + col) ? out_row : r[4 * u + v];
+            }
+            unsigned m = r[0];
+#pragma unroll
+            for (int e = 1; e < 4 * H; ++e) m = max(m, r[e]);
+            if (__builtin_expect(__ballot(m == amb_row) != 0, 0)) {
+#pragma unroll
+              for (int u = 0; u < H; ++u)
+                if (u < nh) {
+#pragma unroll
+                  for (int v = 0; v < 4; ++v)
+                    fix2(r[4 * u + v], r[4 * u + v] == amb_row, i0 + 16 * (h + u) + col,
+                         jb + 4 * grp + v);
+                }
+            }
 #pragma unroll
             for (int u = 0; u < H; ++u)
               if (u < nh) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v)
-                  fix2(r[4 * u + v], i0 + 16 * (h + u) + col, jb + 4 * grp + v);
+                for (int v = 0; v < 4; ++v) atomicAdd(hc_lane + (r[4 * u + v] << CS), 1u);
               }
           }
-#pragma unroll
-          for (int u = 0; u < H; ++u)
-            if (u < nh) {
-#pragma unroll
-              for (int v = 0; v < 4; ++v) atomicAdd(hc_lane + (r[4 * u + v] << CS), 1u);
-            }
         }
+      };
+      switch (nb) {
+        case 4: jloop(std::integral_constant<int, 4>{}); break;
+        case 3: jloop(std::integral_constant<int, 3>{}); break;
+        case 2: jloop(std::integral_constant<int, 2>{}); break;
+        default: jloop(std::integral_constant<int, 1>{}); break;
       }
-    };
-    switch (nb) {
-      case 4: jloop(std::integral_constant<int, 4>{}); break;
-      case 3: jloop(std::integral_constant<int, 3>{}); break;
-      case 2: jloop(std::integral_constant<int, 2>{}); break;
-      default: jloop(std::integral_constant<int, 1>{}); break;
+    } else {
+      // i side, 2ani: two products per block.  Perpendicular: B operand
+      // (-2x, -2y, 1, |x_perp|^2) of i against A (x, y, |x_perp|^2, 1) of j;
+      // line of sight: B (-2z, 1, z^2, 0) against A (z, z^2, 1, 0).  C = 0 for
+      // both (the i side's squared norm rides in the K dimension).
+      float bopp[4], bopz[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int ir = 16 * b + col;
+        const float4 p = xp[i0 + min(ir, ni - 1)];
+        const float x = (p.x - shx) - ox, y = (p.y - shy) - oy, z = (p.z - shz) - oz;
+        // A lane past the tile gets huge squared norms in both products, so
+        // neither component can read as ambiguous (its pair does not exist).
+        const bool iv = ir < ni;
+        const float np2 = iv ? __builtin_fmaf(y, y, x * x) : 1e30f;
+        const float nz2 = iv ? z * z : 1e30f;
+        bopp[b] = __builtin_fmaf(sel_x, -2.0f * x,
+                                 __builtin_fmaf(sel_y, -2.0f * y, __builtin_fmaf(sel_w, np2, sel_z)));
+        bopz[b] = __builtin_fmaf(sel_x, -2.0f * z, __builtin_fmaf(sel_z, nz2, sel_y));
+      }
+      // The j loop for a tile of NB i-blocks (NB*16 >= ni), 16 j per step.
+      auto jloop = [&](auto nbc) {
+        constexpr int NB = decltype(nbc)::value;
+        const f4v zero4 = {0.f, 0.f, 0.f, 0.f};
+        float4 qn4 = yp[min(j0 + col, jend - 1)];
+        for (int jb = j0; jb < jend; jb += 16) {
+          // j side: A operands (component grp of point j = jb + col; a j past
+          // the run's end gets huge squared norms in both).
+          const int jj = jb + col;
+          const float4 q = qn4;
+          qn4 = yp[min(jj + 16, jend - 1)];
+          const float x = q.x - ox, y = q.y - oy, z = q.z - oz;
+          const bool jv = jj < jend;
+          const float np2 = jv ? __builtin_fmaf(y, y, x * x) : 1e30f;
+          const float nz2 = jv ? z * z : 1e30f;
+          const float aopp =
+              __builtin_fmaf(sel_x, x, __builtin_fmaf(sel_y, y, __builtin_fmaf(sel_z, np2, sel_w)));
+          const float aopz = __builtin_fmaf(sel_x, z, __builtin_fmaf(sel_y, nz2, sel_z));
+#pragma unroll
+          for (int h = 0; h < NB; h += 2) {
+            constexpr int H = 2;
+            const int nh = (NB - h) < H ? (NB - h) : H;
+            f4v dp[H], dz[H];
+#pragma unroll
+            for (int u = 0; u < H; ++u) {
+              if (u < nh) {
+                dp[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(aopp, bopp[h + u], zero4, 0, 0, 0);
+                dz[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(aopz, bopz[h + u], zero4, 0, 0, 0);
+              }
+            }
+            // Per element: line-of-sight and perpendicular codes (bin, out,
+            // or ambiguous = 255, the largest); the row is cz * bins + co when
+            // both are bins.
+            unsigned r[4 * H], mc[4 * H];
+#pragma unroll
+            for (int u = 0; u < H; ++u)
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                if (u < nh) {
+                  const unsigned cz = tab[min(__float2uint_rz(dz[u][v]), ntm1)];
+                  const unsigned co = tab[min(__float2uint_rz(dp[u][v]), ntm1)];
+                  const unsigned m = max(cz, co);
+                  mc[4 * u + v] = m;
+                  r[4 * u + v] = m < bins ? __umul24(cz, bins) + co : out_row;
+                } else {
+                  mc[4 * u + v] = 0u;
+                  r[4 * u + v] = out_row;
+                }
+              }
+            if (masked) {
+#pragma unroll
+              for (int u = 0; u < H; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                  if (jb + 4 * grp + v <= i0 + 16 * (h + u) + col) {
+                    r[4 * u + v] = out_row;
+                    mc[4 * u + v] = 0u;
+                  }
+            }
+            unsigned m = mc[0];
+#pragma unroll
+            for (int e = 1; e < 4 * H; ++e) m = max(m, mc[e]);
+            if (__builtin_expect(__ballot(m == kAniAmb8) != 0, 0)) {
+#pragma unroll
+              for (int u = 0; u < H; ++u)
+                if (u < nh) {
+#pragma unroll
+                  for (int v = 0; v < 4; ++v)
+                    fix2(r[4 * u + v], mc[4 * u + v] == kAniAmb8, i0 + 16 * (h + u) + col,
+                         jb + 4 * grp + v);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < H; ++u)
+              if (u < nh) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) atomicAdd(hc_lane + (r[4 * u + v] << CS), 1u);
+              }
+          }
+        }
+      };
+      switch (nb) {
+        case 4: jloop(std::integral_constant<int, 4>{}); break;
+        case 3: jloop(std::integral_constant<int, 3>{}); break;
+        case 2: jloop(std::integral_constant<int, 2>{}); break;
+        default: jloop(std::integral_constant<int, 1>{}); break;
+      }
     }
     return true;
   };
@@ -541,6 +651,10 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
         if (lane == 0) it = cb + atomicAdd(&s_next, 1);
         it = __builtin_amdgcn_readfirstlane(it);
         if (it >= ce) break;
+        if constexpr (MF) {
+          if (run_item_mf(it, jd.xs, jd.ys, jd.xo, jd.yo, jd.zero)) continue;
+          ++ndirect;
+        }
         run_item(it, jd.xs, jd.ys, jd.xo, jd.yo, jd.zero);
       }
       if (qn > 0) drain(qn, jd.xo, jd.yo, jd.zero);
@@ -840,11 +954,14 @@ hipError_t launch_pairs(const PairLaunch& pl, hipStream_t s) {
                        pl.yp, pl.items, pl.counter, pl.table, pl.rules, pl.hist, pl.whist,        \
                        pl.stats, k, pl.jobs, pl.chunks);                                          \
   } while (0)
-#define PCF_LAUNCH_MIXED(A, W, C)                                  \
-  do {                                                             \
-    if (pl.batch) PCF_LAUNCH_MIXED2(A, W, C, 1, 0);                \
-    else if (A == 0 && W == 0 && pl.mfma) PCF_LAUNCH_MIXED2(0, 0, C, 0, 1); \
-    else PCF_LAUNCH_MIXED2(A, W, C, 0, 0);                         \
+#define PCF_LAUNCH_MIXED(A, W, C)                                       \
+  do {                                                                  \
+    const bool mf_ = W == 0 && pl.mfma;                                 \
+    if (pl.batch) {                                                     \
+      if (mf_) PCF_LAUNCH_MIXED2(A, 0, C, 1, 1);                        \
+      else PCF_LAUNCH_MIXED2(A, W, C, 1, 0);                            \
+    } else if (mf_) PCF_LAUNCH_MIXED2(A, 0, C, 0, 1);                   \
+    else PCF_LAUNCH_MIXED2(A, W, C, 0, 0);                              \
   } while (0)
 #define PCF_LAUNCH_CS(A, W)                                   \
   switch (pl.cshift) {                                        \

@@ -116,6 +116,9 @@ int build_table(const BinRules& r, double A, int nt_max, std::vector<uint8_t>* t
 // zsub > 1 cuts every cell into zsub thinner cells along z (the line of the
 // grid's columns), so the j range of a tile (whole z cells) overshoots what
 // the tile can reach by at most a thin cell at either end.
+// Largest tile half-diagonal (unscaled units) the Gram pair path's error
+// bound is built for on geometry g with n points per catalog.
+double gram_rho(const GridGeom& g, int64_t n);
 int choose_geometry(const Catalog* X, const Catalog* Y, float dmax, int bpc, float box,
                     int cells_per_rmax, GridGeom* g, int K[3], float* reach_out, int* m_out,
                     int zsub = 1);

@@ -62,9 +62,10 @@ struct BinRules {
 };
 
 // Table codes of the FP32 bulk path for 2ani (per component; iso tables hold
-// the histogram row directly).
-constexpr uint8_t kAniAmb8 = 254;
-constexpr uint8_t kAniOut8 = 255;
+// the histogram row directly).  Ambiguous is the largest code, so the max of
+// a pair's two codes (or of many) shows whether the exact path is needed.
+constexpr uint8_t kAniAmb8 = 255;
+constexpr uint8_t kAniOut8 = 254;
 
 struct DevGrid {
   GridGeom g;

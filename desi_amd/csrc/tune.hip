@@ -1,7 +1,7 @@
 // tune.hip -- autotuner of the pair kernels (pcf_autotune).
 //
-// The knobs are the pair kernel (single unweighted 2iso jobs: the direct
-// per-lane form or the Gram form on the matrix cores), the cell edge (rmax / m:
+// The knobs are the pair kernel (unweighted jobs: the direct per-lane form or
+// the Gram form on the matrix cores), the cell edge (rmax / m:
 // candidates per useful pair against work items per point), the number of LDS
 // copies of each histogram row
 // (fewer bank conflicts on the deposit against LDS per workgroup) and the
@@ -147,9 +147,9 @@ extern "C" int pcf_autotune(pcf_ctx* c, const pcf_job* jobs, int32_t njobs, pcf_
     return PCF_OK;
   };
 
-  // 0. pair kernel: single unweighted 2iso jobs run the Gram form by default;
-  // the direct form competes for them.
-  const bool gram_ok = njobs == 1 && type == PCF_2ISO && !weighted;
+  // 0. pair kernel: unweighted jobs (and batches) run the Gram form by
+  // default; the direct form competes for them.
+  const bool gram_ok = !weighted;
   Trial best{m_def, 0, 0, gram_ok ? PCF_PAIR_GRAM : PCF_PAIR_AUTO};
   double t_best = t_def;
   if (gram_ok) {

@@ -99,8 +99,9 @@ typedef struct pcf_job {
  * Both give the same counts; they differ in where d2 is formed.  DIRECT:
  * per lane, (xi - xj)^2 summed in FP32 on the vector units.  GRAM: 16 x 16
  * pairs at a time on the matrix cores as |xi|^2 + |xj|^2 - 2 xi.xj in
- * tile-local coordinates (2iso, unweighted; other jobs run DIRECT).  AUTO is
- * GRAM where it applies. */
+ * tile-local coordinates (2ani: perpendicular and line-of-sight parts as two
+ * products; unweighted jobs only, weighted ones run DIRECT).  AUTO is GRAM
+ * where it applies.  In pcf_histogram_batch the first job's value applies. */
 #define PCF_PAIR_AUTO 0
 #define PCF_PAIR_DIRECT 1
 #define PCF_PAIR_GRAM 2
@@ -173,7 +174,7 @@ int pcf_histogram_batch(pcf_ctx* ctx, const pcf_job* jobs, int32_t njobs, uint64
  * cuda/pcf2iso.cuh:74-80, and whose -p partitions flag sets the node size,
  * cuda/main.cu:86).  Runs the given job, or batch of pair jobs (njobs > 1, as
  * for pcf_histogram_batch), on its own catalogs under candidate pair kernels
- * (single 2iso jobs: direct or Gram), cell sizes, LDS histogram layouts and
+ * (unweighted jobs: direct or Gram), cell sizes, LDS histogram layouts and
  * occupancies, verifies that every candidate gives
  * the default configuration's counts, and caches the fastest on the context:
  * later pcf_histogram / pcf_histogram_batch calls on the same workload (same
@@ -186,8 +187,8 @@ typedef struct pcf_tuning {
   int32_t trials;          /* histogram runs spent                           */
   double best_ms;          /* counting-kernel time of the choice             */
   double default_ms;       /* same, default configuration                    */
-  int32_t pair_kernel;     /* PCF_PAIR_DIRECT or PCF_PAIR_GRAM (2iso,        */
-                           /* unweighted); PCF_PAIR_AUTO otherwise           */
+  int32_t pair_kernel;     /* PCF_PAIR_DIRECT or PCF_PAIR_GRAM (unweighted   */
+                           /* jobs); PCF_PAIR_AUTO otherwise                 */
   int32_t reserved;
 } pcf_tuning;
 int pcf_autotune(pcf_ctx* ctx, const pcf_job* jobs, int32_t njobs, pcf_tuning* out);

@@ -41,20 +41,24 @@ def test_cross_config_equality(engine, ani, bpc):
         engine.release(cr)
 
 
+@pytest.mark.parametrize("ani", [False, True])
 @pytest.mark.parametrize("bpc", [False, True])
-def test_cross_config_equality_pair_kernels(engine, bpc):
+def test_cross_config_equality_pair_kernels(engine, ani, bpc):
     # Unweighted 2iso: the Gram path (matrix cores, thin z cells) and the
     # direct path under every cell size and LDS layout give the same counts.
     d = synth.neyman_scott(60, 40, 6.0, 300.0, seed=103)
     r = synth.uniform(3000, 300.0, seed=104)
     cd, cr = engine.upload(d), engine.upload(r)
-    print(f"case pair kernels bpc={bpc}", flush=True)
+    print(f"case pair kernels ani={ani} bpc={bpc}", flush=True)
+    type_ = da.PCF_2ANI if ani else da.PCF_2ISO
+    bins = 12 if ani else 25
     try:
         for hist, y in ((da.PCF_XX, -1), (da.PCF_XY, cr)):
-            co, _ = O.pairs(d, None if y < 0 else r, bpc=bpc, box=300.0, bins=25, dmax=45.0)
+            co, _ = O.pairs(d, None if y < 0 else r, ani=ani, bpc=bpc, box=300.0, bins=bins,
+                            dmax=45.0)
             for kern, m, lds in itertools.product((da.PCF_PAIR_GRAM, da.PCF_PAIR_DIRECT),
                                                   (1, 2, 3, 5), (0, 2, 7)):
-                job = da.make_job(da.PCF_2ISO, hist, bins=25, dmax=45.0, cat_x=cd, cat_y=y,
+                job = da.make_job(type_, hist, bins=bins, dmax=45.0, cat_x=cd, cat_y=y,
                                   bpc=bpc, box=300.0, cells_per_rmax=m, lds_copies_log2=lds,
                                   pair_kernel=kern)
                 c, _ = engine.histogram(job)

@@ -1,6 +1,7 @@
-"""GPU parity of the Gram-form 2iso pair path (squared distances on the matrix
-cores as |xi|^2 + |xj|^2 - 2 xi.xj in tile-local coordinates) against the FP64
-oracle and against the direct FP32 path, through the C ABI.  Counts must be
+"""GPU parity of the Gram-form pair path (squared distances on the matrix cores
+as |xi|^2 + |xj|^2 - 2 xi.xj in tile-local coordinates; 2ani as perpendicular
+and line-of-sight products) against the FP64 oracle and against the direct FP32
+path, single and batched, through the C ABI.  Counts must be
 bit-exact.  Catalogs are chosen so that some tiles exceed the Gram path's error
 bound and run on the direct path inside the same launch (direct_items > 0),
 and so that many pairs sit exactly on bin edges (dyadic lattices)."""
@@ -100,3 +101,72 @@ def test_gram_large_vs_fp64_plain(engine):
                        kernel=da.PCF_PAIR_AUTO, prec=da.PCF_PREC_FP64)
         assert np.array_equal(c, c64)
         assert st["direct_items"] < st["work_items"] // 10
+
+
+def _ani_both(engine, a, b, **kw):
+    print(f"case 2ani n={len(a)} cross={b is not None} {kw}", flush=True)
+    out = []
+    for kern in (da.PCF_PAIR_GRAM, da.PCF_PAIR_DIRECT):
+        ca = engine.upload(a)
+        cb = engine.upload(b) if b is not None else -1
+        try:
+            job = da.make_job(da.PCF_2ANI, da.PCF_XX if b is None else da.PCF_XY, cat_x=ca,
+                              cat_y=cb, pair_kernel=kern, **kw)
+            c, _ = engine.histogram(job)
+            out.append((c, engine.stats()))
+        finally:
+            engine.release(ca)
+            if cb >= 0:
+                engine.release(cb)
+    assert np.array_equal(out[0][0], out[1][0])
+    return out[0]
+
+
+@pytest.mark.parametrize("bpc", [False, True])
+@pytest.mark.parametrize("cross", [False, True])
+def test_gram_2ani_vs_oracle(engine, bpc, cross):
+    d = synth.neyman_scott(300, 40, 5.0, 300.0, seed=71)
+    r = synth.uniform(12000, 300.0, seed=72)
+    b = r if cross else None
+    c, st = _ani_both(engine, d, b, bins=20, dmax=60.0, bpc=bpc, box=300.0)
+    co, _ = O.pairs(d, b, ani=True, bpc=bpc, box=300.0, bins=20, dmax=60.0)
+    assert np.array_equal(c, co)
+
+
+@pytest.mark.parametrize("bpc", [False, True])
+def test_gram_2ani_lattice_edges(engine, bpc):
+    a = synth.lattice(5000, 64.0, 0.5, seed=15)
+    b = synth.lattice(3000, 64.0, 0.5, seed=16)
+    for bb in (None, b):
+        c, st = _ani_both(engine, a, bb, bins=10, dmax=10.0, bpc=bpc, box=64.0)
+        co, _ = O.pairs(a, bb, ani=True, bpc=bpc, box=64.0, bins=10, dmax=10.0)
+        assert np.array_equal(c, co)
+        assert st["rechecks"] > 0
+
+
+def test_gram_2ani_batch_vs_direct_batch(engine):
+    """The batched persistent launch in both forms: every histogram equal, and
+    equal to the oracle."""
+    d = synth.neyman_scott(40, 32, 8.0, 1000.0, seed=81)
+    rs = [synth.uniform(n, 1000.0, seed=90 + n) for n in (900, 2500, 6000)]
+    cd = engine.upload(d)
+    crs = [engine.upload(r) for r in rs]
+    try:
+        res = {}
+        for kern in (da.PCF_PAIR_GRAM, da.PCF_PAIR_DIRECT):
+            jobs = [da.make_job(da.PCF_2ANI, da.PCF_XX, bins=30, dmax=120.0, cat_x=cd,
+                                pair_kernel=kern)]
+            for cr in crs:
+                jobs.append(da.make_job(da.PCF_2ANI, da.PCF_XX, bins=30, dmax=120.0, cat_x=cr,
+                                        pair_kernel=kern))
+                jobs.append(da.make_job(da.PCF_2ANI, da.PCF_XY, bins=30, dmax=120.0, cat_x=cd,
+                                        cat_y=cr, pair_kernel=kern))
+            res[kern] = engine.histogram_batch(jobs)
+        for (c1, _), (c2, _) in zip(res[da.PCF_PAIR_GRAM], res[da.PCF_PAIR_DIRECT]):
+            assert np.array_equal(c1, c2)
+        co, _ = O.pairs(d, rs[1], ani=True, bins=30, dmax=120.0)
+        assert np.array_equal(res[da.PCF_PAIR_GRAM][4][0], co)
+    finally:
+        engine.release(cd)
+        for c in crs:
+            engine.release(c)
