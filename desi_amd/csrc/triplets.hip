@@ -391,7 +391,16 @@ struct ListCap {
   static constexpr int value = ANI ? 256 : 64;  // third-point entries per wave
 };
 
-template <int ANI, int WEIGHTED, int PLAIN>
+// MF = 1 (3iso, mixed precision): phase B forms the 2-3 sides on the matrix
+// cores, 16 list entries x 16 pairs per v_mfma_f32_16x16x4f32 in point 1's
+// frame: rows = list entries (A operand (x3, y3, z3, |x3|^2), all relative to
+// point 1, written by phase A), columns = pairs (B operand (-2x2, -2y2, -2z2,
+// 1), C = |x2|^2), so d23 = |x2|^2 + |x3|^2 - 2 x2.x3.  The FP32 error of the
+// form is part of the third-point table's ambiguity margin (every coordinate
+// is within reach of point 1).  A lane reads one list component, the rows'
+// b31 and w3 four at a time, per 16 entries, instead of three broadcast reads
+// per entry.
+template <int ANI, int WEIGHTED, int PLAIN, int MF>
 __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
     const float4* __restrict__ xs, const float4* __restrict__ xo,
     const float4* __restrict__ ks_all, const float4* __restrict__ ko_all,
@@ -417,6 +426,9 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
             wave * CAP;
   int* Lm = reinterpret_cast<int*>(smem + hist_bytes + (size_t)(BLOCK / kWave) * CAP * 20) +
             wave * CAP;
+  // MF: third-point weights, beside the other list arrays.
+  float* Lw = reinterpret_cast<float*>(smem + hist_bytes + (size_t)(BLOCK / kWave) * CAP * 24) +
+              wave * CAP;
   if (!PLAIN)
     for (int t = threadIdx.x; t < a.nt; t += BLOCK) tab[t] = table[t];
   for (int t = threadIdx.x; t <= bins; t += BLOCK) sT[t] = T[t];
@@ -534,6 +546,142 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
       __builtin_amdgcn_wave_barrier();
     };
 
+    // ---- phase B on the matrix cores (MF): 64 pairs per chunk as four
+    // column blocks of 16, the list 16 entries per step, blocks two at a time.
+    auto run_list_mf = [&]() {
+      typedef float f4v __attribute__((ext_vector_type(4)));
+      constexpr int HB = WEIGHTED ? 1 : 2;  // column blocks per pass (registers)
+      __builtin_amdgcn_wave_barrier();
+      const int grp = lane >> 4, col = lane & 15;
+      const float sel_x = grp == 0 ? 1.0f : 0.0f, sel_y = grp == 1 ? 1.0f : 0.0f;
+      const float sel_z = grp == 2 ? 1.0f : 0.0f, sel_w = grp == 3 ? 1.0f : 0.0f;
+      const int npair = pend - pbeg;
+      const int nch = (npair + kWave - 1) / kWave;
+      const float* Lqf = reinterpret_cast<const float*>(Lq);
+      for (int ch = 0; ch < nch; ++ch) {
+        // Column 16b + col of the chunk is pair (16b + col) * nch + ch: pairs are
+        // sorted by b12, so spreading them keeps the deposits of a wave apart.
+        float bop[4], cin[4], wj[4];
+        unsigned rb[4];
+        unsigned npact = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int pi = (16 * b + col) * nch + ch;
+          const bool act = pi < npair;
+          PairRec r;
+          r.xj = r.yj = r.zj = 0.f;
+          r.wj = 0.f;
+          r.meta = 0;
+          if (act) r = pairs[pbeg + pi];
+          const float x = r.xj - is.x, y = r.yj - is.y, z = r.zj - is.z;
+          bop[b] = __builtin_fmaf(sel_x, -2.0f * x, __builtin_fmaf(sel_y, -2.0f * y,
+                                                                   __builtin_fmaf(sel_z, -2.0f * z, sel_w)));
+          cin[b] = act ? __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x)) : 1e30f;
+          rb[b] = (unsigned)(((r.meta >> 8) & 0xfff) - a.lo) * (unsigned)bins2;
+          wj[b] = r.wj;
+          npact += (unsigned)__popcll(__ballot(act && grp == 0));
+        }
+        evals += (unsigned long long)npact * (unsigned long long)n;
+        for (int jb = 0; jb < n; jb += 16) {
+          // A operand: component grp of entry jb + col (|x3|^2 huge past n).
+          const int e = jb + col;
+          const float comp = Lqf[(size_t)min(e, CAP - 1) * 4 + grp];
+          const float aop = e < n ? comp : sel_w * 1e30f;
+          // Rows 4grp + v of this step: b31 and weight of entry jb + 4grp + v.
+          const int4 mrow = *reinterpret_cast<const int4*>(Lm + jb + 4 * grp);
+          const float4 wrow = *reinterpret_cast<const float4*>(Lw + jb + 4 * grp);
+          const int mr[4] = {mrow.x, mrow.y, mrow.z, mrow.w};
+          const float wr[4] = {wrow.x, wrow.y, wrow.z, wrow.w};
+          unsigned ambm = 0;  // (b, v) -> bit 4b + v: ambiguous elements of this lane
+#pragma unroll
+          for (int h = 0; h < 4; h += HB) {
+            f4v d[HB];
+#pragma unroll
+            for (int u = 0; u < HB; ++u) {
+              const f4v cv = {cin[h + u], cin[h + u], cin[h + u], cin[h + u]};
+              d[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop, bop[h + u], cv, 0, 0, 0);
+            }
+            unsigned c23[4 * HB];
+#pragma unroll
+            for (int u = 0; u < HB; ++u)
+#pragma unroll
+              for (int v = 0; v < 4; ++v) c23[4 * u + v] = tab[min(__float2uint_rz(d[u][v]), ntm1)];
+            unsigned mx = c23[0];
+#pragma unroll
+            for (int q = 1; q < 4 * HB; ++q) mx = max(mx, c23[q]);
+            if (__builtin_expect(__ballot(mx > ubins) != 0, 0)) {
+              // Ambiguous cells (the largest code) are set aside for the exact path.
+#pragma unroll
+              for (int q = 0; q < 4 * HB; ++q)
+                if (c23[q] > ubins) {
+                  ambm |= 1u << (4 * h + q);
+                  c23[q] = ubins;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < HB; ++u) {
+              const double w12 = WEIGHTED ? wi * (double)wj[h + u] : 0.0;  // exact
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                const unsigned cc = c23[4 * u + v];
+                if (cc < ubins) {
+                  const unsigned row = rb[h + u] + cc * ubins + (unsigned)(mr[v] & 0xff);
+                  atomicAdd(&hc[row], 1u);
+                  if (WEIGHTED) atomicAdd(&hw[row], __dmul_rn(w12, (double)wr[v]));
+                }
+              }
+            }
+          }
+          // The exact FP64 2-3 side of the ambiguous elements, each lane taking
+          // its lowest remaining one per round; everything is re-read by index.
+          while (__ballot(ambm != 0)) {
+            if (ambm) {
+              const int q = __builtin_ctz(ambm);
+              ambm &= ambm - 1;
+              const int b = q >> 2, v = q & 3;
+              ++nre;
+              const PairRec r = pairs[pbeg + (16 * b + col) * nch + ch];
+              const int ent = jb + 4 * grp + v;
+              const int k = Lk[ent];
+              const int meta = Lm[ent];
+              if (!(a.auto_ && k == r.j)) {
+                const int im2 = r.meta & 31;
+                const int f2 = ((im2 / 9 - 1) != 0 ? 1 : 0) | (((im2 / 3) % 3 - 1) != 0 ? 2 : 0) |
+                               ((im2 % 3 - 1) != 0 ? 4 : 0);
+                const float4 jo = xo[r.j];
+                const float4 kco = ko_all[k];
+                const int g3 = image_flags(io, kco, meta >> 16, false);
+                if (g3 >= 0) {
+                  const float a2[3] = {jo.x, jo.y, jo.z}, a3[3] = {kco.x, kco.y, kco.z};
+                  double t23[3];
+                  for (int qq = 0; qq < 3; ++qq) {
+                    const double dd = __dsub_rn((double)a3[qq], (double)a2[qq]);
+                    t23[qq] = a.bpc ? proj_term(dd, a.L, ((f2 >> qq) & 1) != ((g3 >> qq) & 1)) : dd;
+                  }
+                  const double s23 = __dadd_rn(
+                      __dadd_rn(__dmul_rn(t23[0], t23[0]), __dmul_rn(t23[1], t23[1])),
+                      __dmul_rn(t23[2], t23[2]));
+                  if (!(a.auto_ && !(s23 > 0.0))) {
+                    const int e23 = class_lds(s23, sT, bins, a.R, 0);
+                    if (e23 >= 0) {
+                      const unsigned row =
+                          (unsigned)(((r.meta >> 8) & 0xfff) - a.lo) * (unsigned)bins2 +
+                          (unsigned)e23 * ubins + (unsigned)(meta & 0xff);
+                      atomicAdd(&hc[row], 1u);
+                      if (WEIGHTED)
+                        atomicAdd(&hw[row], __dmul_rn(wi * (double)r.wj, (double)Lw[ent]));
+                    }
+                  }
+                }
+              }
+            }
+          }
+        }
+      }
+      n = 0;
+      __builtin_amdgcn_wave_barrier();
+    };
+
     // ---- phase A: point 1's third points, one per lane
     for_each_segment(io, a, k_cell_start, [&](int k0, int k1, int sx, int sy, int sz) {
       const float sxL = (float)sx * a.Ls, syL = (float)sy * a.Ls, szL = (float)sz * a.Ls;
@@ -579,18 +727,29 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
         const unsigned long long m = __ballot(v);
         const int cnt = __popcll(m);
         if (cnt == 0) continue;
-        if (n + cnt > CAP) run_list();
+        if (n + cnt > CAP) {
+          if constexpr (MF) run_list_mf();
+          else run_list();
+        }
         const int pos = n + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                                              __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
         if (v) {
-          Lq[pos] = make_float4(kx, ky, kz, kq.w);
+          if (MF) {
+            Lq[pos] = make_float4(ax, ay, az, d31);
+            Lw[pos] = kq.w;
+          } else {
+            Lq[pos] = make_float4(kx, ky, kz, kq.w);
+          }
           Lk[pos] = k;
           Lm[pos] = b31 | (pb << 8) | (simg << 16);
         }
         n += cnt;
       }
     });
-    if (n > 0) run_list();
+    if (n > 0) {
+      if constexpr (MF) run_list_mf();
+      else run_list();
+    }
   }
   if (!ANI) {
     __syncthreads();
@@ -898,8 +1057,13 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   static thread_local std::vector<uint8_t> ptab, ktab;
   float inv_h = 0.f, inv_h2 = 0.f;
   int nt = 0, nt2 = 0;
-  if (build_table(pr, A, kMixedTableMax, &ptab, &inv_h, &nt) != 0 ||
-      build_table(kr, A, kMixedTableMax, &ktab, &inv_h2, &nt2) != 0 || inv_h != inv_h2 ||
+  // 3iso mixed: the 2-3 sides come from the matrix cores in point 1's frame,
+  // where every coordinate is within reach of point 1; both tables carry that
+  // form's error margin (they must share one scale).
+  const bool gram = !ani && !plain;
+  const double rho = gram ? (double)reach * 1.01 : -1.0;
+  if (build_table(pr, A, kMixedTableMax, &ptab, &inv_h, &nt, rho) != 0 ||
+      build_table(kr, A, kMixedTableMax, &ktab, &inv_h2, &nt2, rho) != 0 || inv_h != inv_h2 ||
       nt != nt2)
     return fail(c, PCF_ERR_UNSUPPORTED, "bin table");
   const float sigma = std::sqrt(inv_h);
@@ -1023,7 +1187,7 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
     if (!ani) {
       // LDS budget of one 1024-thread workgroup per CU: the slice of b12 rows
       // whose counts (and FP64 weights) fit beside the table.
-      const size_t lists = (size_t)(kTripBlock / kWave) * ListCap<0>::value * 24;
+      const size_t lists = (size_t)(kTripBlock / kWave) * ListCap<0>::value * (gram ? 28 : 24);
       const size_t budget = 160 * 1024 - kMixedTableMax - 8 * (kMaxTripBins + 2) - 1024 - lists;
       const size_t per_b12 = (size_t)bins * bins * (4 + (weighted ? 8 : 0));
       const int S = (int)std::max<size_t>(1, std::min<size_t>(bins, (budget - 16) / per_b12));
@@ -1035,18 +1199,18 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
             ((((size_t)nrows * 4 + 15) / 16 * 16 + (weighted ? (size_t)nrows * 8 : 0) + 15) &
              ~(size_t)15) + lists;
         HIPCHK(c, launch_zero(d_counter, sizeof(int), c->stream));
-#define PCF_TRIP_ISO(W, P)                                                                     \
+#define PCF_TRIP_ISO(W, P, M)                                                                  \
   do {                                                                                         \
-    HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_triplets<0, W, P>),          \
+    HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_triplets<0, W, P, M>),       \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));      \
-    hipLaunchKernelGGL((k_triplets<0, W, P>), dim3(c->ncu), dim3(kTripBlock), lds, c->stream,  \
-                       xs, gx->pts, ksc, gk->pts, gk->cell_start, d_pairs, d_off, d_counter,   \
-                       d_ktab, d_T, d_V, d_VW, d_stats, t);                                    \
+    hipLaunchKernelGGL((k_triplets<0, W, P, M>), dim3(c->ncu), dim3(kTripBlock), lds,          \
+                       c->stream, xs, gx->pts, ksc, gk->pts, gk->cell_start, d_pairs, d_off,   \
+                       d_counter, d_ktab, d_T, d_V, d_VW, d_stats, t);                         \
   } while (0)
-        if (weighted && plain) PCF_TRIP_ISO(1, 1);
-        else if (weighted) PCF_TRIP_ISO(1, 0);
-        else if (plain) PCF_TRIP_ISO(0, 1);
-        else PCF_TRIP_ISO(0, 0);
+        if (weighted && plain) PCF_TRIP_ISO(1, 1, 0);
+        else if (weighted) PCF_TRIP_ISO(1, 0, 1);
+        else if (plain) PCF_TRIP_ISO(0, 1, 0);
+        else PCF_TRIP_ISO(0, 0, 1);
 #undef PCF_TRIP_ISO
         HIPCHK(c, hipGetLastError());
       }
@@ -1167,7 +1331,7 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
       const int blocks = c->ncu * 4;
       const size_t lds = (size_t)(kAniTripBlock / kWave) * ListCap<1>::value * 24;
 #define PCF_TRIP_ANI(W, P)                                                                     \
-  hipLaunchKernelGGL((k_triplets<1, W, P>), dim3(blocks), dim3(kAniTripBlock), lds, c->stream, \
+  hipLaunchKernelGGL((k_triplets<1, W, P, 0>), dim3(blocks), dim3(kAniTripBlock), lds, c->stream, \
                      xs, gx->pts, ksc, gk->pts, gk->cell_start, d_pairs, d_off, d_counter,     \
                      d_ktab, d_T, d_V, d_VW, d_stats, t)
       if (weighted && plain) PCF_TRIP_ANI(1, 1);
