@@ -38,18 +38,19 @@ namespace pcf {
 
 namespace {
 
-__global__ void k_worklist_count(WorkListArgs a, int* __restrict__ counts) {
-  const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (cell >= a.g.ncell()) return;
-  counts[cell] = enumerate_cell<false>(a, (int)cell, nullptr);
+// One thread per (cell, s), s < S; entry t = cell * S + s of counts/offsets.
+__global__ void k_worklist_count(WorkListArgs a, int S, int* __restrict__ counts) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.g.ncell() * S) return;
+  counts[t] = enumerate_cell<false>(a, (int)(t / S), (int)(t % S), S, nullptr);
 }
 
-__global__ void k_worklist_fill(WorkListArgs a, const int* __restrict__ offsets,
+__global__ void k_worklist_fill(WorkListArgs a, int S, const int* __restrict__ offsets,
                                 PairItem* __restrict__ items, int64_t cap) {
-  const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (cell >= a.g.ncell()) return;
-  if ((int64_t)offsets[cell + 1] > cap) return;  // beyond the list's allocation
-  enumerate_cell<true>(a, (int)cell, items + offsets[cell]);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.g.ncell() * S) return;
+  if ((int64_t)offsets[t + 1] > cap) return;  // beyond the list's allocation
+  enumerate_cell<true>(a, (int)(t / S), (int)(t % S), S, items + offsets[t]);
 }
 
 
@@ -175,7 +176,8 @@ constexpr int kMaxTable = kMixedTableMax;
 // part of the bin table's ambiguity margin.  An item whose tile box exceeds
 // the launch's bound (k.rho2) takes the direct per-lane path instead.  The
 // instruction is exactly a chain of four fmaf in k order on gfx950 (measured
-// bit for bit), which is what the bound assumes, with a factor 2 of margin.
+// bit for bit); the bound allows four roundings of partial sums of at most
+// (2 rho + d)^2 in any order, with a factor 2 of margin.
 template <int ANI, int WEIGHTED, int CS, int BATCH, int MF>
 __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_pairs_mixed(
     const float4* __restrict__ xp, const float4* __restrict__ yp,
@@ -366,6 +368,10 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
 
     const int nb = (ni + 15) >> 4;
     const int jend = j0 + nj;
+    // Self-pair mask (items of a cell with itself): only the j steps that
+    // start before the tile's end hold pairs with j <= i.  A uniform integer,
+    // so the test per step is a scalar compare.
+    const int mask_end = masked ? i0 + ni : j0;
     auto fix2 = [&](unsigned& r, bool is_amb, int ii, int jj) {
       const unsigned long long amb = __ballot(is_amb);
       if (amb) {
@@ -433,7 +439,7 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
               for (int v = 0; v < 4; ++v)
                 r[4 * u + v] = u < nh ? tab[min(__float2uint_rz(d[u][v]), ntm1)] : 0u;
             // Element (b, v) of this lane: i = i0 + 16b + col, j = jb + 4*grp + v.
-            if (masked) {
+            if (jb < mask_end) {
 #pragma unroll
               for (int u = 0; u < H; ++u)
 #pragma unroll
@@ -538,7 +544,7 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
                   r[4 * u + v] = out_row;
                 }
               }
-            if (masked) {
+            if (jb < mask_end) {
 #pragma unroll
               for (int u = 0; u < H; ++u)
 #pragma unroll
@@ -819,15 +825,16 @@ hipError_t launch_scale_points(const float4* in, int64_t n, float sigma, float4*
 hipError_t build_pair_worklist(const WorkListArgs& a, DevBuf& counts, DevBuf& items, int* nitems,
                                hipStream_t s) {
   hipError_t e;
-  const int64_t nc = a.g.ncell();
+  const int S = worklist_split(a, a.g.ncell());
+  const int64_t nc = a.g.ncell() * S;
   const size_t ibytes = (sizeof(int) * (size_t)(2 * nc + 2) + 15) & ~(size_t)15;
   if ((e = counts.ensure(ibytes + sizeof(long long) * (size_t)scan_tiles(nc))) != hipSuccess)
     return e;
   int* cnt = counts.as<int>();
   int* off = cnt + nc + 1;
   long long* tsum = reinterpret_cast<long long*>(counts.as<char>() + ibytes);
-  const int blocks = (int)((nc + 127) / 128);
-  hipLaunchKernelGGL(k_worklist_count, dim3(blocks), dim3(128), 0, s, a, cnt);
+  const int blocks = (int)((nc + 255) / 256);
+  hipLaunchKernelGGL(k_worklist_count, dim3(blocks), dim3(256), 0, s, a, S, cnt);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = launch_scan(cnt, nc, off, tsum, s)) != hipSuccess) return e;
   int total = 0;
@@ -837,7 +844,7 @@ hipError_t build_pair_worklist(const WorkListArgs& a, DevBuf& counts, DevBuf& it
   *nitems = total;
   if (total == 0) return hipSuccess;
   if ((e = items.ensure(sizeof(PairItem) * (size_t)total)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_worklist_fill, dim3(blocks), dim3(128), 0, s, a, off,
+  hipLaunchKernelGGL(k_worklist_fill, dim3(blocks), dim3(256), 0, s, a, S, off,
                      items.as<PairItem>(), (int64_t)total);
   return hipGetLastError();
 }
@@ -851,7 +858,8 @@ int64_t worklist_bound(const WorkListArgs& a, int64_t n_tile_points) {
 hipError_t build_pair_worklist_async(const WorkListArgs& a, int64_t cap, DevBuf& counts,
                                      DevBuf& items, const int** total_dev, hipStream_t s) {
   hipError_t e;
-  const int64_t nc = a.g.ncell();
+  const int S = worklist_split(a, a.g.ncell());
+  const int64_t nc = a.g.ncell() * S;
   const size_t ibytes = (sizeof(int) * (size_t)(2 * nc + 2) + 15) & ~(size_t)15;
   if ((e = counts.ensure(ibytes + sizeof(long long) * (size_t)scan_tiles(nc))) != hipSuccess)
     return e;
@@ -860,11 +868,11 @@ hipError_t build_pair_worklist_async(const WorkListArgs& a, int64_t cap, DevBuf&
   int* cnt = counts.as<int>();
   int* off = cnt + nc + 1;
   long long* tsum = reinterpret_cast<long long*>(counts.as<char>() + ibytes);
-  const int blocks = (int)((nc + 127) / 128);
-  hipLaunchKernelGGL(k_worklist_count, dim3(blocks), dim3(128), 0, s, a, cnt);
+  const int blocks = (int)((nc + 255) / 256);
+  hipLaunchKernelGGL(k_worklist_count, dim3(blocks), dim3(256), 0, s, a, S, cnt);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = launch_scan(cnt, nc, off, tsum, s)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_worklist_fill, dim3(blocks), dim3(128), 0, s, a, off,
+  hipLaunchKernelGGL(k_worklist_fill, dim3(blocks), dim3(256), 0, s, a, S, off,
                      items.as<PairItem>(), cap);
   *total_dev = off + nc;
   return hipGetLastError();
@@ -884,7 +892,13 @@ hipError_t build_batch_worklists(const std::vector<WorkListArgs>& args, DevBuf& 
                                  DevBuf& items, std::vector<int>* job_offsets, hipStream_t s) {
   hipError_t e;
   const int nj = (int)args.size();
-  const int64_t nc = args[0].g.ncell();
+  // One split for all jobs (their counts share one scan); jobs of a batch
+  // share the neighbour reach, else every cell takes one thread.
+  bool same_k = true;
+  for (const WorkListArgs& w : args)
+    for (int a = 0; a < 3; ++a) same_k = same_k && w.K[a] == args[0].K[a];
+  const int S = same_k ? worklist_split(args[0], args[0].g.ncell() * nj) : 1;
+  const int64_t nc = args[0].g.ncell() * S;
   const int64_t m = nc * nj;
   if (m + 2 > INT32_MAX) return hipErrorInvalidValue;
   const size_t ibytes = (sizeof(int) * (size_t)(2 * m + nj + 4) + 15) & ~(size_t)15;
@@ -894,9 +908,9 @@ hipError_t build_batch_worklists(const std::vector<WorkListArgs>& args, DevBuf& 
   int* off = cnt + m + 1;
   int* jo = off + m + 1;
   long long* tsum = reinterpret_cast<long long*>(counts.as<char>() + ibytes);
-  const int blocks = (int)((nc + 127) / 128);
+  const int blocks = (int)((nc + 255) / 256);
   for (int k = 0; k < nj; ++k) {
-    hipLaunchKernelGGL(k_worklist_count, dim3(blocks), dim3(128), 0, s, args[k], cnt + k * nc);
+    hipLaunchKernelGGL(k_worklist_count, dim3(blocks), dim3(256), 0, s, args[k], S, cnt + k * nc);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   if ((e = launch_scan(cnt, m, off, tsum, s)) != hipSuccess) return e;
@@ -911,7 +925,7 @@ hipError_t build_batch_worklists(const std::vector<WorkListArgs>& args, DevBuf& 
   if (total == 0) return hipSuccess;
   if ((e = items.ensure(sizeof(PairItem) * (size_t)total)) != hipSuccess) return e;
   for (int k = 0; k < nj; ++k) {
-    hipLaunchKernelGGL(k_worklist_fill, dim3(blocks), dim3(128), 0, s, args[k], off + k * nc,
+    hipLaunchKernelGGL(k_worklist_fill, dim3(blocks), dim3(256), 0, s, args[k], S, off + k * nc,
                        items.as<PairItem>(), (int64_t)total);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }

@@ -11,9 +11,24 @@ namespace pcf {
 // Tiles are 64 consecutive points of one (cx,cy) column (they may span a few
 // z-cells), so every lane of a wave holds a point; a tile's j-range in a
 // neighbour column covers the z-cells within reach of any of its cells.  The
-// thread of cell c emits the tiles whose first point lies in c.
+// tiles whose first point lies in cell c are emitted by S threads (cell, s):
+// S = 1 takes every neighbour column, S = 2Kx+1 one x offset (dx = s - Kx),
+// S = (2Kx+1)(2Ky+1) one column (dx, dy), so the dependent index loads of a
+// cell spread over S threads instead of running as one serial chain.  The
+// list is ordered by (cell, s, tile, remaining offsets, z segment).
 template <bool FILL>
-__device__ inline int enumerate_cell(const WorkListArgs& a, int cell, PairItem* out) {
+__device__ inline int enumerate_cell(const WorkListArgs& a, int cell, int s, int S,
+                                     PairItem* out) {
+  const int ny = 2 * a.K[1] + 1;
+  int dxl = -a.K[0], dxh = a.K[0], dyl = -a.K[1], dyh = a.K[1];
+  if (S > 1) {
+    if (S == (2 * a.K[0] + 1) * ny) {
+      dxl = dxh = s / ny - a.K[0];
+      dyl = dyh = s % ny - a.K[1];
+    } else {
+      dxl = dxh = s - a.K[0];
+    }
+  }
   const int ib = a.x_cell_start[cell];
   const int ie = a.x_cell_start[cell + 1];
   if (ie <= ib) return 0;
@@ -34,7 +49,7 @@ __device__ inline int enumerate_cell(const WorkListArgs& a, int cell, PairItem* 
     // z-cell of the tile's last point.
     int zb = cz;
     while (zb + 1 < Pz && a.x_cell_start[colbase + zb + 1] <= i0 + ni - 1) ++zb;
-    for (int dx = -a.K[0]; dx <= a.K[0]; ++dx) {
+    for (int dx = dxl; dx <= dxh; ++dx) {
       int ccx = cx + dx, kx = 0;
       if (ccx < 0) {
         if (!g.bpc) continue;
@@ -46,7 +61,7 @@ __device__ inline int enumerate_cell(const WorkListArgs& a, int cell, PairItem* 
         kx = 1;
       }
       const float gx = (float)max(0, abs(dx) - 1) * g.cs[0];
-      for (int dy = -a.K[1]; dy <= a.K[1]; ++dy) {
+      for (int dy = dyl; dy <= dyh; ++dy) {
         int ccy = cy + dy, ky = 0;
         if (ccy < 0) {
           if (!g.bpc) continue;
@@ -112,5 +127,16 @@ __device__ inline int enumerate_cell(const WorkListArgs& a, int cell, PairItem* 
   return count;
 }
 
+
+// Threads per cell (S above) for a work list over `cells` cells in total (all
+// jobs of a batch): the finest split whose count array stays below 2^23
+// entries.
+inline int worklist_split(const WorkListArgs& a, int64_t cells) {
+  const int nx = 2 * a.K[0] + 1, ny = 2 * a.K[1] + 1;
+  const int64_t lim = (int64_t)1 << 23;
+  if (cells * nx * ny <= lim) return nx * ny;
+  if (cells * nx <= lim) return nx;
+  return 1;
+}
 
 }  // namespace pcf
