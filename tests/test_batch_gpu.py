@@ -34,6 +34,40 @@ def _same(batch, single, rel=1e-12):
         assert np.all(np.abs(wb - ws) <= rel * np.maximum(np.abs(ws), 1e-300)), f"job {k} weights"
 
 
+@pytest.mark.parametrize("type_", [da.PCF_2ISO, da.PCF_2ANI])
+def test_ragged_batch_with_empty_and_single_point_catalogs(engine, type_):
+    """Randoms of 0, 1, 2 and a few thousand points in one batch (jobs with no
+    work items beside large ones): every histogram equals its single run and
+    the oracle."""
+    d = synth.neyman_scott(24, 32, 6.0, 400.0, seed=71)
+    rcats = [np.zeros((0, 4), np.float32), synth.uniform(1, 400.0, seed=72),
+             np.array([[10, 10, 10, 1], [12, 11, 10, 1]], np.float32),
+             synth.uniform(3000, 400.0, seed=73)]
+    print("case ragged batch type", type_, flush=True)
+    cd = engine.upload(d)
+    crs = [engine.upload(r) for r in rcats]
+    try:
+        jobs = _jobs(type_, cd, crs, bins=12, dmax=60.0)
+        res = engine.histogram_batch(jobs)
+        _same(res, _singles(engine, jobs))
+        ani = type_ == da.PCF_2ANI
+        for k, r in enumerate(rcats):
+            c, _ = O.pairs(r, None, ani=ani, bins=12, dmax=60.0) if len(r) else (None, None)
+            if c is not None:
+                assert np.array_equal(res[1 + 2 * k][0], c)
+            else:
+                assert res[1 + 2 * k][0].sum() == 0
+            c, _ = O.pairs(d, r, ani=ani, bins=12, dmax=60.0) if len(r) else (None, None)
+            if c is not None:
+                assert np.array_equal(res[2 + 2 * k][0], c)
+            else:
+                assert res[2 + 2 * k][0].sum() == 0
+    finally:
+        engine.release(cd)
+        for c in crs:
+            engine.release(c)
+
+
 @pytest.mark.parametrize("weighted", [False, True])
 def test_2ani_many_randoms(engine, weighted):
     """The 64-random 2ani shape at small size: clustered D, uneven uniform

@@ -49,6 +49,30 @@ def _check(engine, a, b, *, ani, bpc, box, bins, dmax, cpr=0, prec=da.PCF_PREC_M
     return st
 
 
+def test_empty_and_tiny(engine):
+    """Catalogs of 0 to 4 points (one duplicated point: its zero separation is
+    excluded) as XXX and XXY, both ends empty included: every histogram equals
+    the oracle's, all-zero ones too."""
+    one = np.array([[3, 2, 2, 1.5]], np.float32)
+    three = np.array([[1, 1, 1, 1], [2, 1.5, 1, 0.5], [1.5, 3, 2, 2]], np.float32)
+    dup = np.ascontiguousarray(np.concatenate([three, three[:1]]))
+    empty = np.zeros((0, 4), np.float32)
+    cases = ((one, None), (three[:2], None), (three, None), (dup, None), (three, one),
+             (one, three), (three[:2], one), (three, empty), (empty, three))
+    nonzero = 0
+    for ani in (False, True):
+        for bpc in (False, True):
+            for a, b in cases:
+                print(f"case 3{'ani' if ani else 'iso'} bpc={bpc} n={len(a)} "
+                      f"m={None if b is None else len(b)}", flush=True)
+                c, w, _ = _run(engine, a, b, ani=ani, bpc=bpc, box=50.0, bins=5, dmax=10.0)
+                co, wo = O.triplets(a, b, ani=ani, bpc=bpc, box=50.0, bins=5, dmax=10.0)
+                assert np.array_equal(c, co)
+                assert np.all(np.abs(w - wo) <= WREL * np.abs(wo))
+                nonzero += int(co.sum() > 0)
+    assert nonzero == 16
+
+
 @pytest.mark.parametrize("bpc", [False, True])
 @pytest.mark.parametrize("cross", [False, True])
 def test_3iso_sample(engine, bpc, cross):
