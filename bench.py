@@ -241,7 +241,9 @@ def serving_leg(requests=400, rate=50.0, lanes=4, max_batch=8):
     pool of 24 (sizes log-uniform in [1024, 1048576], uniform or Neyman-Scott,
     box 250 or 1000); rmax U[40, 120] for pairs, U[20, 40] and N <= 16384 for
     triplets (a 3iso request at N = 1e6 and rmax 120 in box 250 would be
-    ~1e17 triplets).  Latency = submit -> result ready, per request."""
+    ~1e17 triplets).  Latency = submit -> result ready, per request.  With
+    rmax continuous, queued requests rarely share a setting, so launches are
+    seldom shared (the serving tests cover batched launches)."""
     rng = np.random.default_rng(31)
     pool = []  # (box, n, xyzw)
     for k in range(24):
@@ -372,6 +374,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=400)
+    ap.add_argument("--serving-max-batch", type=int, default=8,
+                    help="most queued pair requests one serving launch may take")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (default launch configuration)")
     ap.add_argument("--batch-steps", type=int, default=3)
@@ -431,7 +435,8 @@ def main():
         ana = None if args.no_analytic else analytic_leg(e)
         bat = None if args.no_batch else batch_leg(e, args.batch_steps, args.no_tune)
 
-    serving = None if args.no_serving else serving_leg(args.serving_requests)
+    serving = None if args.no_serving else serving_leg(args.serving_requests,
+                                                       max_batch=args.serving_max_batch)
     achieved = evals * FLOPS_PER_PAIR / (kms * 1e-3)
     # The dominant kernel: the Gram form forms d2 with four chained FMAs on the
     # matrix cores (FP32 MFMA peak 157.3 TFLOP/s, the same figure as the packed
