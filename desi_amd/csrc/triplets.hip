@@ -857,26 +857,11 @@ __global__ __launch_bounds__(kTripBlock) void k_trip_ani_lds(
       if (qi < nq) kr = Q[qbeg + qi];
       return kr;
     };
-    // Point 1's pairs in the slice, 64 at a time in VGPRs (one per lane; all
-    // of them at once for a typical slice), broadcast by readlane.
+    // Point 1's pairs in the slice, up to 64 per pass, read by uniform index
+    // (scalar loads: no vector register or lane broadcast per pair).
     for (int pc = pbeg; pc < pend; pc += kWave) {
-      PairRec mp;
-      mp.xj = mp.yj = mp.zj = 1e18f;
-      mp.wj = 0.f;
-      mp.j = 0;
-      mp.meta = 0;
-      if (pc + lane < pend) mp = P[pc + lane];
       const int np = min(kWave, pend - pc);
-      auto bcast = [&](int u) {
-        PairRec r;
-        r.xj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mp.xj), u));
-        r.yj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mp.yj), u));
-        r.zj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mp.zj), u));
-        r.wj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mp.wj), u));
-        r.j = __builtin_amdgcn_readlane(mp.j, u);
-        r.meta = __builtin_amdgcn_readlane(mp.meta, u);
-        return r;
-      };
+      auto bcast = [&](int u) -> PairRec { return P[pc + u]; };
       PairRec kr_next = load_q(0);
       for (int ch = 0; ch < nch; ++ch) {
         const PairRec kr = kr_next;
