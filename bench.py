@@ -370,6 +370,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--triplet-steps", type=int, default=1)
     ap.add_argument("--no-triplets", action="store_true")
+    ap.add_argument("--no-3iso", action="store_true", help="skip the 3iso triplet leg only")
     ap.add_argument("--no-analytic", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -430,7 +431,7 @@ def main():
         ms_step64 = (t3 - t2) * 1e3 / args.fp64_steps
         if p64 // args.fp64_steps != pairs // args.steps:
             raise SystemExit("plain FP64 path and mixed path disagree on the pair count")
-        trip = None if args.no_triplets else triplet_leg(e, args.triplet_steps)
+        trip = None if args.no_triplets or args.no_3iso else triplet_leg(e, args.triplet_steps)
         trip_ani = None if args.no_triplets else triplet_ani_leg(e)
         ana = None if args.no_analytic else analytic_leg(e)
         bat = None if args.no_batch else batch_leg(e, args.batch_steps, args.no_tune)
