@@ -19,10 +19,13 @@
 //      list (the 2-3 side).  Both sides go through the FP32 d2 -> bin table;
 //      lanes in an ambiguous cell (and 3ani's cosine bins) take the exact FP64
 //      path;
-//   3. 3iso: pairs are unordered (i < j) and the third point is any point; the
-//      ordered histogram is H[a][b][c] = V[a][b][c] + V[a][c][b] (exchanging
-//      points 1 and 2 swaps the d23 and d31 sides).  3ani: the cosine of the
-//      1-2 direction is not symmetric, so pairs are emitted in both orders.
+//   3. 3iso: pairs are unordered (i < j).  XXY: the third point is any point
+//      of Y and the ordered histogram is H[a][b][c] = V[a][b][c] + V[a][c][b]
+//      (exchanging points 1 and 2 swaps the d23 and d31 sides).  XXX: the
+//      third point has k > i as well, so point 1 is the lowest of the three,
+//      and H is V summed over its cyclic rotations (k_sym_cyc).  3ani: the
+//      cosine of the 1-2 direction is not symmetric, so pairs are emitted in
+//      both orders and every third point is taken.
 //
 // 3iso accumulates in LDS (u32 counts, FP64 weights) over slices of b12 so the
 // slice fits; 3ani's 5-D histogram (bins^5) goes to global atomics.
@@ -688,7 +691,11 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
       const int simg = (sx + 1) * 9 + (sy + 1) * 3 + (sz + 1);
       for (int kb = k0; kb < k1; kb += kWave) {
         const int k = kb + lane;
-        bool v = k < k1 && !(a.auto_ && k == i);
+        // XXX 3iso: only third points above point 1 (point 1 is then the
+        // lowest of the three and each unordered triplet is met twice, not
+        // three times; k_sym_cyc restores the six orders).  3ani and XXY
+        // take every third point but point 1 itself.
+        bool v = k < k1 && !(a.auto_ && (ANI ? k == i : k <= i));
         float4 kq = make_float4(1e18f, 1e18f, 1e18f, 0.f);
         if (v) kq = ks_all[k];
         const float kx = kq.x + sxL, ky = kq.y + syL, kz = kq.z + szL;
@@ -965,7 +972,25 @@ __global__ __launch_bounds__(kTripBlock) void k_trip_ani_lds(
   if (lane == 0 && nre) atomicAdd(&stats[1], nre);
 }
 
-// H[a][b][c] = V[a][b][c] + V[a][c][b]
+// XXX 3iso: point 1 is the lowest index of its triplet (pairs i < j, third
+// points k > i), so an unordered triplet {p < q < r} with sides A = b(pq),
+// B = b(qr), C = b(rp) lands in V at (A, B, C) (pair (p, q), third r) and at
+// (C, B, A) (pair (p, r), third q).  The six ordered triplets sit at those two
+// and their cyclic rotations (y, z, x) and (z, x, y):
+// H[a][b][c] = V[a][b][c] + V[c][a][b] + V[b][c][a].
+__global__ void k_sym_cyc(const unsigned long long* __restrict__ V, const double* __restrict__ W,
+                          int bins, unsigned long long* __restrict__ H, double* __restrict__ HW) {
+  const int64_t n = (int64_t)bins * bins * bins;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int a = (int)(t / (bins * bins)), b = (int)((t / bins) % bins), c = (int)(t % bins);
+  const int64_t u = ((int64_t)c * bins + a) * bins + b;
+  const int64_t w = ((int64_t)b * bins + c) * bins + a;
+  H[t] = V[t] + V[u] + V[w];
+  HW[t] = (W[t] + W[u]) + W[w];
+}
+
+// XXY 3iso: H[a][b][c] = V[a][b][c] + V[a][c][b]
 __global__ void k_sym_bc(const unsigned long long* __restrict__ V, const double* __restrict__ W,
                          int bins, unsigned long long* __restrict__ H, double* __restrict__ HW) {
   const int64_t n = (int64_t)bins * bins * bins;
@@ -1331,8 +1356,12 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   const unsigned long long* outH = d_V;
   const double* outW = d_VW;
   if (!ani) {
-    hipLaunchKernelGGL(k_sym_bc, dim3((unsigned)((nflat + 255) / 256)), dim3(256), 0, c->stream,
-                       d_V, d_VW, bins, d_H, d_HW);
+    if (auto_)
+      hipLaunchKernelGGL(k_sym_cyc, dim3((unsigned)((nflat + 255) / 256)), dim3(256), 0,
+                         c->stream, d_V, d_VW, bins, d_H, d_HW);
+    else
+      hipLaunchKernelGGL(k_sym_bc, dim3((unsigned)((nflat + 255) / 256)), dim3(256), 0,
+                         c->stream, d_V, d_VW, bins, d_H, d_HW);
     HIPCHK(c, hipGetLastError());
     outH = d_H;
     outW = d_HW;
