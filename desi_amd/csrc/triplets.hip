@@ -22,8 +22,11 @@
 //   3. 3iso: pairs are unordered (i < j).  XXY: the third point is any point
 //      of Y and the ordered histogram is H[a][b][c] = V[a][b][c] + V[a][c][b]
 //      (exchanging points 1 and 2 swaps the d23 and d31 sides).  XXX: the
-//      third point has k > i as well, so point 1 is the lowest of the three,
-//      and H is V summed over its cyclic rotations (k_sym_cyc).  3ani: the
+//      third point has k > i as well, so point 1 is the lowest of the three
+//      and an unordered triplet is met twice; XXX launches take one b12 row
+//      each and only third points with b31 >= b12, so just the meeting(s)
+//      with b12 <= b31 are deposited, and k_sym_cyc rebuilds the six orders.
+//      3ani: the
 //      cosine of the 1-2 direction is not symmetric, so pairs are emitted in
 //      both orders and every third point is taken.
 //
@@ -731,6 +734,10 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
             }
           }
         }
+        // XXX 3iso: slices are one b12 row wide (lo == b12 of every pair of
+        // the launch), so dropping third points with b31 < lo keeps exactly
+        // the meetings with b12 <= b31.
+        if (!ANI && a.auto_ && v && b31 < a.lo) v = false;
         const unsigned long long m = __ballot(v);
         const int cnt = __popcll(m);
         if (cnt == 0) continue;
@@ -973,21 +980,36 @@ __global__ __launch_bounds__(kTripBlock) void k_trip_ani_lds(
 }
 
 // XXX 3iso: point 1 is the lowest index of its triplet (pairs i < j, third
-// points k > i), so an unordered triplet {p < q < r} with sides A = b(pq),
-// B = b(qr), C = b(rp) lands in V at (A, B, C) (pair (p, q), third r) and at
-// (C, B, A) (pair (p, r), third q).  The six ordered triplets sit at those two
-// and their cyclic rotations (y, z, x) and (z, x, y):
-// H[a][b][c] = V[a][b][c] + V[c][a][b] + V[b][c][a].
+// points k > i).  An unordered triplet {p < q < r} with sides A = b(pq),
+// B = b(qr), C = b(rp) is met as (A, B, C) (pair (p, q), third r) and as
+// (C, B, A) (pair (p, r), third q), and the kernel deposits the one(s) whose
+// first index is at most its last (b12 <= b31: one-row b12 slices and third
+// points with b31 >= b12): once when A != C, twice (the same cell) when A == C.  So V2[x][y][z] = V[x][y][z] + V[z][y][x] (x != z),
+// V[x][y][x] (x == z) holds both meetings, and the six ordered triplets are
+// V2's cells and their cyclic rotations:
+// H[a][b][c] = V2[a][b][c] + V2[c][a][b] + V2[b][c][a].
 __global__ void k_sym_cyc(const unsigned long long* __restrict__ V, const double* __restrict__ W,
                           int bins, unsigned long long* __restrict__ H, double* __restrict__ HW) {
   const int64_t n = (int64_t)bins * bins * bins;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int a = (int)(t / (bins * bins)), b = (int)((t / bins) % bins), c = (int)(t % bins);
-  const int64_t u = ((int64_t)c * bins + a) * bins + b;
-  const int64_t w = ((int64_t)b * bins + c) * bins + a;
-  H[t] = V[t] + V[u] + V[w];
-  HW[t] = (W[t] + W[u]) + W[w];
+  unsigned long long h = 0;
+  double hw = 0.0;
+  const int cyc[3][3] = {{a, b, c}, {c, a, b}, {b, c, a}};
+  for (int r = 0; r < 3; ++r) {
+    const int x = cyc[r][0], y = cyc[r][1], z = cyc[r][2];
+    const int64_t u = ((int64_t)x * bins + y) * bins + z;
+    h += V[u];
+    hw += W[u];
+    if (x != z) {
+      const int64_t v = ((int64_t)z * bins + y) * bins + x;
+      h += V[v];
+      hw += W[v];
+    }
+  }
+  H[t] = h;
+  HW[t] = hw;
 }
 
 // XXY 3iso: H[a][b][c] = V[a][b][c] + V[a][c][b]
@@ -1200,7 +1222,10 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
       const size_t lists = (size_t)(kTripBlock / kWave) * ListCap<0>::value * (gram ? 28 : 24);
       const size_t budget = 160 * 1024 - kMixedTableMax - 8 * (kMaxTripBins + 2) - 1024 - lists;
       const size_t per_b12 = (size_t)bins * bins * (4 + (weighted ? 8 : 0));
-      const int S = (int)std::max<size_t>(1, std::min<size_t>(bins, (budget - 16) / per_b12));
+      // XXX takes one b12 row per launch: its third points are then cut to
+      // b31 >= b12 (each unordered triplet deposited once, k_sym_cyc).
+      const int S = auto_ ? 1
+                          : (int)std::max<size_t>(1, std::min<size_t>(bins, (budget - 16) / per_b12));
       for (int lo = 0; lo < bins; lo += S) {
         t.lo = lo;
         t.hi = std::min(bins, lo + S);
