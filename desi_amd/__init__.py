@@ -23,6 +23,9 @@ PCF_XX, PCF_XY, PCF_XXX, PCF_XXY = 0, 1, 2, 3
 PCF_PREC_MIXED, PCF_PREC_FP64 = 0, 1
 PCF_PAIR_AUTO, PCF_PAIR_DIRECT, PCF_PAIR_GRAM = 0, 1, 2
 PCF_EST_NORMALIZE = 1
+# Status codes (include/pcf_abi.h).
+PCF_OK, PCF_ERR_ARG, PCF_ERR_HIP, PCF_ERR_NOMEM = 0, -1, -2, -3
+PCF_ERR_RANGE, PCF_ERR_UNSUPPORTED, PCF_ERR_BUSY = -4, -5, -6
 
 ERRORS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "out of device memory",
           -4: "coordinates out of range", -5: "unsupported", -6: "busy"}

@@ -1222,6 +1222,10 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
       const size_t lists = (size_t)(kTripBlock / kWave) * ListCap<0>::value * (gram ? 28 : 24);
       const size_t budget = 160 * 1024 - kMixedTableMax - 8 * (kMaxTripBins + 2) - 1024 - lists;
       const size_t per_b12 = (size_t)bins * bins * (4 + (weighted ? 8 : 0));
+      if (per_b12 + 16 > budget)
+        return fail(c, PCF_ERR_UNSUPPORTED,
+                    "3iso: one b12 row of the histogram does not fit the LDS budget "
+                    "(about 90 bins weighted, 160 unweighted)");
       // XXX takes one b12 row per launch: its third points are then cut to
       // b31 >= b12 (each unordered triplet deposited once, k_sym_cyc).
       const int S = auto_ ? 1

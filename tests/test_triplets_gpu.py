@@ -73,6 +73,21 @@ def test_empty_and_tiny(engine):
     assert nonzero == 16
 
 
+def test_3iso_too_many_bins_is_unsupported(engine):
+    """A weighted 3iso histogram whose single b12 row exceeds the LDS budget
+    (120 x 120 bins x 12 bytes) is refused with PCF_ERR_UNSUPPORTED before any
+    launch; the same bins unweighted fit and match the oracle."""
+    a = synth.uniform(400, 50.0, seed=5, weights="uniform")
+    with pytest.raises(da.PcfError) as ei:
+        _run(engine, a, None, ani=False, bpc=False, box=50.0, bins=120, dmax=12.0)
+    assert ei.value.rc == da.PCF_ERR_UNSUPPORTED
+    a1 = a.copy()
+    a1[:, 3] = 1.0
+    c, _, _ = _run(engine, a1, None, ani=False, bpc=False, box=50.0, bins=120, dmax=12.0)
+    co, _ = O.triplets(a1, None, bins=120, dmax=12.0)
+    assert np.array_equal(c, co) and co.sum() > 0
+
+
 @pytest.mark.parametrize("bpc", [False, True])
 @pytest.mark.parametrize("cross", [False, True])
 def test_3iso_sample(engine, bpc, cross):
