@@ -567,11 +567,14 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
                          jb + 4 * grp + v);
                 }
             }
+            // Most 2ani candidates are out of range, and their deposits would
+            // all land on one row per LDS copy (8 lanes per address): skip them.
 #pragma unroll
             for (int u = 0; u < H; ++u)
               if (u < nh) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) atomicAdd(hc_lane + (r[4 * u + v] << CS), 1u);
+                for (int v = 0; v < 4; ++v)
+                  if (r[4 * u + v] != out_row) atomicAdd(hc_lane + (r[4 * u + v] << CS), 1u);
               }
           }
         }
