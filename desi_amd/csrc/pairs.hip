@@ -536,11 +536,11 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
                 if (u < nh) {
                   const unsigned cz = tab[min(__float2uint_rz(dz[u][v]), ntm1)];
                   const unsigned co = tab[min(__float2uint_rz(dp[u][v]), ntm1)];
-                  const unsigned m = max(cz, co);
-                  mc[4 * u + v] = m;
-                  r[4 * u + v] = m < bins ? __umul24(cz, bins) + co : out_row;
+                  // The row; it counts only when both codes are bins (mc < bins).
+                  mc[4 * u + v] = max(cz, co);
+                  r[4 * u + v] = __umul24(cz, bins) + co;
                 } else {
-                  mc[4 * u + v] = 0u;
+                  mc[4 * u + v] = kAniOut8;
                   r[4 * u + v] = out_row;
                 }
               }
@@ -549,10 +549,7 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
               for (int u = 0; u < H; ++u)
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
-                  if (jb + 4 * grp + v <= i0 + 16 * (h + u) + col) {
-                    r[4 * u + v] = out_row;
-                    mc[4 * u + v] = 0u;
-                  }
+                  if (jb + 4 * grp + v <= i0 + 16 * (h + u) + col) mc[4 * u + v] = kAniOut8;
             }
             unsigned m = mc[0];
 #pragma unroll
@@ -568,13 +565,15 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
                 }
             }
             // Most 2ani candidates are out of range, and their deposits would
-            // all land on one row per LDS copy (8 lanes per address): skip them.
+            // all land on one row per LDS copy (8 lanes per address): only
+            // pairs with both codes bins deposit (ambiguous ones, code 255,
+            // go through the queue; excluded self pairs read as out).
 #pragma unroll
             for (int u = 0; u < H; ++u)
               if (u < nh) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
-                  if (r[4 * u + v] != out_row) atomicAdd(hc_lane + (r[4 * u + v] << CS), 1u);
+                  if (mc[4 * u + v] < bins) atomicAdd(hc_lane + (r[4 * u + v] << CS), 1u);
               }
           }
         }
