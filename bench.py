@@ -371,6 +371,7 @@ def main():
     ap.add_argument("--triplet-steps", type=int, default=1)
     ap.add_argument("--no-triplets", action="store_true")
     ap.add_argument("--no-3iso", action="store_true", help="skip the 3iso triplet leg only")
+    ap.add_argument("--no-3ani", action="store_true", help="skip the 3ani triplet leg only")
     ap.add_argument("--no-analytic", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -432,7 +433,7 @@ def main():
         if p64 // args.fp64_steps != pairs // args.steps:
             raise SystemExit("plain FP64 path and mixed path disagree on the pair count")
         trip = None if args.no_triplets or args.no_3iso else triplet_leg(e, args.triplet_steps)
-        trip_ani = None if args.no_triplets else triplet_ani_leg(e)
+        trip_ani = None if args.no_triplets or args.no_3ani else triplet_ani_leg(e)
         ana = None if args.no_analytic else analytic_leg(e)
         bat = None if args.no_batch else batch_leg(e, args.batch_steps, args.no_tune)
 
