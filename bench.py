@@ -356,9 +356,26 @@ def cpu_baseline(d):
     t = time.perf_counter()
     c, _ = O.pairs(sample, None, bins=BINS, dmax=RMAX, threads=cores)
     dt = time.perf_counter() - t
+    # The reference CPU programs run at a hard-wired 2 threads; the same sample
+    # at 2 threads is reported beside the full-core figure.
+    t = time.perf_counter()
+    c2, _ = O.pairs(sample, None, bins=BINS, dmax=RMAX, threads=2)
+    dt2 = time.perf_counter() - t
+    if not np.array_equal(c, c2):
+        raise SystemExit("oracle results differ between thread counts")
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": float(c.sum()) / dt, "unit": "pairs/s", "cores": cores, "kind": "port",
             "sample": f"oracle FP64 DD of the {len(sample)} data points in [0,500)^3 "
-                      f"(rmax {RMAX}, {BINS} bins), {dt:.2f} s"}
+                      f"(rmax {RMAX}, {BINS} bins), {dt:.2f} s",
+            "value_2_threads": float(c2.sum()) / dt2, "cpu_model": model}
 
 
 def main():
@@ -470,6 +487,7 @@ def main():
                      "traffic": profiled_traffic(prof, "k_pairs_mixed NS")},
         "pair_kernel_ms_per_step": kms / args.steps,
         "evaluations_per_step": evals // args.steps,
+        "candidates_per_useful_pair": evals / max(pairs, 1),
         "rechecks_per_step": rech // args.steps,
         "fp64_plain_ms_per_step": ms_step64,
         "fp64_plain_kernel_ms_per_step": k64 / args.fp64_steps,
