@@ -211,7 +211,8 @@ constexpr int kGramZsub = 4;  // Gram pair path: thin z cells per lateral cell e
 
 // Launch shape of the pair kernels: 2^cshift LDS copies of each histogram row
 // (lanes spread their atomics over the copies) and the resident workgroups per
-// CU of the persistent grid.  Defaults: as many copies as fit 48 KB, as many
+// CU of the persistent grid.  Defaults: as many copies as fit 48 KB (16 KB
+// for unweighted 2ani), as many
 // workgroups as the 160 KB of LDS per CU hold.  The overrides (the job's
 // lds_copies_log2 = log2 + 1 and blocks_per_cu; 0 = default) are the
 // autotuner's knobs; no choice changes a result.
@@ -224,8 +225,11 @@ inline PairShape pair_shape(int nflat, int bins, bool weighted, bool plain, int 
                             int per_cu_override) {
   const int rows = nflat + 2;
   const int per = 4 + (weighted ? 8 : 0);
+  // Unweighted 2ani (nflat = bins^2) deposits only pairs in range, so a few
+  // copies of its 900-odd rows suffice and leave LDS for more workgroups.
+  const size_t budget = (nflat != bins && !weighted) ? 16 * 1024 : 48 * 1024;
   int copies = 64;
-  while (copies > 1 && (size_t)rows * copies * per > 48 * 1024) copies >>= 1;
+  while (copies > 1 && (size_t)rows * copies * per > budget) copies >>= 1;
   int cs = 0;
   while ((1 << cs) < copies) ++cs;
   if (lds_log2p1 > 0) {
