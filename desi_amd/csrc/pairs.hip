@@ -525,9 +525,9 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
                 dz[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(aopz, bopz[h + u], zero4, 0, 0, 0);
               }
             }
-            // Per element: line-of-sight and perpendicular codes (bin, out,
-            // or ambiguous = 255, the largest); the row is cz * bins + co when
-            // both are bins.
+            // Per element: line-of-sight and perpendicular codes (bin,
+            // ambiguous = 254 or out = 255); the row is cz * bins + co when
+            // both are bins, and their max is out when either is out.
             unsigned r[4 * H], mc[4 * H];
 #pragma unroll
             for (int u = 0; u < H; ++u)
@@ -551,10 +551,10 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
                 for (int v = 0; v < 4; ++v)
                   if (jb + 4 * grp + v <= i0 + 16 * (h + u) + col) mc[4 * u + v] = kAniOut8;
             }
-            unsigned m = mc[0];
+            bool any_amb = false;
 #pragma unroll
-            for (int e = 1; e < 4 * H; ++e) m = max(m, mc[e]);
-            if (__builtin_expect(__ballot(m == kAniAmb8) != 0, 0)) {
+            for (int e = 0; e < 4 * H; ++e) any_amb = any_amb || mc[e] == kAniAmb8;
+            if (__builtin_expect(__ballot(any_amb) != 0, 0)) {
 #pragma unroll
               for (int u = 0; u < H; ++u)
                 if (u < nh) {
@@ -566,8 +566,8 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
             }
             // Most 2ani candidates are out of range, and their deposits would
             // all land on one row per LDS copy (8 lanes per address): only
-            // pairs with both codes bins deposit (ambiguous ones, code 255,
-            // go through the queue; excluded self pairs read as out).
+            // pairs with both codes bins deposit (ambiguous ones go through
+            // the queue; excluded self pairs read as out).
 #pragma unroll
             for (int u = 0; u < H; ++u)
               if (u < nh) {

@@ -62,10 +62,12 @@ struct BinRules {
 };
 
 // Table codes of the FP32 bulk path for 2ani (per component; iso tables hold
-// the histogram row directly).  Ambiguous is the largest code, so the max of
-// a pair's two codes (or of many) shows whether the exact path is needed.
-constexpr uint8_t kAniAmb8 = 255;
-constexpr uint8_t kAniOut8 = 254;
+// the histogram row directly).  Out is the largest code and ambiguous the
+// next, so the max of a pair's two codes is out when either component is out
+// (the pair needs no exact check even if the other is ambiguous), ambiguous
+// when one is and neither is out, else the larger bin.
+constexpr uint8_t kAniOut8 = 255;
+constexpr uint8_t kAniAmb8 = 254;
 
 struct DevGrid {
   GridGeom g;
