@@ -789,6 +789,11 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
 // keyed by point 1, b12, t) stream through scalar loads, one per step, against
 // all lanes.  The third-point list of XXX is the pair list itself (point 3 is
 // any point in range of point 1); XXY's comes from k_emit_cross.
+// Largest bin table of the LDS 3ani kernel: the rest of its table array holds
+// the per-wave recheck queues.
+constexpr int kAniTableMax = kMixedTableMax - 8 * kTripBlock;
+static_assert(kAniTableMax % 16 == 0 && kAniTableMax > 0, "3ani table and queues");
+
 struct AniSlice {
   int b12, tlo, tn;
   int qstride;  // key stride of the third-point list (bins^2 for XXX, 1 for XXY)
@@ -823,8 +828,11 @@ __global__ __launch_bounds__(kTripBlock) void k_trip_ani_lds(
   const int si = wg_slice[blockIdx.x];
   const AniSlice sl = slices[si];
   int* const counter = counters + si;
-  __shared__ uint8_t tab[kMixedTableMax];
+  // The bin table (a.nt <= kAniTableMax bytes) and behind it the waves'
+  // recheck queues (64 entries of 8 bytes each).
+  __shared__ __align__(16) uint8_t tab[kMixedTableMax];
   __shared__ double sT[kMaxTripBins + 2];
+  uint2* const rq = reinterpret_cast<uint2*>(tab + kAniTableMax);
   const int B = a.bins;
   const int nrows = B * B * sl.tn * B;  // [b31][b23][t - tlo][p]
   uint32_t* hc = reinterpret_cast<uint32_t*>(smem);
@@ -842,6 +850,55 @@ __global__ __launch_bounds__(kTripBlock) void k_trip_ani_lds(
   const unsigned ubins = (unsigned)B;
   const int B2 = B * B;
   unsigned long long evals = 0, nre = 0;
+  // The exact FP64 bin of the 2-3 side (point 2 of pair r, third point at
+  // kco with image flags g3), or -1 out of range.
+  auto exact23 = [&](const PairRec& r, const float4 jo, const float4 kco, int g3) -> int {
+    const int im2 = r.meta & 31;
+    const int f2 = ((im2 / 9 - 1) != 0 ? 1 : 0) | (((im2 / 3) % 3 - 1) != 0 ? 2 : 0) |
+                   ((im2 % 3 - 1) != 0 ? 4 : 0);
+    const float a2[3] = {jo.x, jo.y, jo.z}, a3[3] = {kco.x, kco.y, kco.z};
+    double t23[3];
+    for (int qd = 0; qd < 3; ++qd) {
+      const double d = __dsub_rn((double)a3[qd], (double)a2[qd]);
+      t23[qd] = a.bpc ? proj_term(d, a.L, ((f2 >> qd) & 1) != ((g3 >> qd) & 1)) : d;
+    }
+    const double s23 = __dadd_rn(__dadd_rn(__dmul_rn(t23[0], t23[0]), __dmul_rn(t23[1], t23[1])),
+                                 __dmul_rn(t23[2], t23[2]));
+    if (a.auto_ && !(s23 > 0.0)) return -1;
+    return class_lds(s23, sT, B, a.R, 0);
+  };
+  // Mixed path: a wave's ambiguous (pair, third point) elements wait in its
+  // LDS queue (pair index, third-point index) and are evaluated 64 at a time,
+  // one per lane, instead of stalling the whole wave on each one's exact
+  // FP64 side and its two coordinate loads.
+  uint2* const wq = rq + (PLAIN ? 0 : (threadIdx.x & ~(kWave - 1)));
+  int qn = 0;
+  double wi = 0.0;
+  auto drain = [&]() {
+    __builtin_amdgcn_wave_barrier();
+    if (lane < qn) {
+      const uint2 e = wq[lane];
+      const PairRec r = P[e.x];
+      const PairRec kq = Q[e.y];
+      if (!(a.auto_ && kq.j == r.j)) {
+        const int img3 = kq.meta & 31;
+        const int g3 = ((img3 / 9 - 1) != 0 ? 1 : 0) | (((img3 / 3) % 3 - 1) != 0 ? 2 : 0) |
+                       ((img3 % 3 - 1) != 0 ? 4 : 0);
+        const int e23 = exact23(r, xo[r.j], ko_all[kq.j], g3);
+        if (e23 >= 0) {
+          const int b31 = (kq.meta >> 8) & 0xfff, p31 = (kq.meta >> 20) & 0xfff;
+          const int t12 = (r.meta >> 20) & 0xfff;
+          const unsigned row =
+              (((unsigned)b31 * ubins + (unsigned)e23) * (unsigned)sl.tn + (unsigned)(t12 - sl.tlo)) *
+                  ubins + (unsigned)p31;
+          atomicAdd(&hc[row], 1u);
+          if (WEIGHTED) atomicAdd(&hw[row], __dmul_rn(wi * (double)r.wj, (double)kq.wj));
+        }
+      }
+    }
+    qn = 0;
+    __builtin_amdgcn_wave_barrier();
+  };
   for (;;) {
     int task = 0;
     if (lane == 0) task = atomicAdd(counter, 1);
@@ -854,8 +911,7 @@ __global__ __launch_bounds__(kTripBlock) void k_trip_ani_lds(
     const int qbeg = __builtin_amdgcn_readfirstlane(qoff[i * sl.qstride]);
     const int qend = __builtin_amdgcn_readfirstlane(qoff[(i + 1) * sl.qstride]);
     if (qend <= qbeg) continue;
-    const float4 io = xo[i];
-    const double wi = (double)io.w;
+    wi = (double)xo[i].w;
     // Lane l of chunk c takes entry l * nch + c: the list is sorted by (b31,
     // p), so consecutive entries share histogram rows; spreading them over the
     // lanes keeps a wave's LDS atomics off one address.
@@ -893,36 +949,33 @@ __global__ __launch_bounds__(kTripBlock) void k_trip_ani_lds(
         bool kco_loaded = false;
         const unsigned rb = ((unsigned)b31 * ubins) * (unsigned)sl.tn * ubins + (unsigned)p31;
         // The 2-3 side of pair r against every lane's third point: the table
-        // class, or the exact FP64 bin for lanes in an ambiguous cell.
-        auto classify = [&](const PairRec& r, unsigned c23) -> unsigned {
+        // class; an ambiguous cell is queued (mixed path) or evaluated exactly
+        // at once (plain path, where every cell is ambiguous).
+        auto classify = [&](const PairRec& r, int pidx, unsigned c23) -> unsigned {
           const bool amb = active && c23 > ubins;
-          if (__ballot(amb)) {
-            if (!PLAIN) nre += __popcll(__ballot(amb));
-            if (amb) {
-              c23 = ubins;
-              if (!(a.auto_ && k == r.j)) {
-                if (!kco_loaded) {
-                  kco = ko_all[k];
-                  kco_loaded = true;
-                }
-                const float4 jo = xo[r.j];
-                const int im2 = r.meta & 31;
-                const int f2 = ((im2 / 9 - 1) != 0 ? 1 : 0) |
-                               (((im2 / 3) % 3 - 1) != 0 ? 2 : 0) | ((im2 % 3 - 1) != 0 ? 4 : 0);
-                const float a2[3] = {jo.x, jo.y, jo.z}, a3[3] = {kco.x, kco.y, kco.z};
-                double t23[3];
-                for (int qd = 0; qd < 3; ++qd) {
-                  const double d = __dsub_rn((double)a3[qd], (double)a2[qd]);
-                  t23[qd] = a.bpc ? proj_term(d, a.L, ((f2 >> qd) & 1) != ((g3 >> qd) & 1)) : d;
-                }
-                const double s23 = __dadd_rn(
-                    __dadd_rn(__dmul_rn(t23[0], t23[0]), __dmul_rn(t23[1], t23[1])),
-                    __dmul_rn(t23[2], t23[2]));
-                if (!(a.auto_ && !(s23 > 0.0))) {
-                  const int e23 = class_lds(s23, sT, B, a.R, 0);
-                  if (e23 >= 0) c23 = (unsigned)e23;
-                }
+          if (!PLAIN) {
+            const unsigned long long m = __ballot(amb);
+            if (m) {
+              const int n = __popcll(m);
+              nre += n;
+              if (qn + n > kWave) drain();
+              if (amb) {
+                const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi(
+                                         (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                wq[pos] = make_uint2((unsigned)pidx, (unsigned)(qbeg + lane * nch + ch));
+                c23 = ubins;
               }
+              qn += n;
+            }
+          } else if (amb) {
+            c23 = ubins;
+            if (!(a.auto_ && k == r.j)) {
+              if (!kco_loaded) {
+                kco = ko_all[k];
+                kco_loaded = true;
+              }
+              const int e23 = exact23(r, xo[r.j], kco, g3);
+              if (e23 >= 0) c23 = (unsigned)e23;
             }
           }
           return c23;
@@ -945,10 +998,10 @@ __global__ __launch_bounds__(kTripBlock) void k_trip_ani_lds(
           const PairRec r0 = bcast(u), r1 = bcast(u + 1), r2 = bcast(u + 2), r3 = bcast(u + 3);
           unsigned c0 = lookup(r0), c1 = lookup(r1), c2 = lookup(r2), c3 = lookup(r3);
           evals += 4ull * nact;
-          c0 = classify(r0, c0);
-          c1 = classify(r1, c1);
-          c2 = classify(r2, c2);
-          c3 = classify(r3, c3);
+          c0 = classify(r0, pc + u, c0);
+          c1 = classify(r1, pc + u + 1, c1);
+          c2 = classify(r2, pc + u + 2, c2);
+          c3 = classify(r3, pc + u + 3, c3);
           deposit(r0, c0);
           deposit(r1, c1);
           deposit(r2, c2);
@@ -957,10 +1010,11 @@ __global__ __launch_bounds__(kTripBlock) void k_trip_ani_lds(
         for (; u < np; ++u) {
           const PairRec r = bcast(u);
           evals += nact;
-          deposit(r, classify(r, lookup(r)));
+          deposit(r, classify(r, pc + u, lookup(r)));
         }
       }
     }
+    if (!PLAIN && qn) drain();
   }
   __syncthreads();
   // LDS row (b31, b23, dt, p) -> global [b12][b31][b23][t][p]
@@ -1094,9 +1148,11 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   // form's error margin (they must share one scale).
   const bool gram = !ani && !plain;
   const double rho = gram ? (double)reach * 1.01 : -1.0;
-  if (build_table(pr, A, kMixedTableMax, &ptab, &inv_h, &nt, rho) != 0 ||
-      build_table(kr, A, kMixedTableMax, &ktab, &inv_h2, &nt2, rho) != 0 || inv_h != inv_h2 ||
-      nt != nt2)
+  // 3ani's LDS kernel keeps its recheck queues behind a smaller table.
+  const int tab_max = ani ? kAniTableMax : kMixedTableMax;
+  if (build_table(pr, A, tab_max, &ptab, &inv_h, &nt, rho) != 0 ||
+      build_table(kr, A, tab_max, &ktab, &inv_h2, &nt2, rho) != 0 || inv_h != inv_h2 ||
+      nt != nt2 || nt > tab_max)
     return fail(c, PCF_ERR_UNSUPPORTED, "bin table");
   const float sigma = std::sqrt(inv_h);
   HIPCHK(c, c->table_buf.ensure(2 * (size_t)nt + sizeof(double) * (bins + 2)));
