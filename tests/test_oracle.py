@@ -164,8 +164,7 @@ def test_weights_are_products():
 
 def test_real_catalog_fp64_vs_cpp32_close():
     # FP64 and the reference's FP32 arithmetic differ only by pairs within a
-    # rounding of a bin edge or the range limit (Appendix of the survey):
-    # a handful out of ~4e8.
+    # rounding of a bin edge or the range limit: a handful out of ~4e8.
     d = catalog("data.dat")
     c, _ = O.pairs(d, bins=20, dmax=150.0)
     ref = _ref("2iso_normal", "DDiso_mesh_3D_ref.dat")
