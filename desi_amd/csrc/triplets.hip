@@ -912,18 +912,21 @@ __global__ __launch_bounds__(kTripBlock) void k_trip_ani_lds(
     const int qend = __builtin_amdgcn_readfirstlane(qoff[(i + 1) * sl.qstride]);
     if (qend <= qbeg) continue;
     wi = (double)xo[i].w;
-    // Lane l of chunk c takes entry l * nch + c: the list is sorted by (b31,
-    // p), so consecutive entries share histogram rows; spreading them over the
-    // lanes keeps a wave's LDS atomics off one address.
+    // The list is cut into 8 regions of nch * 8 entries; chunk c gives the 8
+    // lanes of group g the entries g * nch * 8 + c * 8 + [0, 8).  One load
+    // then reads 8 runs of 8 consecutive records (a few cache lines, not 64),
+    // while the list's sort by (b31, p) still spreads a wave's histogram rows
+    // over 8 regions, keeping its LDS atomics off one address.
     const int nq = qend - qbeg;
     const int nch = (nq + kWave - 1) / kWave;
+    auto qslot = [&](int ch) { return ((lane >> 3) * nch + ch) * 8 + (lane & 7); };
     auto load_q = [&](int ch) {
       PairRec kr;
       kr.xj = kr.yj = kr.zj = 1e18f;
       kr.wj = 0.f;
       kr.j = -1;
       kr.meta = 0;
-      const int qi = lane * nch + ch;
+      const int qi = qslot(ch);
       if (qi < nq) kr = Q[qbeg + qi];
       return kr;
     };
@@ -936,7 +939,7 @@ __global__ __launch_bounds__(kTripBlock) void k_trip_ani_lds(
       for (int ch = 0; ch < nch; ++ch) {
         const PairRec kr = kr_next;
         if (ch + 1 < nch) kr_next = load_q(ch + 1);  // in flight while this chunk runs
-        const bool active = lane * nch + ch < nq;
+        const bool active = qslot(ch) < nq;
         const int k = kr.j;
         const int img3 = kr.meta & 31;
         const int b31 = (kr.meta >> 8) & 0xfff;
@@ -962,7 +965,7 @@ __global__ __launch_bounds__(kTripBlock) void k_trip_ani_lds(
               if (amb) {
                 const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi(
                                          (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-                wq[pos] = make_uint2((unsigned)pidx, (unsigned)(qbeg + lane * nch + ch));
+                wq[pos] = make_uint2((unsigned)pidx, (unsigned)(qbeg + qslot(ch)));
                 c23 = ubins;
               }
               qn += n;
