@@ -107,7 +107,9 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_pairs(
     const double* __restrict__ T, int* __restrict__ cnt, const int* __restrict__ off,
     int* __restrict__ cursor, PairRec* __restrict__ out, EmitArgs e) {
   __shared__ double sT[kMaxTripBins + 2];
+  __shared__ uint8_t tab[kMixedTableMax];  // the bin table (e.nt <= kMixedTableMax bytes)
   for (int t = threadIdx.x; t <= e.bins; t += kEmitBlock) sT[t] = T[t];
+  for (int t = threadIdx.x; t < e.nt; t += kEmitBlock) tab[t] = table[t];
   __syncthreads();
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = (int)((blockIdx.x * (unsigned)kEmitBlock + threadIdx.x) >> 6);
@@ -137,7 +139,7 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_pairs(
       const float4 q = xs[j];
       const float dx = q.x - sx, dy = q.y - sy, dz = q.z - sz;
       const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-      unsigned row = table[min(__float2uint_rz(d2), ntm1)];
+      unsigned row = tab[min(__float2uint_rz(d2), ntm1)];
       if (masked && j <= my_i) row = bins;
       if (!active) row = bins;
       // Everything emitted gets the exact bin: the pairs are few compared
@@ -228,7 +230,9 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_cross(
     const double* __restrict__ T, int* __restrict__ cnt, const int* __restrict__ off,
     int* __restrict__ cursor, PairRec* __restrict__ out, EmitArgs e) {
   __shared__ double sT[kMaxTripBins + 2];
+  __shared__ uint8_t tab[kMixedTableMax];  // the bin table (e.nt <= kMixedTableMax bytes)
   for (int t = threadIdx.x; t <= e.bins; t += kEmitBlock) sT[t] = T[t];
+  for (int t = threadIdx.x; t < e.nt; t += kEmitBlock) tab[t] = table[t];
   __syncthreads();
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = (int)((blockIdx.x * (unsigned)kEmitBlock + threadIdx.x) >> 6);
@@ -256,7 +260,7 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_cross(
       const float4 q = ys[j];
       const float dx = q.x - sx, dy = q.y - sy, dz = q.z - sz;
       const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-      unsigned row = table[min(__float2uint_rz(d2), ntm1)];
+      unsigned row = tab[min(__float2uint_rz(d2), ntm1)];
       if (!active) row = bins;
       if (row < bins || row == amb_row) {
         const float4 qo = yo[j];
