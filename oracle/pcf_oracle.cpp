@@ -554,9 +554,15 @@ extern "C" {
 // double per triplet (the reference multiplies cumulatively across the k loop,
 // cuda/kernels/3iso_k.cuh:79 -- not reproduced).
 // Pinned by: brute force over all 64 projection combinations
-// (pcf_oracle_triplets_brute), the cpp32 comparison with the compiled
-// reference CPU program, the reference's results/*.example, permutation
-// symmetry, and the 3iso marginal of 3ani.
+// (pcf_oracle_triplets_brute: the enumeration), the cpp32 comparison with the
+// compiled reference CPU program, the reference's results/*.example,
+// permutation symmetry, the 3iso marginal of 3ani, and -- independently of
+// triplet_bin -- hand-built 3ani triplets of known angles (t, p, the cos = 1
+// exclusion, the periodic |dz| - L*f sign rule), the 2 <-> 3 exchange and
+// z-reflection identities of the angle bins, a minimum-image brute force of
+// periodic 3iso/3ani on integer lattices (the |f2 - f3| rule of the 2-3 side)
+// and translation invariance; a mutation test shows each planted mistake in
+// these rules fails one of them (tests/test_oracle_triplet_pins.py).
 int pcf_oracle_triplets(const float* a_, int64_t na, const float* b_, int64_t nb, int auto_,
                         int ani, int bpc, float box, int bins, float dmax, int nthreads,
                         uint64_t* counts, double* wsum) {

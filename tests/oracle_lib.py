@@ -19,31 +19,40 @@ _i64 = ctypes.c_int64
 _i32 = ctypes.c_int
 
 
+def _configure(L):
+    L.pcf_oracle_pairs.argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32, _i32, ctypes.c_float,
+                                   _i32, ctypes.c_float, _i32, _u64p, _f64p]
+    L.pcf_oracle_pairs_brute.argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32, _i32,
+                                         ctypes.c_float, _i32, ctypes.c_float, _u64p, _f64p]
+    L.pcf_oracle_cpp32_2iso.argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32, ctypes.c_float,
+                                        _i32, _f64p]
+    L.pcf_oracle_analytic_rr.argtypes = [_i32, _i64, ctypes.c_float, ctypes.c_float, _f64p]
+    L.pcf_oracle_analytic_rr.restype = None
+    L.pcf_oracle_cpp32_3iso.argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32, ctypes.c_float,
+                                        _i32, _f64p]
+    L.pcf_oracle_analytic_3iso.argtypes = [_f32p, _i64, _i32, ctypes.c_float, ctypes.c_float,
+                                           _i64, _i32, _i32, _i32, _f64p, _f64p, _f64p,
+                                           _f64p]
+    for name in ("pcf_oracle_triplets", "pcf_oracle_triplets_brute"):
+        if hasattr(L, name):
+            getattr(L, name).argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32, _i32,
+                                         ctypes.c_float, _i32, ctypes.c_float, _i32,
+                                         _u64p, _f64p]
+    return L
+
+
+def load(path):
+    """A configured handle on an oracle build at `path` (the mutation tests
+    load deliberately broken builds of the oracle this way)."""
+    return _configure(ctypes.CDLL(path))
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             raise RuntimeError("oracle not built: run `make -C oracle`")
-        L = ctypes.CDLL(_LIB_PATH)
-        L.pcf_oracle_pairs.argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32, _i32, ctypes.c_float,
-                                       _i32, ctypes.c_float, _i32, _u64p, _f64p]
-        L.pcf_oracle_pairs_brute.argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32, _i32,
-                                             ctypes.c_float, _i32, ctypes.c_float, _u64p, _f64p]
-        L.pcf_oracle_cpp32_2iso.argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32, ctypes.c_float,
-                                            _i32, _f64p]
-        L.pcf_oracle_analytic_rr.argtypes = [_i32, _i64, ctypes.c_float, ctypes.c_float, _f64p]
-        L.pcf_oracle_analytic_rr.restype = None
-        L.pcf_oracle_cpp32_3iso.argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32, ctypes.c_float,
-                                            _i32, _f64p]
-        L.pcf_oracle_analytic_3iso.argtypes = [_f32p, _i64, _i32, ctypes.c_float, ctypes.c_float,
-                                               _i64, _i32, _i32, _i32, _f64p, _f64p, _f64p,
-                                               _f64p]
-        for name in ("pcf_oracle_triplets", "pcf_oracle_triplets_brute"):
-            if hasattr(L, name):
-                getattr(L, name).argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32, _i32,
-                                             ctypes.c_float, _i32, ctypes.c_float, _i32,
-                                             _u64p, _f64p]
-        _lib = L
+        _lib = load(_LIB_PATH)
     return _lib
 
 
@@ -130,16 +139,17 @@ def analytic_3iso(a, *, bins, dmax, box, np_=0, ptt=100, bn_ref=200, threads=0, 
 
 
 def triplets(a, b=None, *, ani=False, bpc=False, box=0.0, bins=30, dmax=80.0, threads=0,
-             brute=False, ddr=True):
+             brute=False, ddr=True, L=None):
     """Ordered-triplet histogram.  b=None: XXX over a.  Otherwise XXY with the
-    pair (1,2) from a and point 3 from b."""
+    pair (1,2) from a and point 3 from b.  L: an oracle handle from load()
+    (default: the oracle build in oracle/)."""
     a = _cat(a)
     auto = b is None
     b = a if auto else _cat(b)
     n = bins ** 5 if ani else bins ** 3
     c = np.zeros(n, np.uint64)
     w = np.zeros(n, np.float64)
-    L = lib()
+    L = lib() if L is None else L
     if brute:
         rc = L.pcf_oracle_triplets_brute(_p(a, _f32p), len(a), _p(b, _f32p), len(b), int(auto),
                                          int(ani), int(bpc), box, bins, dmax, 1,
