@@ -78,7 +78,9 @@ void make_rules(int type, int bpc, int auto_, int bins, float dmax, float box, B
     r->le = bpc ? 1 : 0;
     r->zero = (!bpc || auto_) ? 2 : 0;
   }
-  bin_thresholds(bins, r->ds, r->T);
+  // Thresholds serve the mixed path's exact re-evaluation (bins <= kMaxBinsIso);
+  // the plain FP64 kernels bin with sqrt directly.
+  if (bins <= kMaxBinsIso) bin_thresholds(bins, r->ds, r->T);
 }
 
 // Exact class of a double d2 for one component: bin in [0,bins) or -1 (out).
@@ -333,7 +335,10 @@ int choose_geometry(const Catalog* X, const Catalog* Y, float dmax, int bpc, flo
   }
   for (int a = 0; a < 3; ++a) {
     K[a] = (int)std::ceil(reach / g->cs[a]);
-    if (bpc && K[a] >= g->P[a]) return -1;
+    // Periodic: offsets up to one period reach every reference term (the
+    // regular separation and the close image, both within L); larger ones
+    // would only add far images.
+    if (bpc && K[a] > g->P[a]) K[a] = g->P[a];
   }
   *reach_out = (float)(reach * (1.0 + 1e-6));
   if (m_out) *m_out = m;
@@ -455,32 +460,65 @@ namespace {
 
 int64_t gx_points(const Catalog* X) { return X->n; }
 
-int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights) {
+// Carves aligned pieces out of a fixed block of device memory.  With base ==
+// nullptr it only adds up the bytes a plan needs.
+struct Carve {
+  char* base;
+  size_t used = 0;
+  explicit Carve(void* b) : base(reinterpret_cast<char*>(b)) {}
+  template <class T>
+  T* take(size_t bytes) {
+    const size_t off = (used + 255) & ~(size_t)255;
+    used = off + std::max<size_t>(bytes, 1);
+    return base ? reinterpret_cast<T*>(base + off) : nullptr;
+  }
+};
+
+}  // namespace
+
+namespace pcf {
+
+// One pair histogram (2iso / 2ani, XX / XY).  Device buffers come from the
+// context (growable buffers, resident grid cache) or, with arena != nullptr,
+// from the arena_bytes of device memory at arena: nothing is allocated then
+// and nothing outlives the call.
+int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights, void* arena,
+              size_t arena_bytes) {
   const bool auto_ = job->hist == PCF_XX;
   Catalog* X = get_cat(c, job->cat_x);
   Catalog* Y = auto_ ? X : get_cat(c, job->cat_y);
   if (!X || !Y) return fail(c, PCF_ERR_ARG, "unknown catalog id");
   const int bins = job->bins;
   const bool ani = job->type == PCF_2ANI;
-  if (bins <= 0 || bins > kMaxBinsIso || (ani && bins > 120))
-    return fail(c, PCF_ERR_ARG, "bins out of range");
+  const int64_t nflat64 = ani ? (int64_t)bins * bins : (int64_t)bins;
+  if (bins <= 0 || nflat64 > kMaxFlatBins) return fail(c, PCF_ERR_ARG, "bins out of range");
   if (!(job->dmax > 0) || !std::isfinite(job->dmax)) return fail(c, PCF_ERR_ARG, "dmax must be > 0");
   const int bpc = job->bpc ? 1 : 0;
   const float L = job->box;
   if (bpc) {
     if (!(L > 0)) return fail(c, PCF_ERR_ARG, "box side required for periodic boundaries");
-    if (!(2.0 * job->dmax < L))
-      return fail(c, PCF_ERR_UNSUPPORTED, "periodic boundaries need 2*dmax < box side");
+    // 2*dmax >= L is handled (a pair may count under its regular and its
+    // projected image, as the reference's per-axis flags allow); dmax >= L
+    // would also reach the far image of every axis, which no reference term is.
+    if (!(job->dmax < L))
+      return fail(c, PCF_ERR_UNSUPPORTED, "periodic boundaries need dmax < box side");
     for (Catalog* cc : {X, Y})
       for (int a = 0; a < 3; ++a)
         if (cc->n && (cc->lo[a] < 0.f || !(cc->hi[a] < L)))
           return fail(c, PCF_ERR_RANGE, "coordinates outside [0, box) with periodic boundaries");
   }
-  const int nflat = ani ? bins * bins : bins;
+  const int nflat = (int)nflat64;
   const bool weighted = !(X->all_w1 && Y->all_w1);
-  // The mixed kernel's recheck queue indexes j-side points with 27 bits.
+  // The mixed (FP32 + FP64 recheck) kernel needs its u8 d2 -> bin table
+  // codes, its thresholds and LDS histogram to fit, and j-side points that its
+  // recheck queue can index (27 bits); everything else runs the plain FP64
+  // kernel, which gives the same results.
+  const bool mixed_fits = pair_mixed_fits(bins, ani, weighted);
   const bool plain = job->precision == PCF_PREC_FP64 || X->n >= kRecheckMaxPoints ||
-                     Y->n >= kRecheckMaxPoints;
+                     Y->n >= kRecheckMaxPoints || !mixed_fits;
+  // Histograms larger than an LDS copy deposit straight into global memory.
+  const bool global_hist =
+      plain && (size_t)(nflat + 1) * (weighted ? 12 : 4) > kPlainLdsHistMax;
   // Gram-form path (matrix cores): unweighted, mixed precision.  Its cells
   // are cut kGramZsub times thinner along z: the j range of a tile then ends
   // within a thin cell of what the tile can reach.
@@ -493,18 +531,6 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   if (choose_geometry(X, auto_ ? nullptr : Y, job->dmax, bpc, L, job->cells_per_rmax, &g, K,
                       &reach, &m, gram ? kGramZsub : 1) != 0)
     return fail(c, PCF_ERR_UNSUPPORTED, "geometry");
-  const bool rebuild = job->rebuild_grid != 0;
-  int rc = PCF_OK;
-  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-  GridPins pins(c);
-  DevGrid* gx = pins.add(get_grid(c, X, g, rebuild, &rc));
-  if (!gx) return rc;
-  DevGrid* gy = auto_ ? gx : pins.add(get_grid(c, Y, g, rebuild, &rc));
-  if (!gy) return rc;
-  if (plain) {
-    HIPCHK(c, ensure_grid64(c, gx));
-    if (!auto_) HIPCHK(c, ensure_grid64(c, gy));
-  }
 
   // Rules, thresholds and the FP32 table.
   static thread_local BinRules rules;
@@ -520,17 +546,54 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   if (!plain && build_table(rules, A, nt_max, &table, &inv_h, &nt, rho) != 0)
     return fail(c, PCF_ERR_UNSUPPORTED, "bin table");
   const float sigma = std::sqrt(inv_h);  // exact power of two
-  // Rules and table in one device buffer, uploaded from page-locked staging
-  // (asynchronous) and only when they differ from the last upload.  The
-  // staging bytes are not touched again before this call's final sync.
   const size_t rt_bytes = sizeof(BinRules) + (plain ? 0 : (size_t)nt);
   const size_t hist_bytes = sizeof(unsigned long long) * nflat;
   const size_t w_bytes = sizeof(double) * nflat;
   const size_t rd_bytes = hist_bytes + w_bytes + 32;  // histogram, weights, statistics
-  const size_t rd_off = (rt_bytes + 255) & ~(size_t)255;
-  HIPCHK(c, c->pin.ensure(rd_off + rd_bytes));
-  HIPCHK(c, c->pair_rt.ensure(rt_bytes));
-  {
+  // tail: 4 u64 statistics, then the work cursors (8 ranges x 64 bytes)
+  const size_t tail = 32 + 8 * 64;
+
+  WorkListArgs wa;
+  wa.g = g;
+  for (int a = 0; a < 3; ++a) wa.K[a] = K[a];
+  wa.reach = reach;
+  wa.auto_ = auto_ ? 1 : 0;
+  wa.cylinder = ani ? 1 : 0;
+  wa.map = nullptr;
+  const int64_t cap = worklist_bound(wa, X->n);
+
+  const bool rebuild = job->rebuild_grid != 0;
+  int rc = PCF_OK;
+  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  GridPins pins(c);
+  DevGrid* gx = nullptr;
+  DevGrid* gy = nullptr;
+  DevGrid ax, ay;  // arena grids (not cached)
+  const uint8_t* rt_dev = nullptr;
+  void* hist_dev = nullptr;
+  float4* scaled_x = nullptr;
+  float4* scaled_y = nullptr;
+  PairItem* items = nullptr;
+  int nitems = 0;
+  const int* nitems_dev = nullptr;
+  std::vector<uint8_t> host_stage;  // arena mode: pageable staging
+  uint8_t* rd = nullptr;
+
+  if (!arena) {
+    gx = pins.add(get_grid(c, X, g, rebuild, &rc));
+    if (!gx) return rc;
+    gy = auto_ ? gx : pins.add(get_grid(c, Y, g, rebuild, &rc));
+    if (!gy) return rc;
+    if (plain) {
+      HIPCHK(c, ensure_grid64(c, gx));
+      if (!auto_) HIPCHK(c, ensure_grid64(c, gy));
+    }
+    // Rules and table in one device buffer, uploaded from page-locked staging
+    // (asynchronous) and only when they differ from the last upload.  The
+    // staging bytes are not touched again before this call's final sync.
+    const size_t rd_off = (rt_bytes + 255) & ~(size_t)255;
+    HIPCHK(c, c->pin.ensure(rd_off + rd_bytes));
+    HIPCHK(c, c->pair_rt.ensure(rt_bytes));
     uint8_t* st = c->pin.as<uint8_t>();
     std::memcpy(st, &rules, sizeof(BinRules));
     if (!plain) std::memcpy(st + sizeof(BinRules), table.data(), nt);
@@ -539,65 +602,123 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
       HIPCHK(c, hipMemcpyAsync(c->pair_rt.p, st, rt_bytes, hipMemcpyHostToDevice, c->stream));
       c->pair_rt_host.assign(st, st + rt_bytes);
     }
-  }
-
-  // Work list.
-  WorkListArgs wa;
-  wa.x_cell_start = gx->cell_start;
-  wa.y_cell_start = gy->cell_start;
-  wa.g = g;
-  for (int a = 0; a < 3; ++a) wa.K[a] = K[a];
-  wa.reach = reach;
-  wa.auto_ = auto_ ? 1 : 0;
-  wa.cylinder = ani ? 1 : 0;
-  wa.map = nullptr;
-  // The list is sized by its bound and its length stays on the device (no
-  // host round trip), unless the bound is too large to allocate casually.
-  int nitems = 0;
-  const int* nitems_dev = nullptr;
-  const int64_t cap = worklist_bound(wa, gx->n);
-  if (cap <= ((int64_t)1 << 25)) {
-    HIPCHK(c, build_pair_worklist_async(wa, cap, c->wl_counts, c->wl_items, &nitems_dev,
-                                        c->stream));
+    rt_dev = c->pair_rt.as<uint8_t>();
+    rd = st + rd_off;
+    // The list is sized by its bound and its length stays on the device (no
+    // host round trip), unless the bound is too large to allocate casually.
+    wa.x_cell_start = gx->cell_start;
+    wa.y_cell_start = gy->cell_start;
+    if (cap <= ((int64_t)1 << 25)) {
+      HIPCHK(c, build_pair_worklist_async(wa, cap, c->wl_counts, c->wl_items, &nitems_dev,
+                                          c->stream));
+    } else {
+      HIPCHK(c, build_pair_worklist(wa, c->wl_counts, c->wl_items, &nitems, c->stream));
+    }
+    items = c->wl_items.as<PairItem>();
+    HIPCHK(c, c->hist_buf.ensure(hist_bytes + w_bytes + tail));
+    hist_dev = c->hist_buf.p;
+    if (!plain) {
+      HIPCHK(c, c->scaled_x.ensure(sizeof(float4) * (size_t)std::max<int64_t>(gx->n, 1)));
+      scaled_x = c->scaled_x.as<float4>();
+      if (!auto_) {
+        HIPCHK(c, c->scaled_y.ensure(sizeof(float4) * (size_t)std::max<int64_t>(gy->n, 1)));
+        scaled_y = c->scaled_y.as<float4>();
+      }
+    }
   } else {
-    HIPCHK(c, build_pair_worklist(wa, c->wl_counts, c->wl_items, &nitems, c->stream));
+    // Plan the arena (sizes only), check it, then carve it for real.
+    const int64_t ncell = g.ncell();
+    const int64_t nmax = std::max<int64_t>(X->n, Y->n);
+    auto plan = [&](Carve& cv) {
+      ax.pts = cv.take<float4>(sizeof(float4) * (size_t)X->n);
+      ax.cell_start = cv.take<int>(sizeof(int) * (size_t)(ncell + 1));
+      if (plain) ax.pts64 = cv.take<double4>(sizeof(double4) * (size_t)X->n);
+      if (!auto_) {
+        ay.pts = cv.take<float4>(sizeof(float4) * (size_t)Y->n);
+        ay.cell_start = cv.take<int>(sizeof(int) * (size_t)(ncell + 1));
+        if (plain) ay.pts64 = cv.take<double4>(sizeof(double4) * (size_t)Y->n);
+      }
+      void* cells_tmp = cv.take<void>(grid_cells_tmp_bytes(ncell));
+      int* perm_tmp = cv.take<int>(2 * sizeof(int) * (size_t)std::max<int64_t>(nmax, 1));
+      rt_dev = cv.take<uint8_t>(rt_bytes);
+      void* wl_counts = cv.take<void>(worklist_count_bytes(wa));
+      items = cv.take<PairItem>(sizeof(PairItem) * (size_t)std::max<int64_t>(cap, 1));
+      hist_dev = cv.take<void>(hist_bytes + w_bytes + tail);
+      if (!plain) {
+        scaled_x = cv.take<float4>(sizeof(float4) * (size_t)X->n);
+        if (!auto_) scaled_y = cv.take<float4>(sizeof(float4) * (size_t)Y->n);
+      }
+      return std::make_pair(cells_tmp, std::make_pair(perm_tmp, wl_counts));
+    };
+    Carve sizing(nullptr);
+    plan(sizing);
+    if (sizing.used > arena_bytes)
+      return fail(c, PCF_ERR_NOMEM,
+                  "job needs " + std::to_string(sizing.used) +
+                      " bytes of scratch; the no-allocation entry point has " +
+                      std::to_string(arena_bytes));
+    Carve cv(arena);
+    auto tmp = plan(cv);
+    void* cells_tmp = tmp.first;
+    int* perm_tmp = tmp.second.first;
+    void* wl_counts = tmp.second.second;
+    ax.g = g;
+    ax.n = X->n;
+    HIPCHK(c, build_grid_into(X->raw, X->n, g, ax.pts, ax.cell_start, cells_tmp, perm_tmp,
+                              c->stream));
+    if (plain) HIPCHK(c, convert_grid64(ax.pts, ax.n, ax.pts64, c->stream));
+    gx = &ax;
+    gy = gx;
+    if (!auto_) {
+      ay.g = g;
+      ay.n = Y->n;
+      HIPCHK(c, build_grid_into(Y->raw, Y->n, g, ay.pts, ay.cell_start, cells_tmp, perm_tmp,
+                                c->stream));
+      if (plain) HIPCHK(c, convert_grid64(ay.pts, ay.n, ay.pts64, c->stream));
+      gy = &ay;
+    }
+    host_stage.resize(std::max(rt_bytes, rd_bytes));
+    std::memcpy(host_stage.data(), &rules, sizeof(BinRules));
+    if (!plain) std::memcpy(host_stage.data() + sizeof(BinRules), table.data(), nt);
+    // Pageable staging, alive until the final synchronisation below.
+    HIPCHK(c, hipMemcpyAsync(const_cast<uint8_t*>(rt_dev), host_stage.data(), rt_bytes,
+                             hipMemcpyHostToDevice, c->stream));
+    rd = host_stage.data();
+    wa.x_cell_start = gx->cell_start;
+    wa.y_cell_start = gy->cell_start;
+    HIPCHK(c, build_pair_worklist_into(wa, cap, wl_counts, items, &nitems_dev, c->stream));
   }
 
   // Histogram + counters.
-  // tail: 4 u64 statistics, then the work cursors (8 ranges x 64 bytes)
-  const size_t tail = 32 + 8 * 64;
-  HIPCHK(c, c->hist_buf.ensure(hist_bytes + w_bytes + tail));
-  unsigned long long* d_hist = c->hist_buf.as<unsigned long long>();
-  double* d_w = reinterpret_cast<double*>(c->hist_buf.as<char>() + hist_bytes);
-  unsigned long long* d_stats =
-      reinterpret_cast<unsigned long long*>(c->hist_buf.as<char>() + hist_bytes + w_bytes);
+  unsigned long long* d_hist = reinterpret_cast<unsigned long long*>(hist_dev);
+  double* d_w = reinterpret_cast<double*>(reinterpret_cast<char*>(hist_dev) + hist_bytes);
+  unsigned long long* d_stats = reinterpret_cast<unsigned long long*>(
+      reinterpret_cast<char*>(hist_dev) + hist_bytes + w_bytes);
   int* d_counter = reinterpret_cast<int*>(d_stats + 4);
-  HIPCHK(c, launch_zero(c->hist_buf.p, hist_bytes + w_bytes + tail, c->stream));
+  HIPCHK(c, launch_zero(hist_dev, hist_bytes + w_bytes + tail, c->stream));
 
   PairLaunch pl{};
   pl.xp = gx->pts;
   pl.yp = gy->pts;
   if (!plain) {
     // Scaled copies for the bulk path (x, y, z times sigma, exact).
-    HIPCHK(c, c->scaled_x.ensure(sizeof(float4) * (size_t)std::max<int64_t>(gx->n, 1)));
-    HIPCHK(c, launch_scale_points(gx->pts, gx->n, sigma, c->scaled_x.as<float4>(), c->stream));
-    pl.xs = c->scaled_x.as<float4>();
+    HIPCHK(c, launch_scale_points(gx->pts, gx->n, sigma, scaled_x, c->stream));
+    pl.xs = scaled_x;
     if (auto_) {
       pl.ys = pl.xs;
     } else {
-      HIPCHK(c, c->scaled_y.ensure(sizeof(float4) * (size_t)std::max<int64_t>(gy->n, 1)));
-      HIPCHK(c, launch_scale_points(gy->pts, gy->n, sigma, c->scaled_y.as<float4>(), c->stream));
-      pl.ys = c->scaled_y.as<float4>();
+      HIPCHK(c, launch_scale_points(gy->pts, gy->n, sigma, scaled_y, c->stream));
+      pl.ys = scaled_y;
     }
   }
   pl.xp64 = gx->pts64;
   pl.yp64 = gy->pts64;
-  pl.items = c->wl_items.as<PairItem>();
+  pl.items = items;
   pl.nitems = nitems;
   pl.nitems_dev = nitems_dev;
   pl.nitems_cap = (int)std::min<int64_t>(cap, INT32_MAX);
   pl.counter = d_counter;
-  pl.table = c->pair_rt.as<uint8_t>() + sizeof(BinRules);
+  pl.table = rt_dev + sizeof(BinRules);
   pl.nt = nt;
   pl.inv_h = inv_h;
   pl.L = L * sigma;
@@ -605,11 +726,12 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   pl.nflat = nflat;
   const PairShape sh =
       pair_shape(nflat, bins, weighted, plain, job->lds_copies_log2, job->blocks_per_cu);
-  pl.cshift = sh.cshift;
+  pl.cshift = global_hist ? 0 : sh.cshift;
+  pl.global_hist = global_hist ? 1 : 0;
   pl.hist = d_hist;
   pl.whist = d_w;
   pl.stats = d_stats;
-  pl.rules = c->pair_rt.as<BinRules>();
+  pl.rules = reinterpret_cast<const BinRules*>(rt_dev);
   pl.ani = ani ? 1 : 0;
   pl.weighted = weighted ? 1 : 0;
   pl.plain64 = plain ? 1 : 0;
@@ -617,7 +739,7 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   pl.Lx = (double)L;
   pl.le = rules.le;
   pl.zero = rules.zero;
-  pl.nblocks = std::max(1, c->ncu * sh.per_cu);
+  pl.nblocks = std::max(1, c->ncu * (global_hist ? 8 : sh.per_cu));
   pl.mfma = gram ? 1 : 0;
   // The kernel's own test on the tile box, 2% inside the table's margin.
   pl.rho2 = gram ? (float)(rho * rho / (1.02 * 1.02) * (double)inv_h) : 0.f;
@@ -628,7 +750,6 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
   }
   // Histogram, weights and statistics are contiguous: one read-back.
-  uint8_t* rd = c->pin.as<uint8_t>() + rd_off;
   HIPCHK(c, hipMemcpyAsync(rd, d_hist, rd_bytes, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -661,6 +782,10 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights)
   c->last = st;
   return PCF_OK;
 }
+
+}  // namespace pcf
+
+namespace {
 
 __global__ void k_analytic_rr(double alpha, int bins, double* rr) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -842,7 +967,7 @@ int dispatch_histogram(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double*
         pcf_job tuned = *job;
         if (get_cat(c, job->cat_x) && (job->hist == PCF_XX || get_cat(c, job->cat_y)))
           apply_tuning(c, &tuned, 1);
-        return run_pairs(c, &tuned, counts, weights);
+        return run_pairs(c, &tuned, counts, weights, nullptr, 0);
       }
     case PCF_3ISO:
     case PCF_3ANI:

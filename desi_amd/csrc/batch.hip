@@ -371,20 +371,24 @@ int run_pairs_batch(pcf_ctx* c, const pcf_job* jobs_in, int njobs, uint64_t* cou
   const int64_t hs = pcf_hist_size(j0.type, j0.bins);
   // The plain FP64 path and out-of-range arguments go through the single-job
   // driver (same kernels, one launch per job).
+  // Histograms the mixed kernel does not serve (bin counts past its table or
+  // LDS) likewise.
   bool single = j0.precision == PCF_PREC_FP64;
+  bool any_weighted = false;
   for (int k = 0; k < njobs && !single; ++k)
-    for (pcf_catalog id : {jobs[k].cat_x, jobs[k].hist == PCF_XY ? jobs[k].cat_y : jobs[k].cat_x})
-      if (get_cat(c, id)->n >= kRecheckMaxPoints) single = true;
+    for (pcf_catalog id : {jobs[k].cat_x, jobs[k].hist == PCF_XY ? jobs[k].cat_y : jobs[k].cat_x}) {
+      const Catalog* X = get_cat(c, id);
+      if (X->n >= kRecheckMaxPoints) single = true;
+      if (!X->all_w1) any_weighted = true;
+    }
+  if (!single && !pair_mixed_fits(j0.bins, j0.type == PCF_2ANI, any_weighted)) single = true;
   if (!single) {
     // Argument checks of the single-job driver (range, box) on each job.
-    const int bins = j0.bins;
-    if (bins <= 0 || bins > 253 || (j0.type == PCF_2ANI && bins > 120))
-      return fail(c, PCF_ERR_ARG, "bins out of range");
     if (!(j0.dmax > 0) || !std::isfinite(j0.dmax)) return fail(c, PCF_ERR_ARG, "dmax must be > 0");
     if (j0.bpc) {
       if (!(j0.box > 0)) return fail(c, PCF_ERR_ARG, "box side required for periodic boundaries");
-      if (!(2.0 * j0.dmax < j0.box))
-        return fail(c, PCF_ERR_UNSUPPORTED, "periodic boundaries need 2*dmax < box side");
+      if (!(j0.dmax < j0.box))
+        return fail(c, PCF_ERR_UNSUPPORTED, "periodic boundaries need dmax < box side");
       for (int k = 0; k < njobs; ++k)
         for (pcf_catalog id : {jobs[k].cat_x, jobs[k].hist == PCF_XY ? jobs[k].cat_y : jobs[k].cat_x}) {
           Catalog* X = get_cat(c, id);

@@ -146,13 +146,27 @@ hipError_t build_grid(const float4* raw, int64_t n, const GridGeom& g, DevGrid* 
   }
   out->g = g;
   out->n = n;
-  const size_t cbytes = (sizeof(int) * (size_t)(nc + 1) + 15) & ~(size_t)15;
-  if ((e = scratch_cells.ensure(cbytes + sizeof(long long) * (size_t)scan_tiles(nc))) !=
-      hipSuccess)
-    return e;
+  if ((e = scratch_cells.ensure(grid_cells_tmp_bytes(nc))) != hipSuccess) return e;
   if ((e = scratch_perm.ensure(2 * sizeof(int) * (n > 0 ? n : 1))) != hipSuccess) return e;
-  int* counts = scratch_cells.as<int>();
-  int* cell_of = scratch_perm.as<int>();
+  if ((e = build_grid_into(raw, n, g, out->pts, out->cell_start, scratch_cells.p,
+                           scratch_perm.as<int>(), s)) != hipSuccess)
+    return e;
+  out->valid = true;
+  return hipSuccess;
+}
+
+size_t grid_cells_tmp_bytes(int64_t ncell) {
+  const size_t cbytes = (sizeof(int) * (size_t)(ncell + 1) + 15) & ~(size_t)15;
+  return cbytes + sizeof(long long) * (size_t)scan_tiles(ncell);
+}
+
+hipError_t build_grid_into(const float4* raw, int64_t n, const GridGeom& g, float4* pts,
+                           int* cell_start, void* cells_tmp, int* perm_tmp, hipStream_t s) {
+  hipError_t e;
+  const int64_t nc = g.ncell();
+  const size_t cbytes = (sizeof(int) * (size_t)(nc + 1) + 15) & ~(size_t)15;
+  int* counts = reinterpret_cast<int*>(cells_tmp);
+  int* cell_of = perm_tmp;
   int* rank = cell_of + (n > 0 ? n : 1);
   if ((e = launch_zero(counts, sizeof(int) * (nc + 1), s)) != hipSuccess) return e;
   const int blocks = (int)((n + 255) / 256);
@@ -161,34 +175,35 @@ hipError_t build_grid(const float4* raw, int64_t n, const GridGeom& g, DevGrid* 
                        counts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  if ((e = launch_scan(counts, nc, out->cell_start,
-                       reinterpret_cast<long long*>(scratch_cells.as<char>() + cbytes), s)) !=
-      hipSuccess)
+  if ((e = launch_scan(counts, nc, cell_start,
+                       reinterpret_cast<long long*>(reinterpret_cast<char*>(cells_tmp) + cbytes),
+                       s)) != hipSuccess)
     return e;
   if (n > 0) {
     hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, raw, n, cell_of, rank,
-                       out->cell_start, out->pts);
+                       cell_start, pts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  out->valid = true;
   return hipSuccess;
+}
+
+hipError_t convert_grid64(const float4* pts, int64_t n, double4* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_to_double, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pts, n,
+                     out);
+  return hipGetLastError();
 }
 
 hipError_t build_grid64(DevGrid* grid, hipStream_t s) {
   if (grid->pts64) return hipSuccess;
   hipError_t e = hipMalloc(&grid->pts64, sizeof(double4) * (grid->n > 0 ? grid->n : 1));
   if (e != hipSuccess) return e;
-  if (grid->n > 0) {
-    hipLaunchKernelGGL(k_to_double, dim3((unsigned)((grid->n + 255) / 256)), dim3(256), 0, s,
-                       grid->pts, grid->n, grid->pts64);
-    e = hipGetLastError();
-  }
-  return e;
+  return convert_grid64(grid->pts, grid->n, grid->pts64, s);
 }
 
 // ---------------------------------------------------------------------------
 // Device-wide exclusive prefix sum of int counts: out[i] = sum(in[0..i)),
-// out[n] = total (totals below 2^31).  Three passes over tiles of 4096:
+// out[n] = total, or -1 when the total is 2^31 or more.  Three passes over tiles of 4096:
 // tile sums, a one-workgroup scan of the tile sums, then each tile scanned in
 // LDS with its offset.  out may alias in.
 // ---------------------------------------------------------------------------
@@ -240,7 +255,9 @@ __global__ __launch_bounds__(kSumsBlock) void k_scan_tile_sums(long long* __rest
     tsum[i] = run;
     run += c;
   }
-  if (t == kSumsBlock - 1) *total = (int)part[t];
+  // Totals of 2^31 or more are flagged as -1 (the tile offsets of such a scan
+  // wrap, so its consumers must not use them); callers test for a negative total.
+  if (t == kSumsBlock - 1) *total = part[t] > 0x7fffffffLL ? -1 : (int)part[t];
 }
 
 __global__ __launch_bounds__(kScanThreads) void k_tile_scan(const int* in, int64_t n,

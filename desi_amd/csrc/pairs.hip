@@ -49,7 +49,8 @@ __global__ void k_worklist_fill(WorkListArgs a, int S, const int* __restrict__ o
                                 PairItem* __restrict__ items, int64_t cap) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.g.ncell() * S) return;
-  if ((int64_t)offsets[t + 1] > cap) return;  // beyond the list's allocation
+  // Beyond the list's allocation, or a count that overflowed the scan.
+  if (offsets[t] < 0 || offsets[t + 1] < offsets[t] || (int64_t)offsets[t + 1] > cap) return;
   enumerate_cell<true>(a, (int)(t / S), (int)(t % S), S, items + offsets[t]);
 }
 
@@ -700,7 +701,10 @@ __device__ __forceinline__ bool axis_sq64(double ai, double aj, int k, double L,
   return true;
 }
 
-template <int ANI, int WEIGHTED>
+// GLOBAL = 1: histograms too large for LDS (the reference's any-size -b):
+// every counted pair goes straight to the global histogram with 64-bit
+// atomics, as the reference kernels deposit (cuda/kernels/2iso_k.cuh:51-55).
+template <int ANI, int WEIGHTED, int GLOBAL>
 __global__ __launch_bounds__(kBlock) void k_pairs_fp64(
     const double4* __restrict__ xp, const double4* __restrict__ yp,
     const PairItem* __restrict__ items, int* __restrict__ counter,
@@ -711,9 +715,11 @@ __global__ __launch_bounds__(kBlock) void k_pairs_fp64(
   const int ncopy = 1 << k.cshift;
   uint32_t* hc = reinterpret_cast<uint32_t*>(smem);
   double* hw = reinterpret_cast<double*>(smem + ((nrows * ncopy * 4 + 15) & ~15));
-  for (int t = threadIdx.x; t < nrows * ncopy; t += kBlock) {
-    hc[t] = 0u;
-    if (WEIGHTED) hw[t] = 0.0;
+  if (!GLOBAL) {
+    for (int t = threadIdx.x; t < nrows * ncopy; t += kBlock) {
+      hc[t] = 0u;
+      if (WEIGHTED) hw[t] = 0.0;
+    }
   }
   __syncthreads();
   const double R = rules->R;
@@ -786,13 +792,20 @@ __global__ __launch_bounds__(kBlock) void k_pairs_fp64(
           }
         }
       }
+      if (GLOBAL) {
+        if (row >= 0) {
+          atomicAdd(&hist[row], 1ull);
+          if (WEIGHTED) atomicAdd(&whist[row], pi.w * q.w);
+        }
+        continue;
+      }
       const unsigned r = row >= 0 ? (unsigned)row : (unsigned)k.nflat;
       atomicAdd(&hc[(r << k.cshift) + copy], 1u);
       if (WEIGHTED) atomicAdd(&hw[(r << k.cshift) + copy], pi.w * q.w);
     }
   }
   __syncthreads();
-  for (int r = threadIdx.x; r < k.nflat; r += kBlock) {
+  for (int r = threadIdx.x; !GLOBAL && r < k.nflat; r += kBlock) {
     unsigned long long c = 0;
     double w = 0.0;
     for (int cpy = 0; cpy < ncopy; ++cpy) {
@@ -843,6 +856,7 @@ hipError_t build_pair_worklist(const WorkListArgs& a, DevBuf& counts, DevBuf& it
   if ((e = hipMemcpyAsync(&total, off + nc, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess)
     return e;
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  if (total < 0) return hipErrorInvalidValue;  // 2^31 items or more
   *nitems = total;
   if (total == 0) return hipSuccess;
   if ((e = items.ensure(sizeof(PairItem) * (size_t)total)) != hipSuccess) return e;
@@ -857,27 +871,38 @@ int64_t worklist_bound(const WorkListArgs& a, int64_t n_tile_points) {
   return tiles * cols * (a.g.bpc ? 3 : 1);
 }
 
-hipError_t build_pair_worklist_async(const WorkListArgs& a, int64_t cap, DevBuf& counts,
-                                     DevBuf& items, const int** total_dev, hipStream_t s) {
+size_t worklist_count_bytes(const WorkListArgs& a) {
+  const int S = worklist_split(a, a.g.ncell());
+  const int64_t nc = a.g.ncell() * S;
+  const size_t ibytes = (sizeof(int) * (size_t)(2 * nc + 2) + 15) & ~(size_t)15;
+  return ibytes + sizeof(long long) * (size_t)scan_tiles(nc);
+}
+
+hipError_t build_pair_worklist_into(const WorkListArgs& a, int64_t cap, void* counts,
+                                    PairItem* items, const int** total_dev, hipStream_t s) {
   hipError_t e;
   const int S = worklist_split(a, a.g.ncell());
   const int64_t nc = a.g.ncell() * S;
   const size_t ibytes = (sizeof(int) * (size_t)(2 * nc + 2) + 15) & ~(size_t)15;
-  if ((e = counts.ensure(ibytes + sizeof(long long) * (size_t)scan_tiles(nc))) != hipSuccess)
-    return e;
-  if ((e = items.ensure(sizeof(PairItem) * (size_t)std::max<int64_t>(cap, 1))) != hipSuccess)
-    return e;
-  int* cnt = counts.as<int>();
+  int* cnt = reinterpret_cast<int*>(counts);
   int* off = cnt + nc + 1;
-  long long* tsum = reinterpret_cast<long long*>(counts.as<char>() + ibytes);
+  long long* tsum = reinterpret_cast<long long*>(reinterpret_cast<char*>(counts) + ibytes);
   const int blocks = (int)((nc + 255) / 256);
   hipLaunchKernelGGL(k_worklist_count, dim3(blocks), dim3(256), 0, s, a, S, cnt);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = launch_scan(cnt, nc, off, tsum, s)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_worklist_fill, dim3(blocks), dim3(256), 0, s, a, S, off,
-                     items.as<PairItem>(), cap);
+  hipLaunchKernelGGL(k_worklist_fill, dim3(blocks), dim3(256), 0, s, a, S, off, items, cap);
   *total_dev = off + nc;
   return hipGetLastError();
+}
+
+hipError_t build_pair_worklist_async(const WorkListArgs& a, int64_t cap, DevBuf& counts,
+                                     DevBuf& items, const int** total_dev, hipStream_t s) {
+  hipError_t e;
+  if ((e = counts.ensure(worklist_count_bytes(a))) != hipSuccess) return e;
+  if ((e = items.ensure(sizeof(PairItem) * (size_t)std::max<int64_t>(cap, 1))) != hipSuccess)
+    return e;
+  return build_pair_worklist_into(a, cap, counts.p, items.as<PairItem>(), total_dev, s);
 }
 
 __global__ void k_gather_offsets(const int* __restrict__ off, int64_t stride, int njobs,
@@ -924,6 +949,7 @@ hipError_t build_batch_worklists(const std::vector<WorkListArgs>& args, DevBuf& 
     return e;
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
   const int total = (*job_offsets)[nj];
+  if (total < 0) return hipErrorInvalidValue;  // 2^31 items or more
   if (total == 0) return hipSuccess;
   if ((e = items.ensure(sizeof(PairItem) * (size_t)total)) != hipSuccess) return e;
   for (int k = 0; k < nj; ++k) {
@@ -999,17 +1025,23 @@ hipError_t launch_pairs(const PairLaunch& pl, hipStream_t s) {
   } else {
     const size_t hc_bytes = (size_t)(((pl.nflat + 1) * ncopy * 4 + 15) & ~15);
     const size_t hw_bytes = pl.weighted ? (size_t)(pl.nflat + 1) * ncopy * 8 : 0;
-    const size_t lds = hc_bytes + hw_bytes;
-#define PCF_LAUNCH_64(A, W)                                                                    \
-  do {                                                                                         \
-    if ((e = set_lds(k_pairs_fp64<A, W>, lds)) != hipSuccess) return e;                        \
-    hipLaunchKernelGGL((k_pairs_fp64<A, W>), grid, block, lds, s, pl.xp64, pl.yp64, pl.items,  \
-                       pl.counter, pl.rules, pl.hist, pl.whist, pl.stats, k);                  \
+    const size_t lds = pl.global_hist ? 0 : hc_bytes + hw_bytes;
+#define PCF_LAUNCH_64(A, W, G)                                                                    \
+  do {                                                                                            \
+    if ((e = set_lds(k_pairs_fp64<A, W, G>, lds)) != hipSuccess) return e;                        \
+    hipLaunchKernelGGL((k_pairs_fp64<A, W, G>), grid, block, lds, s, pl.xp64, pl.yp64, pl.items,  \
+                       pl.counter, pl.rules, pl.hist, pl.whist, pl.stats, k);                     \
   } while (0)
-    if (!pl.ani && !pl.weighted) PCF_LAUNCH_64(0, 0);
-    else if (!pl.ani && pl.weighted) PCF_LAUNCH_64(0, 1);
-    else if (pl.ani && !pl.weighted) PCF_LAUNCH_64(1, 0);
-    else PCF_LAUNCH_64(1, 1);
+#define PCF_LAUNCH_64G(A, W)                 \
+  do {                                       \
+    if (pl.global_hist) PCF_LAUNCH_64(A, W, 1); \
+    else PCF_LAUNCH_64(A, W, 0);             \
+  } while (0)
+    if (!pl.ani && !pl.weighted) PCF_LAUNCH_64G(0, 0);
+    else if (!pl.ani && pl.weighted) PCF_LAUNCH_64G(0, 1);
+    else if (pl.ani && !pl.weighted) PCF_LAUNCH_64G(1, 0);
+    else PCF_LAUNCH_64G(1, 1);
+#undef PCF_LAUNCH_64G
 #undef PCF_LAUNCH_64
   }
   return hipGetLastError();

@@ -125,6 +125,11 @@ int choose_geometry(const Catalog* X, const Catalog* Y, float dmax, int bpc, flo
 DevGrid* get_grid(pcf_ctx* c, Catalog* cat, const GridGeom& g, bool force_rebuild, int* rc);
 void free_catalog(Catalog* cat);
 
+// Defined in abi.hip: one pair histogram; with arena != nullptr every device
+// buffer is carved from those arena_bytes (nothing allocated).
+int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights, void* arena,
+              size_t arena_bytes);
+
 // Defined in triplets.hip.
 int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights);
 

@@ -15,7 +15,14 @@ namespace pcf {
 constexpr int kWave = 64;        // CDNA wavefront
 constexpr int kTile = 64;        // points of one i-tile (one per lane)
 constexpr int kBlock = 256;      // threads per workgroup of the counting kernels
-constexpr int kMaxBinsIso = 1024;
+constexpr int kMaxBinsIso = 1024;      // bins with exact FP64 thresholds (BinRules::T)
+constexpr int kMaxBinsIsoMixed = 253;  // u8 table codes: bins, out, ambiguous
+constexpr int kMaxBinsAniMixed = 120;
+constexpr int64_t kMaxFlatBins = (int64_t)1 << 27;  // pair histogram entries (2 GB of output)
+constexpr size_t kLdsPerCu = 160 * 1024;
+// Plain FP64 pair kernel: histograms (counts + weights) above this many LDS
+// bytes deposit straight into global memory.
+constexpr size_t kPlainLdsHistMax = 96 * 1024;
 
 // Cell grid of one job.  Cells are ordered x-major, z-fastest
 // (cell = (cx*Py + cy)*Pz + cz), the order the reference linearises its nodes
@@ -143,6 +150,12 @@ hipError_t launch_catalog_stats(const float4* raw, int64_t n, float* partials, i
                                 hipStream_t s);
 hipError_t build_grid(const float4* raw, int64_t n, const GridGeom& g, DevGrid* out,
                       DevBuf& scratch_cells, DevBuf& scratch_perm, hipStream_t s);
+// The same counting sort into caller-provided memory: pts[n], cell_start[ncell+1],
+// and temporaries cells_tmp (grid_cells_tmp_bytes) and perm_tmp (2n ints).
+size_t grid_cells_tmp_bytes(int64_t ncell);
+hipError_t build_grid_into(const float4* raw, int64_t n, const GridGeom& g, float4* pts,
+                           int* cell_start, void* cells_tmp, int* perm_tmp, hipStream_t s);
+hipError_t convert_grid64(const float4* pts, int64_t n, double4* out, hipStream_t s);
 hipError_t build_grid64(DevGrid* grid, hipStream_t s);
 
 // One histogram of a batched pair launch (k_pairs_mixed<..., BATCH = 1>).
@@ -195,6 +208,7 @@ struct PairLaunch {
   int mfma = 0;       // 2iso unweighted: Gram-form distances on the matrix cores
   float rho2 = 0.f;   // ... for tiles whose box half-diagonal^2 (scaled) is <= rho2
   int prepare_only = 0;  // only set the kernel's LDS attribute (before graph capture)
+  int global_hist = 0;   // plain path: deposit into the global histogram (no LDS copy)
   const BatchJob* jobs = nullptr;
   const BatchChunk* chunks = nullptr;
   int nchunks = 0;
@@ -251,12 +265,21 @@ inline PairShape pair_shape(int nflat, int bins, bool weighted, bool plain, int 
   sh.per_cu = per_cu;
   return sh;
 }
+// Whether the mixed (FP32 + FP64 recheck) pair kernel serves a histogram: its
+// u8 table codes, exact thresholds and LDS histogram must fit.  Other shapes
+// run the plain FP64 kernel, which gives the same results.
+inline bool pair_mixed_fits(int bins, bool ani, bool weighted) {
+  if (bins <= 0 || bins > (ani ? kMaxBinsAniMixed : kMaxBinsIsoMixed)) return false;
+  const int nflat = ani ? bins * bins : bins;
+  return pair_shape(nflat, bins, weighted, false, 0, 0).lds_bytes <= kLdsPerCu;
+}
 hipError_t launch_pairs(const PairLaunch& pl, hipStream_t s);
 hipError_t launch_scale_points(const float4* in, int64_t n, float sigma, float4* out,
                                hipStream_t s);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
-// Exclusive prefix sum over the device: out[0..n) and the total in out[n]
-// (totals below 2^31); tsum holds scan_tiles(n) tile sums.  out may alias in.
+// Exclusive prefix sum over the device: out[0..n) and the total in out[n];
+// a total of 2^31 or more is written as -1 (and out[0..n) is then not usable).
+// tsum holds scan_tiles(n) tile sums.  out may alias in.
 inline int64_t scan_tiles(int64_t n) { return (n + 4095) / 4096 + 1; }
 hipError_t launch_scan(const int* in, int64_t n, int* out, long long* tsum, hipStream_t s);
 
@@ -284,5 +307,10 @@ int64_t worklist_bound(const WorkListArgs& a, int64_t n_tile_points);
 // cap were short; the kernels clamp the count to cap and report it.
 hipError_t build_pair_worklist_async(const WorkListArgs& a, int64_t cap, DevBuf& counts,
                                      DevBuf& items, const int** total_dev, hipStream_t s);
+// The same into caller-provided memory: `counts` of worklist_count_bytes(a)
+// bytes and `items` of cap entries.
+size_t worklist_count_bytes(const WorkListArgs& a);
+hipError_t build_pair_worklist_into(const WorkListArgs& a, int64_t cap, void* counts,
+                                    PairItem* items, const int** total_dev, hipStream_t s);
 
 }  // namespace pcf

@@ -140,3 +140,16 @@ def test_3ani_full_5d_file(tmp_path):
     c, _ = O.triplets(d, None, ani=True, bpc=True, box=250.0, bins=5, dmax=40.0)
     assert np.array_equal(full, c.astype(np.float64))
     assert np.array_equal(slab, full[:5 ** 3])
+
+
+def test_2ani_bpc_2dmax_ge_box_files(tmp_path):
+    """`PCF 2ani -bpc -d 150 -s 250 -b 50` (the reference CPU program's
+    configuration, 2*dmax > box) on the full sample catalogs: the written
+    DD/RR/DR equal the oracle's (tests/golden/oracle_out)."""
+    res = _run(tmp_path, "2ani", "-n", "32768", "-f", "data.dat", "-r", "rand0.dat",
+               "-b", "50", "-d", "150", "-bpc", "-s", "250")
+    fix = os.path.join(GOLDEN, "oracle_out")
+    for name, ref in (("DDani_BPC_data.dat", "DD"), ("RRani_BPC_rand0.dat", "RR"),
+                      ("DRani_BPC_rand0.dat", "DR")):
+        want = np.loadtxt(os.path.join(fix, f"2ani_bpc_d150_s250_b50_{ref}.txt"))
+        assert np.array_equal(_load(res / name), want), name

@@ -104,8 +104,8 @@ def test_poll_wait_errors_and_release(served):
     c, w, st = e.wait(t, j.type, j.bins)
     with pytest.raises(da.PcfError):
         e.poll(t)  # collected
-    bad = da.make_job(da.PCF_2ISO, da.PCF_XX, bins=20, dmax=130.0, cat_x=ids["u4k"], bpc=True,
-                      box=250.0)  # 2*dmax >= box
+    bad = da.make_job(da.PCF_2ISO, da.PCF_XX, bins=20, dmax=260.0, cat_x=ids["u4k"], bpc=True,
+                      box=250.0)  # dmax >= box
     tb = e.submit(bad)
     with pytest.raises(da.PcfError):
         e.wait(tb, bad.type, bad.bins)
