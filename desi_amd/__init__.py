@@ -36,7 +36,10 @@ EXPORTED_SYMBOLS = (
     "pcf_catalog_size", "pcf_histogram", "pcf_last_stats", "pcf_analytic_rr",
     "pcf_analytic_3iso", "pcf_histogram_batch", "pcf_autotune", "pcf_set_max_batch",
     "pcf_set_hbm_budget", "pcf_submit", "pcf_poll", "pcf_wait", "pcf_estimator",
+    "pcf_histogram_noalloc",
 )
+# Bytes of the static device array pcf_histogram_noalloc works in (per GPU).
+PCF_NOALLOC_SCRATCH_BYTES = 128 * 1024 * 1024
 
 
 class PcfJob(ctypes.Structure):
@@ -105,6 +108,7 @@ def load_library(path=LIB_PATH):
     L.pcf_catalog_size.argtypes = [vp, c_i32]
     L.pcf_catalog_size.restype = c_i64
     L.pcf_histogram.argtypes = [vp, ctypes.POINTER(PcfJob), vp, vp]
+    L.pcf_histogram_noalloc.argtypes = [vp, ctypes.POINTER(PcfJob), vp, vp]
     L.pcf_last_stats.argtypes = [vp, ctypes.POINTER(PcfStats)]
     L.pcf_analytic_rr.argtypes = [vp, c_i32, c_i64, ctypes.c_float, ctypes.c_float, vp]
     L.pcf_histogram_batch.argtypes = [vp, ctypes.POINTER(PcfJob), c_i32, vp, vp]
@@ -198,6 +202,16 @@ class Engine:
         w = np.zeros(n, np.float64)
         self._check(self.lib.pcf_histogram(self.ctx, ctypes.byref(job), c.ctypes.data,
                                            w.ctypes.data))
+        return c, w
+
+    def histogram_noalloc(self, job):
+        """pcf_histogram_noalloc: the same pair histogram with no device
+        allocation (buffers in the library's static scratch array)."""
+        n = hist_size(job.type, job.bins)
+        c = np.zeros(n, np.uint64)
+        w = np.zeros(n, np.float64)
+        self._check(self.lib.pcf_histogram_noalloc(self.ctx, ctypes.byref(job), c.ctypes.data,
+                                                   w.ctypes.data))
         return c, w
 
     def stats(self):

@@ -836,6 +836,8 @@ int pcf_create(pcf_ctx** out, int32_t device) {
   hipDeviceProp_t prop;
   if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) goto fail;
   c->ncu = prop.multiProcessorCount;
+  // The no-allocation entry point's scratch array on this device.
+  if ((e = resolve_noalloc_scratch(device)) != hipSuccess) goto fail;
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) goto fail;
   if ((e = hipStreamCreateWithFlags(&c->grid_stream, hipStreamNonBlocking)) != hipSuccess)
     goto fail;

@@ -130,6 +130,10 @@ void free_catalog(Catalog* cat);
 int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights, void* arena,
               size_t arena_bytes);
 
+// Defined in noalloc.hip: device address of this GPU's scratch array of
+// pcf_histogram_noalloc (once per device, with the device current).
+hipError_t resolve_noalloc_scratch(int device);
+
 // Defined in triplets.hip.
 int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights);
 

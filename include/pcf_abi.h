@@ -169,6 +169,22 @@ int pcf_histogram(pcf_ctx* ctx, const pcf_job* job, uint64_t* counts, double* we
 int pcf_histogram_batch(pcf_ctx* ctx, const pcf_job* jobs, int32_t njobs, uint64_t* counts,
                         double* weights);
 
+/* pcf_histogram for pair jobs (PCF_2ISO / PCF_2ANI, PCF_XX / PCF_XY) that
+ * allocates no device memory.  No counterpart in the reference: it is for
+ * callers that cannot allocate (tight serving loops, harnesses that forbid
+ * hipMalloc).  Inputs are resident catalogs and outputs the caller's host
+ * arrays, as for pcf_histogram, with the same results; every device buffer of
+ * the call (grids, scaled copies, work list, bin table, device histogram) is
+ * carved from one static device array of PCF_NOALLOC_SCRATCH_BYTES per GPU,
+ * held by each GPU the library runs on until the process ends.  A job whose
+ * buffers exceed it fails with PCF_ERR_NOMEM and a message giving both sizes
+ * (about 40 bytes per catalog point plus the work list: two catalogs of about
+ * 1.5 million points).  Calls on one GPU share its array and would race, so
+ * the library serialises them per device (a lock held for the whole call).
+ * Grids are not kept between calls. */
+#define PCF_NOALLOC_SCRATCH_BYTES (128LL * 1024 * 1024)
+int pcf_histogram_noalloc(pcf_ctx* ctx, const pcf_job* job, uint64_t* counts, double* weights);
+
 /* Autotuner of the pair kernels (no counterpart in the reference, whose
  * launch shape is fixed: 8x8 threads per block, one thread per node pair,
  * cuda/pcf2iso.cuh:74-80, and whose -p partitions flag sets the node size,
