@@ -3,10 +3,16 @@
 
 Workload (the north-star 2PCF shape): 2iso, data and random catalogs of
 N = 1,048,576 points each, uniform FP32 in a box of side 1000 (seeded), rmax
-80, 30 bins, no periodic boundaries.  One step = the hot path over one batch:
-the device grid build of both catalogs (counting sort into cell order) plus the
-DD, RR and DR histograms in the FP32 + FP64-edge-recheck form.  Inputs are
-resident in HBM before the timed region (catalog upload is not timed).
+80, 30 bins, no periodic boundaries.  One step = the hot path over one batch,
+in the batched form: one pcf_histogram_batch call (a replayed HIP graph) that
+re-runs the device grid build of both catalogs (counting sort into cell
+order) and then the DD, RR and DR histograms in one persistent launch of the
+FP32 + FP64-edge-recheck kernel.  Inputs are resident in HBM before the timed
+region (catalog upload is not timed).
+
+Every leg checks its mixed-precision histograms bin by bin against the plain
+FP64 GPU path (counts equal, weighted sums within 1e-12 relative) and reports
+the largest relative error of the weighted sums per histogram.
 
 value = useful pairs per second (sum of the three unweighted histograms divided
 by the wall time of the step), the reference's "pairs/s".  The pair kernel is
@@ -58,6 +64,42 @@ def profiled_traffic(path, kernel):
     return None
 
 
+WREL = 1e-12  # relative tolerance of weighted sums against the plain FP64 path
+
+
+def compare(name, mixed, plain):
+    """Per-bin parity of one histogram: counts equal, weights' largest
+    relative error.  Raises on a count mismatch or an error above WREL."""
+    (c, w), (c64, w64) = mixed, plain
+    if not np.array_equal(c, c64):
+        bad = np.nonzero(c != c64)[0]
+        raise SystemExit(f"{name}: mixed and plain FP64 counts differ in {len(bad)} bins "
+                         f"(first {bad[:5].tolist()})")
+    den = np.maximum(np.abs(w64), 1e-300)
+    err = float(np.max(np.abs(w - w64) / den)) if len(w) else 0.0
+    if err > WREL:
+        raise SystemExit(f"{name}: weighted sums off by {err:.3e} relative")
+    return {"bins": int(len(c)), "counts_equal": True, "max_rel_err_w": err,
+            "total": int(c.sum())}
+
+
+def ns_jobs(cd, cr, precision=da.PCF_PREC_MIXED, rebuild=True):
+    """DD, RR and DR of the NS step; rebuild_grid makes every launch re-run the
+    device grid build of both catalogs."""
+    return [da.make_job(da.PCF_2ISO, da.PCF_XX, bins=BINS, dmax=RMAX, cat_x=cd,
+                        precision=precision, rebuild_grid=rebuild),
+            da.make_job(da.PCF_2ISO, da.PCF_XX, bins=BINS, dmax=RMAX, cat_x=cr,
+                        precision=precision, rebuild_grid=rebuild),
+            da.make_job(da.PCF_2ISO, da.PCF_XY, bins=BINS, dmax=RMAX, cat_x=cd, cat_y=cr,
+                        precision=precision, rebuild_grid=rebuild)]
+
+
+def run_batched_step(e, jobs):
+    res = e.histogram_batch(jobs)
+    st = e.stats()
+    return res, st
+
+
 def run_step(e, cd, cr, precision, rebuild=True):
     tot_pairs = 0
     evals = 0
@@ -80,21 +122,34 @@ def run_step(e, cd, cr, precision, rebuild=True):
     return tot_pairs, evals, kms, rech
 
 
+def run_step_hists(e, cd, cr, precision):
+    """The NS histograms as separate calls: [(counts, weights)] for DD, RR, DR."""
+    out = []
+    for j in ns_jobs(cd, cr, precision, rebuild=False):
+        out.append(e.histogram(j))
+    return out
+
+
+TRIP_NAMES = ("DDD", "DDR", "DRR", "RRR")
+
+
 def run_triplet_step(e, cd, cr, precision, rebuild=True):
     """Config 2: 3iso with periodic boundaries, DDD + DDR + DRR + RRR."""
     tot = 0
     evals = 0
     kms = 0.0
+    hists = []
     for hist, x, y in ((da.PCF_XXX, cd, -1), (da.PCF_XXY, cd, cr), (da.PCF_XXY, cr, cd),
                        (da.PCF_XXX, cr, -1)):
         job = da.make_job(da.PCF_3ISO, hist, bins=30, dmax=80.0, cat_x=x, cat_y=y, bpc=True,
                           box=250.0, precision=precision, rebuild_grid=rebuild)
-        c, _ = e.histogram(job)
+        c, w = e.histogram(job)
+        hists.append((c, w))
         st = e.stats()
         tot += int(c.sum())
         evals += st["evaluations"]
         kms += st["kernel_ms"]
-    return tot, evals, kms
+    return tot, evals, kms, hists
 
 
 def triplet_ani_leg(e):
@@ -105,15 +160,18 @@ def triplet_ani_leg(e):
     r = synth.uniform(65536, 500.0, seed=42, weights="uniform")
     cd, cr = e.upload(d), e.upload(r)
     out = {}
+    hists = {}
     for prec, name in ((da.PCF_PREC_MIXED, "mixed"), (da.PCF_PREC_FP64, "fp64_plain")):
         tot = evals = 0
         kms = 0.0
+        hists[name] = []
         t0 = time.perf_counter()
         for hist, x, y in ((da.PCF_XXX, cd, -1), (da.PCF_XXY, cd, cr), (da.PCF_XXY, cr, cd),
                            (da.PCF_XXX, cr, -1)):
             job = da.make_job(da.PCF_3ANI, hist, bins=15, dmax=60.0, cat_x=x, cat_y=y, bpc=True,
                               box=500.0, precision=prec)
-            c, _ = e.histogram(job)
+            c, w = e.histogram(job)
+            hists[name].append((c, w))
             st = e.stats()
             tot += int(c.sum())
             evals += st["evaluations"]
@@ -123,8 +181,8 @@ def triplet_ani_leg(e):
                      "evaluations": evals}
     e.release(cd)
     e.release(cr)
-    if out["mixed"]["triplets"] != out["fp64_plain"]["triplets"]:
-        raise SystemExit("plain FP64 and mixed 3ani paths disagree on the count")
+    parity = {n: compare(f"3ani {n}", hm, hp)
+              for n, hm, hp in zip(TRIP_NAMES, hists["mixed"], hists["fp64_plain"])}
     m = out["mixed"]
     ach = m["evaluations"] * (FLOPS_PER_TRIPLET + 2) / (m["kernel_ms"] * 1e-3)
     return {"workload": "3ani periodic box 500, N=65536 Neyman-Scott (2048x32, sigma 4) + 65536 "
@@ -134,6 +192,7 @@ def triplet_ani_leg(e):
             "evaluations_per_step": m["evaluations"],
             "fp64_plain_ms_per_step": out["fp64_plain"]["ms"],
             "speedup_vs_fp64_plain": out["fp64_plain"]["ms"] / m["ms"],
+            "parity_vs_fp64": parity,
             "roofline": {"bound": "valu", "achieved": ach / 1e12, "peak": PEAK_FP32 / 1e12,
                          "unit": "TFLOP/s", "frac": ach / PEAK_FP32, "traffic": None}}
 
@@ -147,15 +206,14 @@ def triplet_leg(e, steps):
     tot = evals = 0
     kms = 0.0
     for _ in range(steps):
-        t, ev, k = run_triplet_step(e, cd, cr, da.PCF_PREC_MIXED)
+        t, ev, k, hm = run_triplet_step(e, cd, cr, da.PCF_PREC_MIXED)
         tot += t
         evals += ev
         kms += k
     t1 = time.perf_counter()
-    t64, _, k64 = run_triplet_step(e, cd, cr, da.PCF_PREC_FP64)
+    t64, _, k64, hp = run_triplet_step(e, cd, cr, da.PCF_PREC_FP64)
     t2 = time.perf_counter()
-    if t64 != tot // steps:
-        raise SystemExit("plain FP64 and mixed triplet paths disagree on the count")
+    parity = {n: compare(f"3iso {n}", a, b) for n, a, b in zip(TRIP_NAMES, hm, hp)}
     e.release(cd)
     e.release(cr)
     ms = (t1 - t0) * 1e3 / steps
@@ -170,6 +228,7 @@ def triplet_leg(e, steps):
         "fp64_plain_ms_per_step": (t2 - t1) * 1e3,
         "fp64_plain_kernel_ms_per_step": k64,
         "speedup_vs_fp64_plain": (t2 - t1) * 1e3 / ms,
+        "parity_vs_fp64": parity,
         # the 2-3 side of every triplet is formed on the matrix cores
         "roofline": {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": ach / PEAK_FP32, "traffic": None},
@@ -215,6 +274,14 @@ def batch_leg(e, steps, no_tune=False):
         if not np.array_equal(c, res[k][0]):
             raise SystemExit(f"batched and single histograms differ (job {k})")
     t4 = time.perf_counter()
+    # Every histogram of the batch against the plain FP64 path, bin by bin.
+    worst = 0.0
+    for k, j in enumerate(jobs):
+        j64 = da.make_job(j.type, j.hist, bins=j.bins, dmax=j.dmax, cat_x=j.cat_x,
+                          cat_y=j.cat_y, precision=da.PCF_PREC_FP64)
+        name = "DD" if k == 0 else ("RR%d" if j.hist == da.PCF_XX else "DR%d") % ((k - 1) // 2)
+        worst = max(worst, compare(f"2ani batch {name}", res[k], e.histogram(j64))["max_rel_err_w"])
+    parity = {"histograms": len(jobs), "counts_equal": True, "max_rel_err_w": worst}
     e.release(cd)
     for cr in crs:
         e.release(cr)
@@ -229,6 +296,7 @@ def batch_leg(e, steps, no_tune=False):
             "work_items": first["work_items"],
             "unbatched_ms": (t4 - t3) * 1e3, "unbatched_kernel_ms": single_ms,
             "speedup_vs_unbatched": (t4 - t3) * 1e3 / ms,
+            "parity_vs_fp64": parity,
             "roofline": {"bound": "valu", "achieved": ach / 1e12, "peak": PEAK_FP32 / 1e12,
                          "unit": "TFLOP/s", "frac": ach / PEAK_FP32, "traffic": None}}
 
@@ -344,25 +412,33 @@ def copy_probe():
           flush=True)
 
 
-def cpu_baseline(d):
-    """The oracle (a C++/OpenMP restatement, FP64 canonical mode) on a bounded
-    sample: the DD histogram of the data points inside the sub-box [0,500)^3."""
+def cpu_baseline(d, r):
+    """The oracle (a C++/OpenMP restatement of the reference's algorithm) timed
+    on the host cores, on the same NS step the GPU runs (DD + RR + DR of the
+    two 1,048,576-point catalogs): in its FP64 canonical mode (the GPU path's
+    arithmetic) and in its cpp32 mode (the reference CPU programs' FP32
+    arithmetic and unordered-pair loop, cpp/2PCFisotropic/normal/2PCF.h:151-305).
+    Threads: the box's CPU share (16 for one GPU) or fewer; the reference CPU
+    programs' hard-wired 2 threads are timed on the DD of the data points in
+    [0,500)^3."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
+    nproc = os.cpu_count() or 1
+    cores = min(nproc, 16)
+    t = time.perf_counter()
+    tot = 0
+    for a, b in ((d, None), (r, None), (d, r)):
+        c, _ = O.pairs(a, b, bins=BINS, dmax=RMAX, threads=cores)
+        tot += int(c.sum())
+    dt = time.perf_counter() - t
     sel = np.all(d[:, :3] < 500.0, axis=1)
     sample = np.ascontiguousarray(d[sel])
-    cores = os.cpu_count() or 1
-    cores = min(cores, 16)
     t = time.perf_counter()
-    c, _ = O.pairs(sample, None, bins=BINS, dmax=RMAX, threads=cores)
-    dt = time.perf_counter() - t
-    # The reference CPU programs run at a hard-wired 2 threads; the same sample
-    # at 2 threads is reported beside the full-core figure.
+    tot32 = float(O.cpp32_2iso(sample, None, bins=BINS, dmax=RMAX, threads=cores).sum())
+    dt32 = time.perf_counter() - t
     t = time.perf_counter()
     c2, _ = O.pairs(sample, None, bins=BINS, dmax=RMAX, threads=2)
     dt2 = time.perf_counter() - t
-    if not np.array_equal(c, c2):
-        raise SystemExit("oracle results differ between thread counts")
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -372,10 +448,21 @@ def cpu_baseline(d):
                     break
     except OSError:
         pass
-    return {"value": float(c.sum()) / dt, "unit": "pairs/s", "cores": cores, "kind": "port",
-            "sample": f"oracle FP64 DD of the {len(sample)} data points in [0,500)^3 "
-                      f"(rmax {RMAX}, {BINS} bins), {dt:.2f} s",
-            "value_2_threads": float(c2.sum()) / dt2, "cpu_model": model}
+    return {"value": tot / dt, "unit": "pairs/s", "cores": cores, "kind": "port",
+            "sample": f"the full NS step (DD + RR + DR, 2 x 1048576 points, rmax {RMAX}, "
+                      f"{BINS} bins) in the oracle's FP64 grid mode: {dt:.2f} s",
+            "algorithm": "oracle/pcf_oracle.cpp cell grid, ordered pairs, FP64 distances",
+            "cpp32_value": tot32 / dt32,
+            "cpp32_algorithm": "oracle cpp32 mode: the reference CPU program's FP32 "
+                               "arithmetic and unordered-pair loop, same cell grid",
+            "cpp32_sample": f"DD of the {len(sample)} data points in [0,500)^3, "
+                            f"{cores} threads, {dt32:.2f} s",
+            "nproc": nproc,
+            "threads_note": "threads capped at the GPU box's CPU share of 16",
+            "value_2_threads": float(c2.sum()) / dt2,
+            "sample_2_threads": f"oracle FP64 DD of the {len(sample)} data points in "
+                                f"[0,500)^3, {dt2:.2f} s",
+            "cpu_model": model}
 
 
 def main():
@@ -413,30 +500,27 @@ def main():
     r = synth.uniform(N, BOX, seed=2)
     with da.Engine(0) as e:
         cd, cr = e.upload(d), e.upload(r)
-        tuning = None
-        if not args.no_tune:
-            # One-time per workload (cached on the context, like the resident
-            # grids): cell size, LDS layout and occupancy of the pair kernel.
-            tuning = {}
-            for name, hist, y in (("XX", da.PCF_XX, -1), ("XY", da.PCF_XY, cr)):
-                tuning[name] = e.autotune(da.make_job(da.PCF_2ISO, hist, bins=BINS, dmax=RMAX,
-                                                      cat_x=cd, cat_y=y))
+        jobs = ns_jobs(cd, cr)
+        # One-time per workload (cached on the context, like the resident
+        # grids): pair kernel form, cell size, LDS layout and occupancy.
+        tuning = None if args.no_tune else e.autotune(jobs)
         for _ in range(args.warmup):
-            run_step(e, cd, cr, da.PCF_PREC_MIXED)
+            run_batched_step(e, jobs)
         t0 = time.perf_counter()
-        pairs = evals = rech = 0
+        pairs = evals = rech = replays = 0
         kms = 0.0
         for _ in range(args.steps):
-            p, ev, k, rc = run_step(e, cd, cr, da.PCF_PREC_MIXED)
-            pairs += p
-            evals += ev
-            kms += k
-            rech += rc
+            res, st = run_batched_step(e, jobs)
+            pairs += sum(int(c.sum()) for c, _ in res)
+            evals += st["evaluations"]
+            kms += st["kernel_ms"]
+            rech += st["rechecks"]
+            replays += st["graph_replay"]
         t1 = time.perf_counter()
         ms_step = (t1 - t0) * 1e3 / args.steps
         value = pairs / (t1 - t0)
 
-        # Plain FP64 path, same step.
+        # Plain FP64 path, same step (grid build + DD + RR + DR).
         run_step(e, cd, cr, da.PCF_PREC_FP64)
         t2 = time.perf_counter()
         p64 = 0
@@ -447,8 +531,9 @@ def main():
             k64 += k
         t3 = time.perf_counter()
         ms_step64 = (t3 - t2) * 1e3 / args.fp64_steps
-        if p64 // args.fp64_steps != pairs // args.steps:
-            raise SystemExit("plain FP64 path and mixed path disagree on the pair count")
+        ns_parity = {n: compare(f"NS {n}", a, b)
+                     for n, a, b in zip(("DD", "RR", "DR"), res,
+                                        run_step_hists(e, cd, cr, da.PCF_PREC_FP64))}
         trip = None if args.no_triplets or args.no_3iso else triplet_leg(e, args.triplet_steps)
         trip_ani = None if args.no_triplets or args.no_3ani else triplet_ani_leg(e)
         ana = None if args.no_analytic else analytic_leg(e)
@@ -460,10 +545,7 @@ def main():
     # The dominant kernel: the Gram form forms d2 with four chained FMAs on the
     # matrix cores (FP32 MFMA peak 157.3 TFLOP/s, the same figure as the packed
     # FP32 vector peak); the direct form runs on the vector units.
-    kern = None
-    if tuning is not None:
-        ks = {t.get("pair_kernel") for t in tuning.values()}
-        kern = ks.pop() if len(ks) == 1 else None
+    kern = None if tuning is None else tuning.get("pair_kernel")
     bound = "mfma" if kern == da.PCF_PAIR_GRAM else "valu"
     prof = PAIR_PROFILE.get(kern if kern in PAIR_PROFILE else da.PCF_PAIR_GRAM)
     out = {
@@ -480,7 +562,8 @@ def main():
         "dtype": "f32+f64-recheck",
         "data": "synthetic (uniform, seeded)",
         "config": {"workload": "2iso N=1048576 data + 1048576 random, box 1000, rmax 80, "
-                               "30 bins, DD+RR+DR, no BPC",
+                               "30 bins, DD+RR+DR in one batched launch (grid build "
+                               "included), no BPC",
                    "n_data": N, "n_random": N, "box": BOX, "rmax": RMAX, "bins": BINS},
         "roofline": {"bound": bound, "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
@@ -489,6 +572,8 @@ def main():
         "evaluations_per_step": evals // args.steps,
         "candidates_per_useful_pair": evals / max(pairs, 1),
         "rechecks_per_step": rech // args.steps,
+        "graph_replays": replays,
+        "parity_vs_fp64": ns_parity,
         "fp64_plain_ms_per_step": ms_step64,
         "fp64_plain_kernel_ms_per_step": k64 / args.fp64_steps,
         "speedup_vs_fp64_plain": ms_step64 / ms_step,
@@ -506,7 +591,7 @@ def main():
     if serving is not None:
         out["serving"] = serving
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(d)
+        out["cpu_baseline"] = cpu_baseline(d, r)
     print(json.dumps(out), flush=True)
 
 

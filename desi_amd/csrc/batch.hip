@@ -43,19 +43,34 @@ struct BatchPlan {
   std::vector<int> job_offsets;
   std::vector<BatchChunk> chunks;
   std::vector<int> mult;  // 2 for auto histograms (unordered pairs counted once)
+  // jobs with rebuild_grid: the launch re-runs the device counting sort of
+  // every catalog into its resident grid first (same data, same cell counts,
+  // so the work lists stay valid; only the order inside a cell may change)
+  bool regrid = false;
+  std::vector<const float4*> regrid_raw;
+  std::vector<int64_t> regrid_n;
+  DevBuf regrid_cells, regrid_perm;
   // scaled copies: one per distinct catalog
   std::vector<const float4*> scale_src;
   std::vector<int64_t> scale_n;
   std::vector<int64_t> scale_off;
   // device memory owned by the plan
   DevBuf counts, items, jobs_d, chunks_d, table_d, rules_d, hist_d, scaled;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // Two captured graphs: the preparation (re-gridding, scaled copies,
+  // histogram clear) and the counting kernel, launched back to back with an
+  // event between them, so the kernel's time is measured on its own.
+  hipGraph_t graph_prep = nullptr, graph = nullptr;
+  hipGraphExec_t exec_prep = nullptr, exec = nullptr;
+  hipEvent_t kev = nullptr;
   uint64_t evals_per_run = 0;
   ~BatchPlan() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    for (DevBuf* b : {&counts, &items, &jobs_d, &chunks_d, &table_d, &rules_d, &hist_d, &scaled})
+    if (exec_prep) (void)hipGraphExecDestroy(exec_prep);
+    if (graph_prep) (void)hipGraphDestroy(graph_prep);
+    if (kev) (void)hipEventDestroy(kev);
+    for (DevBuf* b : {&counts, &items, &jobs_d, &chunks_d, &table_d, &rules_d, &hist_d, &scaled,
+                      &regrid_cells, &regrid_perm})
       b->release();
   }
 };
@@ -69,17 +84,27 @@ bool same_job(const pcf_job& a, const pcf_job& b) {
          a.dmax == b.dmax && a.box == b.box && a.precision == b.precision &&
          a.cat_x == b.cat_x && a.cat_y == b.cat_y && a.cells_per_rmax == b.cells_per_rmax &&
          a.lds_copies_log2 == b.lds_copies_log2 && a.blocks_per_cu == b.blocks_per_cu &&
-         a.pair_kernel == b.pair_kernel;
+         a.pair_kernel == b.pair_kernel && (a.rebuild_grid != 0) == (b.rebuild_grid != 0);
 }
 
-// Graph body: scaled copies, histogram + counter clear, the batched kernel.
-// prepare: only set the kernel attributes (not allowed during capture).
-hipError_t enqueue_plan(pcf_ctx* c, BatchPlan* p, size_t njobs, bool prepare) {
+// Graph bodies.  Part 0 (preparation): with regrid the counting sort of every
+// catalog, then the scaled copies and the histogram + counter clear.  Part 1:
+// the batched kernel.  prepare: only set the kernel attributes (not allowed
+// during capture).
+hipError_t enqueue_plan(pcf_ctx* c, BatchPlan* p, size_t njobs, bool prepare, bool regrid,
+                        int part) {
   hipError_t e;
   float4* sc = p->scaled.as<float4>();
   const size_t hist_bytes = sizeof(unsigned long long) * p->nflat * njobs;
   const size_t w_bytes = sizeof(double) * p->nflat * njobs;
-  if (!prepare) {
+  if (part == 0 && !prepare && regrid) {
+    for (size_t k = 0; k < p->grids.size(); ++k)
+      if ((e = build_grid_into(p->regrid_raw[k], p->regrid_n[k], p->g, p->grids[k]->pts,
+                               p->grids[k]->cell_start, p->regrid_cells.p,
+                               p->regrid_perm.as<int>(), c->stream)) != hipSuccess)
+        return e;
+  }
+  if (part == 0 && !prepare) {
     for (size_t k = 0; k < p->scale_src.size(); ++k)
       if ((e = launch_scale_points(p->scale_src[k], p->scale_n[k], p->sigma,
                                    sc + p->scale_off[k], c->stream)) != hipSuccess)
@@ -119,7 +144,7 @@ hipError_t enqueue_plan(pcf_ctx* c, BatchPlan* p, size_t njobs, bool prepare) {
   pl.chunks = p->chunks_d.as<BatchChunk>();
   pl.nchunks = (int)p->chunks.size();
   pl.prepare_only = prepare ? 1 : 0;
-  if (pl.nchunks == 0) return hipSuccess;
+  if (part == 0 || pl.nchunks == 0) return hipSuccess;
   return launch_pairs(pl, c->stream);
 }
 
@@ -284,31 +309,50 @@ int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p, GridPins
   p->serials.clear();
   p->grid_pts.clear();
   p->grids.clear();
+  p->regrid_raw.clear();
+  p->regrid_n.clear();
+  int64_t nmax = 1;
   for (pcf_catalog id : ids) {
-    p->serials.push_back(get_cat(c, id)->serial);
+    Catalog* X = get_cat(c, id);
+    p->serials.push_back(X->serial);
     p->grid_pts.push_back(grid_of[id]->pts);
     p->grids.push_back(grid_of[id]);
+    p->regrid_raw.push_back(X->raw);
+    p->regrid_n.push_back(X->n);
+    nmax = std::max(nmax, X->n);
+  }
+  p->regrid = false;
+  for (int k = 0; k < njobs; ++k) p->regrid = p->regrid || jobs[k].rebuild_grid != 0;
+  if (p->regrid) {
+    HIPCHK(c, p->regrid_cells.ensure(grid_cells_tmp_bytes(p->g.ncell())));
+    HIPCHK(c, p->regrid_perm.ensure(2 * sizeof(int) * (size_t)nmax));
   }
 
   // Capture the run into a graph (serving lanes run their batches directly:
   // their job sets rarely repeat).
-  if (p->exec) (void)hipGraphExecDestroy(p->exec);
-  if (p->graph) (void)hipGraphDestroy(p->graph);
-  p->exec = nullptr;
-  p->graph = nullptr;
-  HIPCHK(c, enqueue_plan(c, p, (size_t)njobs, true));
+  for (hipGraphExec_t* x : {&p->exec, &p->exec_prep})
+    if (*x) (void)hipGraphExecDestroy(*x);
+  for (hipGraph_t* g : {&p->graph, &p->graph_prep})
+    if (*g) (void)hipGraphDestroy(*g);
+  p->exec = p->exec_prep = nullptr;
+  p->graph = p->graph_prep = nullptr;
+  if (!p->kev) HIPCHK(c, hipEventCreate(&p->kev));
+  HIPCHK(c, enqueue_plan(c, p, (size_t)njobs, true, false, 1));
   if (c->lane) return PCF_OK;
-  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  hipError_t e = enqueue_plan(c, p, (size_t)njobs, false);
-  hipGraph_t graph = nullptr;
-  hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
-  if (e != hipSuccess || e2 != hipSuccess) {
-    if (graph) (void)hipGraphDestroy(graph);
-    (void)hipGetLastError();
-    return hipfail(c, e != hipSuccess ? e : e2, "batch graph capture");
+  for (int part = 0; part < 2; ++part) {
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueue_plan(c, p, (size_t)njobs, false, p->regrid, part);
+    hipGraph_t graph = nullptr;
+    hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      (void)hipGetLastError();
+      return hipfail(c, e != hipSuccess ? e : e2, "batch graph capture");
+    }
+    (part == 0 ? p->graph_prep : p->graph) = graph;
+    HIPCHK(c, hipGraphInstantiate(part == 0 ? &p->exec_prep : &p->exec, graph, nullptr, nullptr,
+                                  0));
   }
-  p->graph = graph;
-  HIPCHK(c, hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
   return PCF_OK;
 }
 
@@ -318,7 +362,7 @@ int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p, GridPins
 bool plan_valid(pcf_ctx* c, const BatchPlan* p, const pcf_job* jobs, int njobs, GridPins& pins) {
   if (!p || !p->exec || (int)p->jobs.size() != njobs) return false;
   for (int k = 0; k < njobs; ++k) {
-    if (!same_job(p->jobs[k], jobs[k]) || jobs[k].rebuild_grid) return false;
+    if (!same_job(p->jobs[k], jobs[k])) return false;
   }
   std::vector<pcf_catalog> ids;
   for (int k = 0; k < njobs; ++k) {
@@ -428,9 +472,25 @@ int run_pairs_batch(pcf_ctx* c, const pcf_job* jobs_in, int njobs, uint64_t* cou
     }
   }
   BatchPlan* p = c->batch;
+  // Re-gridding rewrites resident grids in place: only while no other job
+  // (a serving lane) holds them; otherwise the grids, whose contents would be
+  // the same, are used as they are.
+  bool shared = false;
+  if (p->regrid) {
+    std::lock_guard<std::mutex> lk(c->store->mu);
+    for (DevGrid* g : p->grids) shared = shared || g->users > 1;
+  }
+  // ev[1] .. kev: preparation; kev .. ev[2]: the counting kernel.
   HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-  if (p->exec) HIPCHK(c, hipGraphLaunch(p->exec, c->stream));
-  else HIPCHK(c, enqueue_plan(c, p, (size_t)njobs, false));
+  if (p->exec && !(p->regrid && shared)) {
+    HIPCHK(c, hipGraphLaunch(p->exec_prep, c->stream));
+    HIPCHK(c, hipEventRecord(p->kev, c->stream));
+    HIPCHK(c, hipGraphLaunch(p->exec, c->stream));
+  } else {
+    HIPCHK(c, enqueue_plan(c, p, (size_t)njobs, false, p->regrid && !shared, 0));
+    HIPCHK(c, hipEventRecord(p->kev, c->stream));
+    HIPCHK(c, enqueue_plan(c, p, (size_t)njobs, false, false, 1));
+  }
   HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
   const size_t nf = (size_t)p->nflat * njobs;
   std::vector<unsigned long long> hc(nf + 8);
@@ -458,7 +518,7 @@ int run_pairs_batch(pcf_ctx* c, const pcf_job* jobs_in, int njobs, uint64_t* cou
   st.rechecks = hc[nf + 1];
   st.work_items = (uint64_t)p->total_items;
   float ms = 0.f;
-  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));
+  HIPCHK(c, hipEventElapsedTime(&ms, p->kev, c->ev[2]));
   st.kernel_ms = ms;
   HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[3]));
   st.total_ms = ms;

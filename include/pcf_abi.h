@@ -165,7 +165,10 @@ int pcf_histogram(pcf_ctx* ctx, const pcf_job* job, uint64_t* counts, double* we
  * values (either may be NULL), each equal to what pcf_histogram returns for
  * that job (counts bit-exact, weights to 1e-12 relative).  The plan and its
  * HIP graph stay on the context: repeating the same batch over the same
- * resident catalogs replays the graph (pcf_last_stats: graph_replay, chunks). */
+ * resident catalogs replays the graph (pcf_last_stats: graph_replay, chunks).
+ * A batch whose jobs set rebuild_grid re-runs the device grid build (counting
+ * sort) of its catalogs at the start of every launch, replays included.
+ * kernel_ms is the counting kernel alone, total_ms the whole call. */
 int pcf_histogram_batch(pcf_ctx* ctx, const pcf_job* jobs, int32_t njobs, uint64_t* counts,
                         double* weights);
 
