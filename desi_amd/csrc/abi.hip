@@ -125,30 +125,33 @@ static double err_bound_gram(double d2, double A, double rho) {
 // coordinates by sigma = sqrt(inv_h) (a power of two, exact in FP32) and
 // convert d2 straight to a table index.  *nt_out <= nt_max entries cover
 // [0, Vmax) where everything beyond Vmax is certainly out of range.
-static int build_table_uncached(const BinRules& r, double A, double rho, int nt_max,
-                                std::vector<uint8_t>* table, float* inv_h_out, int* nt_out);
+static int build_table_uncached(const BinRules& r, double A, double rho, int nt_max, bool flip,
+                                std::vector<uint8_t>* table, float* inv_h_out, int* nt_out,
+                                float* top_out);
 
-// The table depends only on (bins, ani, R, ds, le, zero, A, nt_max) and takes
-// ~1 ms of host time to build: built tables are kept (process-wide, bounded).
+// The table depends only on (bins, ani, R, ds, le, zero, A, nt_max, rho,
+// flip) and takes ~1 ms of host time to build: built tables are kept
+// (process-wide, bounded).
 int build_table(const BinRules& r, double A, int nt_max, std::vector<uint8_t>* table,
-                float* inv_h_out, int* nt_out, double rho) {
+                float* inv_h_out, int* nt_out, double rho, bool flip, float* top_out) {
   struct Key {
-    int bins, ani, le, zero, nt_max;
+    int bins, ani, le, zero, nt_max, flip;
     double R, ds, A, rho;
     bool operator<(const Key& o) const {
-      return std::tie(bins, ani, le, zero, nt_max, R, ds, A, rho) <
-             std::tie(o.bins, o.ani, o.le, o.zero, o.nt_max, o.R, o.ds, o.A, o.rho);
+      return std::tie(bins, ani, le, zero, nt_max, flip, R, ds, A, rho) <
+             std::tie(o.bins, o.ani, o.le, o.zero, o.nt_max, o.flip, o.R, o.ds, o.A, o.rho);
     }
   };
   struct Val {
     std::vector<uint8_t> table;
     float inv_h;
     int nt;
+    float top;
     int rc;
   };
   static std::mutex mu;
   static std::map<Key, Val> cache;
-  const Key key{r.bins, r.ani, r.le, r.zero, nt_max, r.R, r.ds, A, rho};
+  const Key key{r.bins, r.ani, r.le, r.zero, nt_max, flip ? 1 : 0, r.R, r.ds, A, rho};
   {
     std::lock_guard<std::mutex> lk(mu);
     auto it = cache.find(key);
@@ -156,16 +159,19 @@ int build_table(const BinRules& r, double A, int nt_max, std::vector<uint8_t>* t
       *table = it->second.table;
       *inv_h_out = it->second.inv_h;
       *nt_out = it->second.nt;
+      if (top_out) *top_out = it->second.top;
       return it->second.rc;
     }
   }
   Val v;
   v.inv_h = 1.f;
   v.nt = 2;
-  v.rc = build_table_uncached(r, A, rho, nt_max, &v.table, &v.inv_h, &v.nt);
+  v.top = 0.f;
+  v.rc = build_table_uncached(r, A, rho, nt_max, flip, &v.table, &v.inv_h, &v.nt, &v.top);
   *table = v.table;
   *inv_h_out = v.inv_h;
   *nt_out = v.nt;
+  if (top_out) *top_out = v.top;
   const int rc = v.rc;
   std::lock_guard<std::mutex> lk(mu);
   if (cache.size() >= 64) cache.erase(cache.begin());
@@ -173,30 +179,79 @@ int build_table(const BinRules& r, double A, int nt_max, std::vector<uint8_t>* t
   return rc;
 }
 
-static int build_table_uncached(const BinRules& r, double A, double rho, int nt_max,
-                                std::vector<uint8_t>* table, float* inv_h_out, int* nt_out) {
+static int build_table_uncached(const BinRules& r, double A, double rho, int nt_max, bool flip,
+                                std::vector<uint8_t>* table, float* inv_h_out, int* nt_out,
+                                float* top_out) {
   const int bins = r.bins;
   const bool ani = r.ani != 0;
+  const double u = std::ldexp(1.0, -24);
+  // Flipped tables carry two more cells (see below).
+  const int budget = flip ? nt_max - 2 : nt_max;
   // Error margin of the paths sharing the table: the direct FP32 difference
-  // form, and with rho >= 0 also the Gram form.
+  // form, and with rho >= 0 also the Gram form.  Flipped, both also round
+  // Vtop - d2 (Vtop: the table top in unscaled units, set below): the direct
+  // form's final subtraction adds u (Vtop + d2); in the Gram form every
+  // partial sum of the product carries Vtop - |x_i|^2, so its four roundings
+  // and that of the accumulator's start add at most 5 u Vtop (both doubled).
+  double vtop = 0.0;
   auto err = [&](double d2) {
-    const double e = err_bound(d2, A);
-    return rho >= 0 ? std::max(e, err_bound_gram(d2, A, rho)) : e;
+    double e = err_bound(d2, A);
+    if (flip) e += 2.0 * u * (vtop + d2);
+    if (rho >= 0) e = std::max(e, err_bound_gram(d2, A, rho) + (flip ? 10.0 * u * vtop : 0.0));
+    return e;
   };
   double U = std::min(r.R, r.T[bins]);
   double vmax = U * (1.0 + 1e-3) + 4.0 * err(2 * U + 1) + 1e-30;
-  int e2 = (int)std::floor(std::log2((double)(nt_max - 2) / vmax) / 2.0);
+  // The u * Vtop terms are far below a cell; a 1e-3 margin on vmax covers them.
+  if (flip) vmax *= 1.0 + 1e-3;
+  int e2 = (int)std::floor(std::log2((double)(budget - 2) / vmax) / 2.0);
   if (e2 > 60) e2 = 60;
   if (e2 < -60) e2 = -60;
   double ih = std::ldexp(1.0, 2 * e2);
-  while (vmax * ih > (double)(nt_max - 2)) ih *= 0.25;
+  while (vmax * ih > (double)(budget - 2)) ih *= 0.25;
   const int nt = (int)std::ceil(vmax * ih) + 2;
   const float inv_h = (float)ih;
-  const double u = std::ldexp(1.0, -24);
-  table->assign(nt, 0);
   if (bins > (ani ? 200 : 253)) return -1;
   const uint8_t out_code = ani ? kAniOut8 : (uint8_t)bins;
   const uint8_t amb_code = ani ? kAniAmb8 : (uint8_t)(bins + 1);
+  if (flip) {
+    // Flipped table: the kernels index it by cvt_rz(Tf - d2) (table units,
+    // Tf = nt), so everything beyond the top saturates to cell 0 and no clamp
+    // is needed.  Cell t holds the FP32 values D = Tf - d2 with
+    // cvt_rz(D) == t: t = 0 is D < 1 (d2 beyond the top, and the NaN-free
+    // negatives), t >= 1 is D in [t, t + 1); d2 >= 0 bounds D by Tf plus the
+    // rounding, hence the two cells above Tf.
+    const double Tf = (double)nt;
+    vtop = Tf / ih;
+    const int ntf = nt + 2;
+    table->assign(ntf, 0);
+    for (int t = 0; t < ntf; ++t) {
+      const double D_lo = t == 0 ? -std::numeric_limits<double>::infinity()
+                                 : (double)t * (1.0 - 4 * u);
+      const double D_hi = (double)(t + 1) * (1.0 + 4 * u);
+      const double v_lo = (Tf - D_hi) / ih;  // smallest d2 of the cell (may be < 0)
+      const double v_hi = (Tf - D_lo) / ih;  // largest (+inf for cell 0)
+      const double d_lo = v_lo - 2.0 * err(2 * std::fabs(v_lo) + 1);
+      const double d_hi =
+          std::isinf(v_hi) ? v_hi : v_hi + 2.0 * err(2 * std::max(v_hi, 0.0) + 1);
+      uint8_t code;
+      if (r.zero != 0 && d_lo <= 0.0) {
+        code = amb_code;
+      } else {
+        const int c_lo = host_class(std::max(d_lo, 0.0), r);
+        const int c_hi = std::isinf(d_hi) ? -1 : host_class(std::max(d_hi, 0.0), r);
+        if (c_lo == c_hi) code = (c_lo < 0) ? out_code : (uint8_t)c_lo;
+        else code = amb_code;
+      }
+      (*table)[t] = code;
+    }
+    if ((*table)[0] != out_code) return -1;
+    *inv_h_out = inv_h;
+    *nt_out = ntf;
+    if (top_out) *top_out = (float)Tf;
+    return 0;
+  }
+  table->assign(nt, 0);
   for (int t = 0; t < nt; ++t) {
     // FP32 values v with min(cvt_rz(fl(v*inv_h)), nt-1) == t.
     double v_lo = (double)t * (1.0 - 4 * u) / ih;
@@ -543,7 +598,8 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights,
   static thread_local std::vector<uint8_t> table;
   float inv_h = 1.f;
   int nt = 2;
-  if (!plain && build_table(rules, A, nt_max, &table, &inv_h, &nt, rho) != 0)
+  float tflip = 0.f;  // 2iso: flipped table (index cvt_rz(tflip - d2))
+  if (!plain && build_table(rules, A, nt_max, &table, &inv_h, &nt, rho, !ani, &tflip) != 0)
     return fail(c, PCF_ERR_UNSUPPORTED, "bin table");
   const float sigma = std::sqrt(inv_h);  // exact power of two
   const size_t rt_bytes = sizeof(BinRules) + (plain ? 0 : (size_t)nt);
@@ -743,6 +799,7 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights,
   pl.mfma = gram ? 1 : 0;
   // The kernel's own test on the tile box, 2% inside the table's margin.
   pl.rho2 = gram ? (float)(rho * rho / (1.02 * 1.02) * (double)inv_h) : 0.f;
+  pl.tflip = tflip;
   const bool launched = nitems_dev != nullptr || nitems > 0;
   if (launched) {
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));

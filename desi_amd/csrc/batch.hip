@@ -37,7 +37,7 @@ struct BatchPlan {
   GridGeom g{};
   int nflat = 0, bins = 0, ani = 0, weighted = 0, cshift = 0, nt = 0, nblocks = 0, m = 0;
   bool gram = false;  // Gram form of the pair kernel (matrix cores)
-  float inv_h = 1.f, sigma = 1.f, rho2 = 0.f;
+  float inv_h = 1.f, sigma = 1.f, rho2 = 0.f, tflip = 0.f;
   BinRules rules{};
   int total_items = 0;
   std::vector<int> job_offsets;
@@ -140,6 +140,7 @@ hipError_t enqueue_plan(pcf_ctx* c, BatchPlan* p, size_t njobs, bool prepare, bo
   pl.batch = 1;
   pl.mfma = p->gram ? 1 : 0;
   pl.rho2 = p->rho2;
+  pl.tflip = p->tflip;
   pl.jobs = p->jobs_d.as<BatchJob>();
   pl.chunks = p->chunks_d.as<BatchChunk>();
   pl.nchunks = (int)p->chunks.size();
@@ -217,7 +218,8 @@ int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p, GridPins
   const double A = bpc ? (double)maxabs + (double)L : 0.0;
   std::vector<uint8_t> table;
   const double rho = p->gram ? gram_rho(p->g, U.n) : -1.0;
-  if (build_table(tab_rules, A, kMixedTableMax, &table, &p->inv_h, &p->nt, rho) != 0)
+  if (build_table(tab_rules, A, kMixedTableMax, &table, &p->inv_h, &p->nt, rho, !ani,
+                  &p->tflip) != 0)
     return fail(c, PCF_ERR_UNSUPPORTED, "bin table");
   p->rho2 = p->gram ? (float)(rho * rho / (1.02 * 1.02) * (double)p->inv_h) : 0.f;
   p->sigma = std::sqrt(p->inv_h);

@@ -75,6 +75,7 @@ struct KArgs {
   int cap;                // items allocated (the count is clamped to it)
   float rho2;             // Gram path: largest squared half-diagonal of a tile's
                           // bounding box (scaled units) its error bound covers
+  float tflip;            // 2iso: the table is indexed by cvt_rz(tflip - d2)
 };
 
 // Item count of a launch (from device memory when the list was built without
@@ -93,6 +94,17 @@ __device__ __forceinline__ void launch_items(const KArgs& k, int* nitems, int* c
   *claim = cl;
 }
 
+// Loads of catalog points through the global address space.  The batched
+// launch takes its point arrays from the job table, where the compiler cannot
+// tell their address space and would emit flat loads, whose completion the
+// LDS waits of the inner loop would then also wait for.
+typedef float gv4f __attribute__((ext_vector_type(4)));
+typedef const gv4f __attribute__((address_space(1)))* GV4;
+__device__ __forceinline__ float4 gld(const float4* p, int i) {
+  const gv4f v = ((GV4)p)[i];
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // FP32 bulk + FP64 edge re-evaluation.
 //
 // Table codes (u8): iso -> the histogram row (bin, nflat = out, nflat+1 =
@@ -102,14 +114,15 @@ __device__ __forceinline__ void launch_items(const KArgs& k, int* nitems, int* c
 template <int ANI>
 __device__ __forceinline__ unsigned pair_row(const float4 q, float xs, float ys, float zs,
                                              unsigned ntm1, const uint8_t* __restrict__ tab,
-                                             unsigned bins, unsigned out_row, unsigned amb_row) {
+                                             unsigned bins, unsigned out_row, unsigned amb_row,
+                                             float tflip) {
   const float dx = q.x - xs;
   const float dy = q.y - ys;
   const float dz = q.z - zs;
   if (!ANI) {
+    // Flipped index: d2 beyond the table saturates to cell 0 (out).
     const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-    const unsigned t = min(__float2uint_rz(d2), ntm1);
-    return tab[t];
+    return tab[__float2uint_rz(tflip - d2)];
   } else {
     // 2ani reads two table cells: lanes with a component beyond the table's
     // last cell (out of range) skip both reads.
@@ -154,6 +167,10 @@ __device__ __forceinline__ unsigned exact_row(const float4 pi, const float4 pj, 
   return (unsigned)(bz * bins + bo);
 }
 
+#ifndef PCF_GRAM_H
+#define PCF_GRAM_H 2
+#endif
+constexpr int kGramH = PCF_GRAM_H;  // 2iso Gram path: i blocks per matrix-product group
 constexpr int kBlockMixed = kMixedBlock;
 constexpr int kXcds = 8;  // MI355X: 8 XCDs, each with its own L2
 constexpr int kMaxTable = kMixedTableMax;
@@ -231,8 +248,8 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
     __builtin_amdgcn_wave_barrier();  // the queue was written by other lanes
     if (lane < nd) {
       const int2 e = wq[qn - nd + lane];
-      const float4 pa = xo[e.x];
-      const float4 pb = yo[e.y & kRecheckIndexMask];
+      const float4 pa = gld(xo, e.x);
+      const float4 pb = gld(yo, e.y & kRecheckIndexMask);
       const unsigned r = exact_row<ANI>(pa, pb, (int)((unsigned)e.y >> kRecheckImgShift), sT,
                                         k.bins, k.R, k.le, zero, k.Lx, out_row);
       if (r < out_row) {
@@ -298,10 +315,10 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
       }
     };
     auto quad = [&](int j, const float4 q0, const float4 q1, const float4 q2, const float4 q3) {
-      unsigned r0 = pair_row<ANI>(q0, xs, ys, zs, ntm1, tab, bins, out_row, amb_row);
-      unsigned r1 = pair_row<ANI>(q1, xs, ys, zs, ntm1, tab, bins, out_row, amb_row);
-      unsigned r2 = pair_row<ANI>(q2, xs, ys, zs, ntm1, tab, bins, out_row, amb_row);
-      unsigned r3 = pair_row<ANI>(q3, xs, ys, zs, ntm1, tab, bins, out_row, amb_row);
+      unsigned r0 = pair_row<ANI>(q0, xs, ys, zs, ntm1, tab, bins, out_row, amb_row, k.tflip);
+      unsigned r1 = pair_row<ANI>(q1, xs, ys, zs, ntm1, tab, bins, out_row, amb_row, k.tflip);
+      unsigned r2 = pair_row<ANI>(q2, xs, ys, zs, ntm1, tab, bins, out_row, amb_row, k.tflip);
+      unsigned r3 = pair_row<ANI>(q3, xs, ys, zs, ntm1, tab, bins, out_row, amb_row, k.tflip);
       if (masked) {
         r0 = (j <= my_i) ? out_row : r0;
         r1 = (j + 1 <= my_i) ? out_row : r1;
@@ -331,7 +348,7 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
     }
     for (; j < jend; ++j) {
       const float4 q = yp[j];
-      unsigned r = pair_row<ANI>(q, xs, ys, zs, ntm1, tab, bins, out_row, amb_row);
+      unsigned r = pair_row<ANI>(q, xs, ys, zs, ntm1, tab, bins, out_row, amb_row, k.tflip);
       if (masked) r = (j <= my_i) ? out_row : r;
       fix(r, j);
       deposit(r, q.w);
@@ -357,7 +374,7 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
     const float shy = (float)((img / 3) % 3 - 1) * k.L;
     const float shz = (float)(img % 3 - 1) * k.L;
     auto uni = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
-    const float4 pl = xp[i0 + min(lane, ni - 1)];
+    const float4 pl = gld(xp, i0 + min(lane, ni - 1));
     const float tx = pl.x - shx, ty = pl.y - shy, tz = pl.z - shz;
     const float lox = uni(__ockl_wfred_min_f32(tx)), hix = uni(__ockl_wfred_max_f32(tx));
     const float loy = uni(__ockl_wfred_min_f32(ty)), hiy = uni(__ockl_wfred_max_f32(ty));
@@ -389,40 +406,41 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
       }
     };
     if constexpr (!ANI) {
-      // i side: B operand (component grp of point i = 16b + col, times -2;
-      // grp 3: 1) and the C column |x_i|^2; lanes past the tile get a huge
-      // |x_i|^2 (always out).
+      // i side: B operand (component grp of point i = 16b + col, times 2;
+      // grp 3: -1) and the C column Tf - |x_i|^2, so the product is
+      // Tf - d2, the flipped table index (beyond the table: negative, cell
+      // 0).  Lanes past the tile get a hugely negative C (always out).
       float bop[4], cin[4];
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         const int ir = 16 * b + col;
-        const float4 p = xp[i0 + min(ir, ni - 1)];
+        const float4 p = gld(xp, i0 + min(ir, ni - 1));
         const float x = (p.x - shx) - ox, y = (p.y - shy) - oy, z = (p.z - shz) - oz;
         // Exact select by 0/1 coefficients (products by 0 and 1 are exact).
-        bop[b] = __builtin_fmaf(sel_x, -2.0f * x,
-                                __builtin_fmaf(sel_y, -2.0f * y, __builtin_fmaf(sel_z, -2.0f * z, sel_w)));
-        cin[b] = ir < ni ? __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x)) : 1e30f;
+        bop[b] = __builtin_fmaf(sel_x, 2.0f * x,
+                                __builtin_fmaf(sel_y, 2.0f * y, __builtin_fmaf(sel_z, 2.0f * z, -sel_w)));
+        cin[b] = ir < ni ? k.tflip - __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x)) : -1e30f;
       }
       // The j loop for a tile of NB i-blocks (NB*16 >= ni), 16 j per step.
       auto jloop = [&](auto nbc) {
         constexpr int NB = decltype(nbc)::value;
         // The next step's j point is loaded one step ahead (its latency hides
         // behind this step's matrix, LDS and histogram work).
-        float4 qn4 = yp[min(j0 + col, jend - 1)];
+        float4 qn4 = gld(yp, min(j0 + col, jend - 1));
         for (int jb = j0; jb < jend; jb += 16) {
           // j side: A operand (component grp of point j = jb + col; grp 3:
           // |x_j|^2, huge past the run's end).
           const int jj = jb + col;
           const float4 q = qn4;
-          qn4 = yp[min(jj + 16, jend - 1)];
+          qn4 = gld(yp, min(jj + 16, jend - 1));
           const float x = q.x - ox, y = q.y - oy, z = q.z - oz;
           const float bj = jj < jend ? __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x)) : 1e30f;
           const float aop =
               __builtin_fmaf(sel_x, x, __builtin_fmaf(sel_y, y, __builtin_fmaf(sel_z, z, sel_w * bj)));
-          // Two blocks at a time (fewer live registers, so more waves fit).
+          // kGramH blocks at a time.
 #pragma unroll
-          for (int h = 0; h < NB; h += 2) {
-            constexpr int H = 2;
+          for (int h = 0; h < NB; h += kGramH) {
+            constexpr int H = kGramH;
             const int nh = (NB - h) < H ? (NB - h) : H;
             f4v d[H];
 #pragma unroll
@@ -438,7 +456,7 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
             for (int u = 0; u < H; ++u)
 #pragma unroll
               for (int v = 0; v < 4; ++v)
-                r[4 * u + v] = u < nh ? tab[min(__float2uint_rz(d[u][v]), ntm1)] : 0u;
+                r[4 * u + v] = u < nh ? tab[__float2uint_rz(d[u][v])] : 0u;
             // Element (b, v) of this lane: i = i0 + 16b + col, j = jb + 4*grp + v.
             if (jb < mask_end) {
 #pragma unroll
@@ -485,7 +503,7 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         const int ir = 16 * b + col;
-        const float4 p = xp[i0 + min(ir, ni - 1)];
+        const float4 p = gld(xp, i0 + min(ir, ni - 1));
         const float x = (p.x - shx) - ox, y = (p.y - shy) - oy, z = (p.z - shz) - oz;
         // A lane past the tile gets huge squared norms in both products, so
         // neither component can read as ambiguous (its pair does not exist).
@@ -500,13 +518,13 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
       auto jloop = [&](auto nbc) {
         constexpr int NB = decltype(nbc)::value;
         const f4v zero4 = {0.f, 0.f, 0.f, 0.f};
-        float4 qn4 = yp[min(j0 + col, jend - 1)];
+        float4 qn4 = gld(yp, min(j0 + col, jend - 1));
         for (int jb = j0; jb < jend; jb += 16) {
           // j side: A operands (component grp of point j = jb + col; a j past
           // the run's end gets huge squared norms in both).
           const int jj = jb + col;
           const float4 q = qn4;
-          qn4 = yp[min(jj + 16, jend - 1)];
+          qn4 = gld(yp, min(jj + 16, jend - 1));
           const float x = q.x - ox, y = q.y - oy, z = q.z - oz;
           const bool jv = jj < jend;
           const float np2 = jv ? __builtin_fmaf(y, y, x * x) : 1e30f;
@@ -978,6 +996,7 @@ hipError_t launch_pairs(const PairLaunch& pl, hipStream_t s) {
   k.nitems_dev = pl.batch ? nullptr : pl.nitems_dev;
   k.cap = pl.nitems_cap;
   k.rho2 = pl.rho2;
+  k.tflip = pl.tflip;
   const int ncopy = 1 << pl.cshift;
   dim3 grid(pl.nblocks), block(kBlock);
   hipError_t e;
