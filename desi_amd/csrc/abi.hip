@@ -800,6 +800,8 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights,
   // The kernel's own test on the tile box, 2% inside the table's margin.
   pl.rho2 = gram ? (float)(rho * rho / (1.02 * 1.02) * (double)inv_h) : 0.f;
   pl.tflip = tflip;
+  pl.geo = g;
+  pl.sigma = sigma;
   const bool launched = nitems_dev != nullptr || nitems > 0;
   if (launched) {
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));

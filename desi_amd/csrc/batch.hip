@@ -141,6 +141,8 @@ hipError_t enqueue_plan(pcf_ctx* c, BatchPlan* p, size_t njobs, bool prepare, bo
   pl.mfma = p->gram ? 1 : 0;
   pl.rho2 = p->rho2;
   pl.tflip = p->tflip;
+  pl.geo = p->g;
+  pl.sigma = p->sigma;
   pl.jobs = p->jobs_d.as<BatchJob>();
   pl.chunks = p->chunks_d.as<BatchChunk>();
   pl.nchunks = (int)p->chunks.size();

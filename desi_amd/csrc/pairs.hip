@@ -30,6 +30,7 @@
 #include "worklist.h"
 #include "exact.h"
 
+
 // Wave-wide min / max (ROCm device library, DPP-based).
 extern "C" __device__ __attribute__((const)) float __ockl_wfred_min_f32(float);
 extern "C" __device__ __attribute__((const)) float __ockl_wfred_max_f32(float);
@@ -76,6 +77,10 @@ struct KArgs {
   float rho2;             // Gram path: largest squared half-diagonal of a tile's
                           // bounding box (scaled units) its error bound covers
   float tflip;            // 2iso: the table is indexed by cvt_rz(tflip - d2)
+  // Grid geometry in the bulk path's (scaled) units: the Gram path bounds a
+  // tile by its cells.
+  float glo[3], gcs[3], ginv[3];
+  int gP[3];
 };
 
 // Item count of a launch (from device memory when the list was built without
@@ -374,14 +379,36 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
     const float shy = (float)((img / 3) % 3 - 1) * k.L;
     const float shz = (float)(img % 3 - 1) * k.L;
     auto uni = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
-    const float4 pl = gld(xp, i0 + min(lane, ni - 1));
-    const float tx = pl.x - shx, ty = pl.y - shy, tz = pl.z - shz;
-    const float lox = uni(__ockl_wfred_min_f32(tx)), hix = uni(__ockl_wfred_max_f32(tx));
-    const float loy = uni(__ockl_wfred_min_f32(ty)), hiy = uni(__ockl_wfred_max_f32(ty));
-    const float loz = uni(__ockl_wfred_min_f32(tz)), hiz = uni(__ockl_wfred_max_f32(tz));
-    const float ex = hix - lox, ey = hiy - loy, ez = hiz - loz;
-    if (!(0.25f * (ex * ex + ey * ey + ez * ez) <= k.rho2)) return false;
-    const float ox = 0.5f * (lox + hix), oy = 0.5f * (loy + hiy), oz = 0.5f * (loz + hiz);
+    // Tile box from the grid, with no reduction over the lanes: the tile's
+    // points lie in the (cx, cy) column of its first point, between the z
+    // cells of its first and last points.  Cell indices are formed as the
+    // grid build forms them (the scaled units scale both operands by a power
+    // of two, so the same cells result); a point can sit outside its cell by
+    // FP32 rounding only, far inside the 2% margin of the bound k.rho2.
+    const float4 pf = gld(xp, i0), pe = gld(xp, i0 + ni - 1);
+    auto cell = [&](float v, int a) {
+      const int c = (int)floorf((uni(v) - k.glo[a]) * k.ginv[a]);
+      return min(max(c, 0), k.gP[a] - 1);
+    };
+    const int cx = cell(pf.x, 0), cy = cell(pf.y, 1), czf = cell(pf.z, 2), czl = cell(pe.z, 2);
+    const float ex = k.gcs[0], ey = k.gcs[1], ez = (float)(czl - czf + 1) * k.gcs[2];
+    float ox = k.glo[0] + ((float)cx + 0.5f) * k.gcs[0] - shx;
+    float oy = k.glo[1] + ((float)cy + 0.5f) * k.gcs[1] - shy;
+    float oz = k.glo[2] + 0.5f * (float)(czf + czl + 1) * k.gcs[2] - shz;
+    if (!(0.25f * (ex * ex + ey * ey + ez * ez) <= k.rho2)) {
+      // Cells too tall for the bound (a sparse column): the points' own box,
+      // by wave-wide reductions, may still fit.
+      const float4 pl = gld(xp, i0 + min(lane, ni - 1));
+      const float tx = pl.x - shx, ty = pl.y - shy, tz = pl.z - shz;
+      const float lox = uni(__ockl_wfred_min_f32(tx)), hix = uni(__ockl_wfred_max_f32(tx));
+      const float loy = uni(__ockl_wfred_min_f32(ty)), hiy = uni(__ockl_wfred_max_f32(ty));
+      const float loz = uni(__ockl_wfred_min_f32(tz)), hiz = uni(__ockl_wfred_max_f32(tz));
+      const float bx = hix - lox, by = hiy - loy, bz = hiz - loz;
+      if (!(0.25f * (bx * bx + by * by + bz * bz) <= k.rho2)) return false;
+      ox = 0.5f * (lox + hix);
+      oy = 0.5f * (loy + hiy);
+      oz = 0.5f * (loz + hiz);
+    }
     evals += (unsigned long long)ni * (unsigned long long)nj;
 
     const int nb = (ni + 15) >> 4;
@@ -997,6 +1024,13 @@ hipError_t launch_pairs(const PairLaunch& pl, hipStream_t s) {
   k.cap = pl.nitems_cap;
   k.rho2 = pl.rho2;
   k.tflip = pl.tflip;
+  for (int a = 0; a < 3; ++a) {
+    // Scaling by sigma (a power of two) is exact.
+    k.glo[a] = pl.geo.lo[a] * pl.sigma;
+    k.gcs[a] = pl.geo.cs[a] * pl.sigma;
+    k.ginv[a] = pl.geo.inv_cs[a] / pl.sigma;
+    k.gP[a] = pl.geo.P[a];
+  }
   const int ncopy = 1 << pl.cshift;
   dim3 grid(pl.nblocks), block(kBlock);
   hipError_t e;

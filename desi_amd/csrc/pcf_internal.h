@@ -208,6 +208,8 @@ struct PairLaunch {
   int mfma = 0;       // 2iso unweighted: Gram-form distances on the matrix cores
   float rho2 = 0.f;   // ... for tiles whose box half-diagonal^2 (scaled) is <= rho2
   float tflip = 0.f;  // 2iso mixed path: the table is indexed by cvt_rz(tflip - d2)
+  GridGeom geo{};     // grid of the launch (Gram path: tile boxes from cells)
+  float sigma = 1.f;  // scale of the bulk path's coordinates
   int prepare_only = 0;  // only set the kernel's LDS attribute (before graph capture)
   int global_hist = 0;   // plain path: deposit into the global histogram (no LDS copy)
   const BatchJob* jobs = nullptr;
