@@ -36,7 +36,7 @@ EXPORTED_SYMBOLS = (
     "pcf_catalog_size", "pcf_histogram", "pcf_last_stats", "pcf_analytic_rr",
     "pcf_analytic_3iso", "pcf_histogram_batch", "pcf_autotune", "pcf_set_max_batch",
     "pcf_set_hbm_budget", "pcf_submit", "pcf_poll", "pcf_wait", "pcf_estimator",
-    "pcf_histogram_noalloc",
+    "pcf_histogram_noalloc", "pcf_selftest_scan",
 )
 # Bytes of the static device array pcf_histogram_noalloc works in (per GPU).
 PCF_NOALLOC_SCRATCH_BYTES = 128 * 1024 * 1024
@@ -109,6 +109,7 @@ def load_library(path=LIB_PATH):
     L.pcf_catalog_size.restype = c_i64
     L.pcf_histogram.argtypes = [vp, ctypes.POINTER(PcfJob), vp, vp]
     L.pcf_histogram_noalloc.argtypes = [vp, ctypes.POINTER(PcfJob), vp, vp]
+    L.pcf_selftest_scan.argtypes = [vp, vp, c_i64, ctypes.POINTER(c_i32)]
     L.pcf_last_stats.argtypes = [vp, ctypes.POINTER(PcfStats)]
     L.pcf_analytic_rr.argtypes = [vp, c_i32, c_i64, ctypes.c_float, ctypes.c_float, vp]
     L.pcf_histogram_batch.argtypes = [vp, ctypes.POINTER(PcfJob), c_i32, vp, vp]
@@ -213,6 +214,15 @@ class Engine:
         self._check(self.lib.pcf_histogram_noalloc(self.ctx, ctypes.byref(job), c.ctypes.data,
                                                    w.ctypes.data))
         return c, w
+
+    def selftest_scan(self, counts):
+        """The library's device prefix sum over `counts`: its total, or -1
+        when the total is 2^31 or more (pcf_selftest_scan)."""
+        a = np.ascontiguousarray(counts, dtype=np.int32)
+        tot = ctypes.c_int32()
+        self._check(self.lib.pcf_selftest_scan(self.ctx, a.ctypes.data, len(a),
+                                               ctypes.byref(tot)))
+        return tot.value
 
     def stats(self):
         s = PcfStats()

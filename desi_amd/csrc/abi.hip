@@ -12,6 +12,7 @@
 #include <map>
 
 #include "../../include/pcf_abi.h"
+#include "../../include/pcf_selftest.h"
 #include "pcf_internal.h"
 #include "pcf_ctx.h"
 
@@ -1094,3 +1095,21 @@ int pcf_analytic_rr(pcf_ctx* c, int32_t bins, int64_t np, float dmax, float box,
 }
 
 }  // extern "C"
+
+extern "C" int pcf_selftest_scan(pcf_ctx* c, const int32_t* counts, int64_t n, int32_t* total) {
+  using namespace pcf;
+  if (!c || !total || n < 0 || (n > 0 && !counts)) return fail(c, PCF_ERR_ARG, "bad scan arguments");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t ibytes = (sizeof(int) * (size_t)(2 * n + 2) + 255) & ~(size_t)255;
+  HIPCHK(c, c->misc_buf.ensure(ibytes + sizeof(long long) * (size_t)scan_tiles(n)));
+  int* in = c->misc_buf.as<int>();
+  int* out = in + n + 1;
+  long long* tsum = reinterpret_cast<long long*>(c->misc_buf.as<char>() + ibytes);
+  if (n > 0)
+    HIPCHK(c, hipMemcpyAsync(in, counts, sizeof(int) * (size_t)n, hipMemcpyHostToDevice,
+                             c->stream));
+  HIPCHK(c, launch_scan(in, n, out, tsum, c->stream));
+  HIPCHK(c, hipMemcpyAsync(total, out + n, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return PCF_OK;
+}

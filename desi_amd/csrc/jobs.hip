@@ -155,15 +155,22 @@ struct JobEngine {
     std::vector<uint64_t> c((size_t)hs * jobs.size());
     std::vector<double> w((size_t)hs * jobs.size());
     const int rc = run_pairs_batch(lane, jobs.data(), (int)jobs.size(), c.data(), w.data());
+    if (rc != PCF_OK) {
+      // One job of the batch is refused (its catalog outside the box, say):
+      // every job runs alone, so each gets exactly its own result or error.
+      for (Ticket* t : take) {
+        t->rc = dispatch_histogram(lane, &t->job, t->counts.data(), t->weights.data());
+        t->st = lane->last;
+        t->st.batched_jobs = 1;
+        if (t->rc != PCF_OK) t->err = lane->err;
+      }
+      return;
+    }
     for (size_t k = 0; k < take.size(); ++k) {
       Ticket* t = take[k];
-      t->rc = rc;
+      t->rc = PCF_OK;
       t->st = lane->last;
       t->st.batched_jobs = (int32_t)take.size();
-      if (rc != PCF_OK) {
-        t->err = lane->err;
-        continue;
-      }
       std::copy(c.begin() + hs * k, c.begin() + hs * (k + 1), t->counts.begin());
       std::copy(w.begin() + hs * k, w.begin() + hs * (k + 1), t->weights.begin());
     }
@@ -262,12 +269,28 @@ int pcf_set_hbm_budget(pcf_ctx* c, int64_t bytes, int32_t max_inflight) {
   using namespace pcf;
   if (!c) return PCF_ERR_ARG;
   int mb = 1;
+  // Finished tickets not yet collected move to the restarted engine (the old
+  // one drains first, so every ticket is finished by then).
+  std::unordered_map<pcf_ticket, std::unique_ptr<Ticket>> kept;
+  pcf_ticket next_id = 1;
   if (c->engine) {
     mb = c->engine->max_batch;
+    c->engine->drain();
+    {
+      std::lock_guard<std::mutex> lk(c->engine->mu);
+      kept.swap(c->engine->tickets);
+      next_id = c->engine->next_id;
+    }
     destroy_engine(c->engine);
     c->engine = nullptr;
   }
-  return start_engine(c, bytes, max_inflight, mb);
+  const int rc = start_engine(c, bytes, max_inflight, mb);
+  if (rc == PCF_OK) {
+    std::lock_guard<std::mutex> lk(c->engine->mu);
+    c->engine->tickets.swap(kept);
+    c->engine->next_id = next_id;
+  }
+  return rc;
 }
 
 int pcf_set_max_batch(pcf_ctx* c, int32_t max_batch) {

@@ -934,11 +934,38 @@ __global__ __launch_bounds__(kTripBlock) void k_trip_ani_lds(
       if (qi < nq) kr = Q[qbeg + qi];
       return kr;
     };
-    // Point 1's pairs in the slice, up to 64 per pass, read by uniform index
-    // (scalar loads: no vector register or lane broadcast per pair).
+    // Point 1's pairs in the slice, up to 64 per pass: one vector load per
+    // lane (the next pass's in flight while this one runs), each pair then
+    // broadcast from its lane by readlane.  Scalar loads here would share the
+    // LDS counter with the table reads, so every table wait would also wait
+    // for them.
+    auto load_p = [&](int pc) {
+      PairRec r;
+      r.xj = r.yj = r.zj = 0.f;
+      r.wj = 0.f;
+      r.j = -1;
+      r.meta = 0;
+      if (pc + lane < pend) r = P[pc + lane];
+      return r;
+    };
+    PairRec pnext = load_p(pbeg);
     for (int pc = pbeg; pc < pend; pc += kWave) {
       const int np = min(kWave, pend - pc);
-      auto bcast = [&](int u) -> PairRec { return P[pc + u]; };
+      const PairRec pmine = pnext;
+      if (pc + kWave < pend) pnext = load_p(pc + kWave);
+      auto rdl = [](float v, int u) {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), u));
+      };
+      auto bcast = [&](int u) -> PairRec {
+        PairRec r;
+        r.xj = rdl(pmine.xj, u);
+        r.yj = rdl(pmine.yj, u);
+        r.zj = rdl(pmine.zj, u);
+        r.wj = WEIGHTED ? rdl(pmine.wj, u) : 1.f;
+        r.j = PLAIN ? __builtin_amdgcn_readlane(pmine.j, u) : 0;
+        r.meta = __builtin_amdgcn_readlane(pmine.meta, u);
+        return r;
+      };
       PairRec kr_next = load_q(0);
       for (int ch = 0; ch < nch; ++ch) {
         const PairRec kr = kr_next;

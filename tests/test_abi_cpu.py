@@ -12,13 +12,16 @@ import pytest
 import desi_amd as da
 from conftest import GOLDEN, ROOT
 
-HEADER = os.path.join(ROOT, "include", "pcf_abi.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("pcf_abi.h", "pcf_selftest.h")]
 
 
 def declared_functions():
-    src = open(HEADER).read()
-    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(pcf_\w+)\s*\(", src)))
+    names = set()
+    for h in HEADERS:
+        src = open(h).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        names.update(re.findall(r"\b(pcf_\w+)\s*\(", src))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():

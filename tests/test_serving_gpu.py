@@ -116,3 +116,47 @@ def test_poll_wait_errors_and_release(served):
         assert e.poll(tt)
         c2, _, _ = e.wait(tt, j.type, j.bins)
         assert np.array_equal(c2, c)
+
+
+def test_refused_job_does_not_fail_its_batch(served):
+    """A job refused on its own arguments (a periodic job whose catalog lies
+    outside the box) queued with compatible jobs: they still get their own
+    results, and it gets its error."""
+    e = served
+    ids = _catalogs(e)
+    e.set_hbm_budget(0, 1)  # one lane: jobs queue behind the first and batch
+    e.set_max_batch(8)
+    J = da.make_job
+    first = J(da.PCF_2ISO, da.PCF_XY, bins=30, dmax=80.0, cat_x=ids["big"], cat_y=ids["big2"])
+    kw = dict(bins=20, dmax=60.0, bpc=True, box=250.0)
+    good1 = J(da.PCF_2ISO, da.PCF_XX, cat_x=ids["u4k"], **kw)
+    bad = J(da.PCF_2ISO, da.PCF_XX, cat_x=ids["big"], **kw)  # points up to 1000
+    good2 = J(da.PCF_2ISO, da.PCF_XY, cat_x=ids["ns3k"], cat_y=ids["u9k"], **kw)
+    t0 = e.submit(first)
+    tg1, tb, tg2 = e.submit(good1), e.submit(bad), e.submit(good2)
+    e.wait(t0, first.type, first.bins)
+    for t, j in ((tg1, good1), (tg2, good2)):
+        c, w, _ = e.wait(t, j.type, j.bins)
+        c1, w1 = e.histogram(j)
+        assert np.array_equal(c, c1)
+        assert np.all(np.abs(w - w1) <= 1e-12 * np.abs(w1))
+    with pytest.raises(da.PcfError) as err:
+        e.wait(tb, bad.type, bad.bins)
+    assert err.value.rc == da.PCF_ERR_RANGE
+
+
+def test_budget_change_keeps_finished_tickets(served):
+    e = served
+    ids = _catalogs(e)
+    J = da.make_job
+    jobs = [J(da.PCF_2ISO, da.PCF_XY, bins=20, dmax=60.0, cat_x=ids["u4k"], cat_y=ids["u9k"]),
+            J(da.PCF_2ANI, da.PCF_XX, bins=20, dmax=60.0, cat_x=ids["ns3k"])]
+    tickets = [e.submit(j) for j in jobs]
+    e.set_hbm_budget(0, 2)  # restarts the engine after the queue drains
+    for t, j in zip(tickets, jobs):
+        c, _, _ = e.wait(t, j.type, j.bins)
+        c1, _ = e.histogram(j)
+        assert np.array_equal(c, c1)
+    t = e.submit(jobs[0])  # ticket ids keep counting on the new engine
+    assert t > max(tickets)
+    e.wait(t, jobs[0].type, jobs[0].bins)
