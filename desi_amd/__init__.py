@@ -255,11 +255,11 @@ class Engine:
         of (counts, weights) per job."""
         n = len(jobs)
         arr = (PcfJob * n)(*jobs)
-        hs = hist_size(jobs[0].type, jobs[0].bins)
-        c = np.zeros(n * hs, np.uint64)
-        w = np.zeros(n * hs, np.float64)
+        off = np.cumsum([0] + [hist_size(j.type, j.bins) for j in jobs])
+        c = np.zeros(off[-1], np.uint64)
+        w = np.zeros(off[-1], np.float64)
         self._check(self.lib.pcf_histogram_batch(self.ctx, arr, n, c.ctypes.data, w.ctypes.data))
-        return [(c[k * hs:(k + 1) * hs], w[k * hs:(k + 1) * hs]) for k in range(n)]
+        return [(c[off[k]:off[k + 1]], w[off[k]:off[k + 1]]) for k in range(n)]
 
     def autotune(self, jobs):
         """Tune the pair kernels for one job or a batch (pcf_autotune); the

@@ -320,6 +320,20 @@ int finish_upload(pcf_ctx* c, Catalog* cat) {
 // ---------------------------------------------------------------------------
 namespace pcf {
 
+// Neighbour reach of pair counting up to dmax on a grid over [lo, hi] (or the
+// periodic box): dmax plus the slack of the FP32 cell assignment.  A point may
+// sit ~3u*(|coordinate| + extent) outside its nominal cell after the FP32
+// cell assignment (per axis, both points of a pair), and the FP32 pruning
+// arithmetic rounds at ~1e-7 relative.
+double pair_reach(const double lo[3], const double hi[3], float dmax, int bpc, float box) {
+  const double u = std::ldexp(1.0, -24);
+  double ext_max = 0;
+  for (int a = 0; a < 3; ++a) ext_max = std::max(ext_max, bpc ? (double)box : hi[a] - lo[a]);
+  const double delta = 4 * u * (2 * ext_max + std::fabs(lo[0]) + std::fabs(lo[1]) +
+                                std::fabs(lo[2]) + 2 * (double)dmax) + 1e-7;
+  return (double)dmax * (1.0 + 1e-5) + 4 * delta;
+}
+
 int choose_geometry(const Catalog* X, const Catalog* Y, float dmax, int bpc, float box,
                     int cells_per_rmax, GridGeom* g, int K[3], float* reach_out, int* m_out,
                     int zsub) {
@@ -349,14 +363,7 @@ int choose_geometry(const Catalog* X, const Catalog* Y, float dmax, int bpc, flo
     }
   }
   if (m > 8) m = 8;
-  double ext_max = 0;
-  for (int a = 0; a < 3; ++a) ext_max = std::max(ext_max, bpc ? (double)box : hi[a] - lo[a]);
-  // Slack: a point may sit ~3u*(|coordinate| + extent) outside its nominal
-  // cell after the FP32 cell assignment (per axis, both points of a pair), and
-  // the FP32 pruning arithmetic rounds at ~1e-7 relative.
-  const double delta = 4 * u * (2 * ext_max + std::fabs(lo[0]) + std::fabs(lo[1]) +
-                                std::fabs(lo[2]) + 2 * (double)dmax) + 1e-7;
-  double reach = (double)dmax * (1.0 + 1e-5) + 4 * delta;
+  double reach = pair_reach(lo, hi, dmax, bpc, box);
   // Cell edge a hair above reach/m, so the neighbour reach is exactly m cells.
   double cs_target = reach / m * (1.0 + 1e-6);
   g->bpc = bpc;

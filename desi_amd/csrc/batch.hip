@@ -33,12 +33,25 @@ struct BatchPlan {
   std::vector<uint64_t> serials;      // catalog serial per job side (x, y)
   std::vector<const void*> grid_pts;  // grid point arrays the plan was built on
   std::vector<DevGrid*> grids;        // and their grids (same order as serials)
-  // geometry / rules
+  // geometry / rules.  Jobs of one (bins, dmax) share a group: its bin
+  // table, exact rules and scale; nflat and bins are the largest of the batch
+  // (they size the kernel's LDS layout).
   GridGeom g{};
-  int nflat = 0, bins = 0, ani = 0, weighted = 0, cshift = 0, nt = 0, nblocks = 0, m = 0;
+  int nflat = 0, bins = 0, ani = 0, weighted = 0, cshift = 0, nblocks = 0, m = 0;
   bool gram = false;  // Gram form of the pair kernel (matrix cores)
-  float inv_h = 1.f, sigma = 1.f, rho2 = 0.f, tflip = 0.f;
-  BinRules rules{};
+  struct Group {
+    int bins = 0;
+    float dmax = 0.f;
+    BinRules rules{};
+    float inv_h = 1.f, sigma = 1.f, rho2 = 0.f, tflip = 0.f;
+    int nt = 0;
+    size_t table_off = 0;  // in table_d
+  };
+  std::vector<Group> groups;
+  std::vector<int> job_group;   // group of each job
+  std::vector<int> job_nflat;   // histogram size of each job
+  std::vector<int64_t> job_row;  // first row of each job in the output
+  int64_t total_rows = 0;
   int total_items = 0;
   std::vector<int> job_offsets;
   std::vector<BatchChunk> chunks;
@@ -50,10 +63,11 @@ struct BatchPlan {
   std::vector<const float4*> regrid_raw;
   std::vector<int64_t> regrid_n;
   DevBuf regrid_cells, regrid_perm;
-  // scaled copies: one per distinct catalog
+  // scaled copies: one per distinct (catalog, group scale)
   std::vector<const float4*> scale_src;
   std::vector<int64_t> scale_n;
   std::vector<int64_t> scale_off;
+  std::vector<float> scale_sigma;
   // device memory owned by the plan
   DevBuf counts, items, jobs_d, chunks_d, table_d, rules_d, hist_d, scaled;
   // Two captured graphs: the preparation (re-gridding, scaled copies,
@@ -94,9 +108,10 @@ bool same_job(const pcf_job& a, const pcf_job& b) {
 hipError_t enqueue_plan(pcf_ctx* c, BatchPlan* p, size_t njobs, bool prepare, bool regrid,
                         int part) {
   hipError_t e;
+  (void)njobs;
   float4* sc = p->scaled.as<float4>();
-  const size_t hist_bytes = sizeof(unsigned long long) * p->nflat * njobs;
-  const size_t w_bytes = sizeof(double) * p->nflat * njobs;
+  const size_t hist_bytes = sizeof(unsigned long long) * (size_t)p->total_rows;
+  const size_t w_bytes = sizeof(double) * (size_t)p->total_rows;
   if (part == 0 && !prepare && regrid) {
     for (size_t k = 0; k < p->grids.size(); ++k)
       if ((e = build_grid_into(p->regrid_raw[k], p->regrid_n[k], p->g, p->grids[k]->pts,
@@ -106,7 +121,7 @@ hipError_t enqueue_plan(pcf_ctx* c, BatchPlan* p, size_t njobs, bool prepare, bo
   }
   if (part == 0 && !prepare) {
     for (size_t k = 0; k < p->scale_src.size(); ++k)
-      if ((e = launch_scale_points(p->scale_src[k], p->scale_n[k], p->sigma,
+      if ((e = launch_scale_points(p->scale_src[k], p->scale_n[k], p->scale_sigma[k],
                                    sc + p->scale_off[k], c->stream)) != hipSuccess)
         return e;
     if ((e = launch_zero(p->hist_d.p, hist_bytes + w_bytes + 64, c->stream)) != hipSuccess)
@@ -114,35 +129,38 @@ hipError_t enqueue_plan(pcf_ctx* c, BatchPlan* p, size_t njobs, bool prepare, bo
   }
   unsigned long long* d_stats =
       reinterpret_cast<unsigned long long*>(p->hist_d.as<char>() + hist_bytes + w_bytes);
+  // Launch parameters of the first job's group; the kernel switches to each
+  // chunk's job parameters from the job table.
+  const BatchPlan::Group& g0 = p->groups[p->job_group[0]];
   PairLaunch pl{};
   pl.items = p->items.as<PairItem>();
   pl.nitems = p->total_items;
   pl.counter = reinterpret_cast<int*>(d_stats + 4);
-  pl.table = p->table_d.as<uint8_t>();
-  pl.nt = p->nt;
-  pl.inv_h = p->inv_h;
-  pl.L = (float)p->rules.L * p->sigma;
+  pl.table = p->table_d.as<uint8_t>() + g0.table_off;
+  pl.nt = g0.nt;
+  pl.inv_h = g0.inv_h;
+  pl.L = (float)g0.rules.L * g0.sigma;
   pl.bins = p->bins;
   pl.nflat = p->nflat;
   pl.cshift = p->cshift;
   pl.hist = p->hist_d.as<unsigned long long>();
   pl.whist = reinterpret_cast<double*>(p->hist_d.as<char>() + hist_bytes);
   pl.stats = d_stats;
-  pl.rules = p->rules_d.as<BinRules>();
+  pl.rules = p->rules_d.as<BinRules>() + p->job_group[0];
   pl.ani = p->ani;
   pl.weighted = p->weighted;
   pl.plain64 = 0;
   pl.nblocks = p->nblocks;
-  pl.R = p->rules.R;
-  pl.Lx = p->rules.L;
-  pl.le = p->rules.le;
-  pl.zero = p->rules.zero;
+  pl.R = g0.rules.R;
+  pl.Lx = g0.rules.L;
+  pl.le = g0.rules.le;
+  pl.zero = g0.rules.zero;
   pl.batch = 1;
   pl.mfma = p->gram ? 1 : 0;
-  pl.rho2 = p->rho2;
-  pl.tflip = p->tflip;
+  pl.rho2 = g0.rho2;
+  pl.tflip = g0.tflip;
   pl.geo = p->g;
-  pl.sigma = p->sigma;
+  pl.sigma = g0.sigma;
   pl.jobs = p->jobs_d.as<BatchJob>();
   pl.chunks = p->chunks_d.as<BatchChunk>();
   pl.nchunks = (int)p->chunks.size();
@@ -155,7 +173,6 @@ hipError_t enqueue_plan(pcf_ctx* c, BatchPlan* p, size_t njobs, bool prepare, bo
 int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p, GridPins& pins) {
   const pcf_job& j0 = jobs[0];
   const bool ani = j0.type == PCF_2ANI;
-  const int bins = j0.bins;
   const int bpc = j0.bpc ? 1 : 0;
   const float L = j0.box;
   // Distinct catalogs and the joint bounds / occupancy used for the geometry.
@@ -163,9 +180,13 @@ int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p, GridPins
   auto add_id = [&](pcf_catalog id) {
     if (std::find(ids.begin(), ids.end(), id) == ids.end()) ids.push_back(id);
   };
+  float dmax_max = 0.f;
+  int bins_max = 0;
   for (int k = 0; k < njobs; ++k) {
     add_id(jobs[k].cat_x);
     if (jobs[k].hist == PCF_XY) add_id(jobs[k].cat_y);
+    dmax_max = std::max(dmax_max, jobs[k].dmax);
+    bins_max = std::max(bins_max, jobs[k].bins);
   }
   Catalog U;
   bool any = false;
@@ -188,10 +209,12 @@ int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p, GridPins
   // Gram form (matrix cores) for unweighted mixed-precision batches, on thin
   // z cells; PCF_PAIR_DIRECT in the first job selects the direct form.
   p->gram = all_w1 && j0.pair_kernel != PCF_PAIR_DIRECT;
-  int K[3];
-  float reach;
-  if (choose_geometry(&U, nullptr, j0.dmax, bpc, L, j0.cells_per_rmax, &p->g, K, &reach, &p->m,
-                      p->gram ? kGramZsub : 1) != 0)
+  // One grid for the batch, cut for its largest range; each job reaches its
+  // own number of cells.
+  int Kmax[3];
+  float reach_max;
+  if (choose_geometry(&U, nullptr, dmax_max, bpc, L, j0.cells_per_rmax, &p->g, Kmax, &reach_max,
+                      &p->m, p->gram ? kGramZsub : 1) != 0)
     return fail(c, PCF_ERR_UNSUPPORTED, "geometry");
   // Grids (resident, cached per catalog and geometry).
   int rc = PCF_OK;
@@ -211,45 +234,87 @@ int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p, GridPins
     if (!gr) return rc;
     grid_of[id] = gr;
   }
-  // Rules: one table for the batch.  Near-zero table cells are marked
-  // ambiguous when any job excludes zero separations; each job's exact path
+  // Groups: one bin table, rule set and scale per (bins, dmax).  Near-zero
+  // table cells are marked ambiguous (the rules of an auto histogram) so one
+  // table serves the XX and XY jobs of a group; each job's exact path
   // applies its own zero rule.
-  BinRules tab_rules;
-  make_rules(j0.type, bpc, 1, bins, j0.dmax, L, &tab_rules);
-  make_rules(j0.type, bpc, 1, bins, j0.dmax, L, &p->rules);
   const double A = bpc ? (double)maxabs + (double)L : 0.0;
-  std::vector<uint8_t> table;
   const double rho = p->gram ? gram_rho(p->g, U.n) : -1.0;
-  if (build_table(tab_rules, A, kMixedTableMax, &table, &p->inv_h, &p->nt, rho, !ani,
-                  &p->tflip) != 0)
-    return fail(c, PCF_ERR_UNSUPPORTED, "bin table");
-  p->rho2 = p->gram ? (float)(rho * rho / (1.02 * 1.02) * (double)p->inv_h) : 0.f;
-  p->sigma = std::sqrt(p->inv_h);
-  p->bins = bins;
+  p->groups.clear();
+  p->job_group.assign(njobs, 0);
+  std::vector<uint8_t> tables;
+  for (int k = 0; k < njobs; ++k) {
+    int gi = -1;
+    for (size_t q = 0; q < p->groups.size(); ++q)
+      if (p->groups[q].bins == jobs[k].bins && p->groups[q].dmax == jobs[k].dmax) gi = (int)q;
+    if (gi < 0) {
+      BatchPlan::Group gr;
+      gr.bins = jobs[k].bins;
+      gr.dmax = jobs[k].dmax;
+      make_rules(j0.type, bpc, 1, gr.bins, gr.dmax, L, &gr.rules);
+      std::vector<uint8_t> table;
+      if (build_table(gr.rules, A, kMixedTableMax, &table, &gr.inv_h, &gr.nt, rho, !ani,
+                      &gr.tflip) != 0)
+        return fail(c, PCF_ERR_UNSUPPORTED, "bin table");
+      gr.rho2 = p->gram ? (float)(rho * rho / (1.02 * 1.02) * (double)gr.inv_h) : 0.f;
+      gr.sigma = std::sqrt(gr.inv_h);
+      gr.table_off = tables.size();
+      tables.insert(tables.end(), table.begin(), table.end());
+      tables.resize((tables.size() + 255) & ~(size_t)255, 0);
+      p->groups.push_back(gr);
+      gi = (int)p->groups.size() - 1;
+    }
+    p->job_group[k] = gi;
+  }
+  p->bins = bins_max;
   p->ani = ani ? 1 : 0;
-  p->nflat = ani ? bins * bins : bins;
+  p->nflat = ani ? bins_max * bins_max : bins_max;
+  p->job_nflat.assign(njobs, 0);
+  p->job_row.assign(njobs, 0);
+  p->total_rows = 0;
+  for (int k = 0; k < njobs; ++k) {
+    p->job_nflat[k] = ani ? jobs[k].bins * jobs[k].bins : jobs[k].bins;
+    p->job_row[k] = p->total_rows;
+    p->total_rows += p->job_nflat[k];
+  }
   p->weighted = all_w1 ? 0 : 1;
-  const PairShape sh = pair_shape(p->nflat, bins, p->weighted != 0, false, j0.lds_copies_log2,
+  const PairShape sh = pair_shape(p->nflat, p->bins, p->weighted != 0, false, j0.lds_copies_log2,
                                   j0.blocks_per_cu);
   p->cshift = sh.cshift;
   p->nblocks = std::max(1, c->ncu * sh.per_cu);
 
-  // Scaled copies of every distinct catalog, one buffer.
+  // Scaled copies of every distinct (catalog, group scale), one buffer.
   p->scale_src.clear();
   p->scale_n.clear();
   p->scale_off.clear();
-  std::vector<int64_t> off_of(ncat, 0);
-  int64_t tot = 0;
-  for (pcf_catalog id : ids) {
+  p->scale_sigma.clear();
+  auto scaled_of = [&](pcf_catalog id, float sigma) -> int64_t {
+    for (size_t q = 0; q < p->scale_src.size(); ++q)
+      if (p->scale_src[q] == grid_of[id]->pts && p->scale_sigma[q] == sigma)
+        return p->scale_off[q];
+    const int64_t off = p->scale_src.empty() ? 0 : p->scale_off.back() + p->scale_n.back();
     p->scale_src.push_back(grid_of[id]->pts);
     p->scale_n.push_back(grid_of[id]->n);
-    p->scale_off.push_back(tot);
-    off_of[id] = tot;
-    tot += grid_of[id]->n;
+    p->scale_off.push_back(off);
+    p->scale_sigma.push_back(sigma);
+    return off;
+  };
+  std::vector<int64_t> xs_off(njobs), ys_off(njobs);
+  for (int k = 0; k < njobs; ++k) {
+    const float sg = p->groups[p->job_group[k]].sigma;
+    xs_off[k] = scaled_of(jobs[k].cat_x, sg);
+    ys_off[k] = scaled_of(jobs[k].hist == PCF_XX ? jobs[k].cat_x : jobs[k].cat_y, sg);
   }
+  const int64_t tot = p->scale_src.empty() ? 0 : p->scale_off.back() + p->scale_n.back();
   HIPCHK(c, p->scaled.ensure(sizeof(float4) * (size_t)std::max<int64_t>(tot, 1)));
 
-  // Work lists of all jobs, concatenated.
+  // Work lists of all jobs, concatenated; each job reaches the cells of its
+  // own range on the batch's grid.
+  double ulo[3], uhi[3];
+  for (int a = 0; a < 3; ++a) {
+    ulo[a] = any ? U.lo[a] : 0.0;
+    uhi[a] = any ? U.hi[a] : 1.0;
+  }
   std::vector<WorkListArgs> wa(njobs);
   for (int k = 0; k < njobs; ++k) {
     const bool auto_ = jobs[k].hist == PCF_XX;
@@ -258,8 +323,17 @@ int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p, GridPins
     wa[k].x_cell_start = gx->cell_start;
     wa[k].y_cell_start = gy->cell_start;
     wa[k].g = p->g;
-    for (int a = 0; a < 3; ++a) wa[k].K[a] = K[a];
-    wa[k].reach = reach;
+    if (jobs[k].dmax == dmax_max) {
+      for (int a = 0; a < 3; ++a) wa[k].K[a] = Kmax[a];
+      wa[k].reach = reach_max;
+    } else {
+      const double reach = pair_reach(ulo, uhi, jobs[k].dmax, bpc, L);
+      for (int a = 0; a < 3; ++a) {
+        wa[k].K[a] = (int)std::ceil(reach / p->g.cs[a]);
+        if (bpc && wa[k].K[a] > p->g.P[a]) wa[k].K[a] = p->g.P[a];
+      }
+      wa[k].reach = (float)(reach * (1.0 + 1e-6));
+    }
     wa[k].auto_ = auto_ ? 1 : 0;
     wa[k].cylinder = ani ? 1 : 0;
     wa[k].map = nullptr;
@@ -274,38 +348,57 @@ int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p, GridPins
     for (int b = p->job_offsets[k]; b < p->job_offsets[k + 1]; b += target)
       p->chunks.push_back({b, std::min(b + target, p->job_offsets[k + 1]), k});
 
-  // Job table.
+  // Device tables and rules of the groups, then the job table.
+  std::vector<BinRules> grules(p->groups.size());
+  for (size_t q = 0; q < p->groups.size(); ++q) grules[q] = p->groups[q].rules;
+  HIPCHK(c, p->table_d.ensure(std::max<size_t>(tables.size(), 1)));
+  HIPCHK(c, p->rules_d.ensure(sizeof(BinRules) * grules.size()));
   std::vector<BatchJob> jd(njobs);
   p->mult.assign(njobs, 1);
   float4* sc = p->scaled.as<float4>();
   for (int k = 0; k < njobs; ++k) {
     const bool auto_ = jobs[k].hist == PCF_XX;
     const pcf_catalog yid = auto_ ? jobs[k].cat_x : jobs[k].cat_y;
+    const BatchPlan::Group& gr = p->groups[p->job_group[k]];
     BinRules r;
-    make_rules(j0.type, bpc, auto_ ? 1 : 0, bins, j0.dmax, L, &r);
-    jd[k].xs = sc + off_of[jobs[k].cat_x];
-    jd[k].ys = sc + off_of[yid];
+    make_rules(j0.type, bpc, auto_ ? 1 : 0, jobs[k].bins, jobs[k].dmax, L, &r);
+    jd[k].xs = sc + xs_off[k];
+    jd[k].ys = sc + ys_off[k];
     jd[k].xo = grid_of[jobs[k].cat_x]->pts;
     jd[k].yo = grid_of[yid]->pts;
-    jd[k].hoff = k * p->nflat;
+    jd[k].hoff = (int)p->job_row[k];
     jd[k].zero = r.zero;
+    jd[k].table = p->table_d.as<uint8_t>() + gr.table_off;
+    jd[k].rules = p->rules_d.as<BinRules>() + p->job_group[k];
+    jd[k].nt = gr.nt;
+    jd[k].bins = gr.bins;
+    jd[k].nflat = p->job_nflat[k];
+    jd[k].Ls = (float)L * gr.sigma;
+    jd[k].tflip = gr.tflip;
+    jd[k].rho2 = gr.rho2;
+    for (int a = 0; a < 3; ++a) {
+      // Scaling by sigma (a power of two) is exact.
+      jd[k].glo[a] = p->g.lo[a] * gr.sigma;
+      jd[k].gcs[a] = p->g.cs[a] * gr.sigma;
+      jd[k].ginv[a] = p->g.inv_cs[a] / gr.sigma;
+    }
     p->mult[k] = auto_ ? 2 : 1;
   }
   HIPCHK(c, p->jobs_d.ensure(sizeof(BatchJob) * njobs));
   HIPCHK(c, p->chunks_d.ensure(sizeof(BatchChunk) * std::max<size_t>(1, p->chunks.size())));
-  HIPCHK(c, p->table_d.ensure(p->nt));
-  HIPCHK(c, p->rules_d.ensure(sizeof(BinRules)));
-  HIPCHK(c, p->hist_d.ensure((sizeof(unsigned long long) + sizeof(double)) * p->nflat * njobs +
-                             64));
+  HIPCHK(c, p->hist_d.ensure((sizeof(unsigned long long) + sizeof(double)) *
+                                 (size_t)p->total_rows + 64));
   HIPCHK(c, hipMemcpyAsync(p->jobs_d.p, jd.data(), sizeof(BatchJob) * njobs,
                            hipMemcpyHostToDevice, c->stream));
   if (!p->chunks.empty())
     HIPCHK(c, hipMemcpyAsync(p->chunks_d.p, p->chunks.data(),
                              sizeof(BatchChunk) * p->chunks.size(), hipMemcpyHostToDevice,
                              c->stream));
-  HIPCHK(c, hipMemcpyAsync(p->table_d.p, table.data(), p->nt, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(p->rules_d.p, &p->rules, sizeof(BinRules), hipMemcpyHostToDevice,
-                           c->stream));
+  if (!tables.empty())
+    HIPCHK(c, hipMemcpyAsync(p->table_d.p, tables.data(), tables.size(), hipMemcpyHostToDevice,
+                             c->stream));
+  HIPCHK(c, hipMemcpyAsync(p->rules_d.p, grules.data(), sizeof(BinRules) * grules.size(),
+                           hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));  // host vectors above go out of scope
 
   // Signature.
@@ -406,17 +499,21 @@ int run_pairs_batch(pcf_ctx* c, const pcf_job* jobs_in, int njobs, uint64_t* cou
     return fail(c, PCF_ERR_UNSUPPORTED, "batched launches are for pair histograms");
   for (int k = 0; k < njobs; ++k) {
     const pcf_job& j = jobs[k];
-    if (j.type != j0.type || j.bins != j0.bins || j.dmax != j0.dmax || j.bpc != j0.bpc ||
-        (j0.bpc && j.box != j0.box) || j.precision != j0.precision ||
-        j.cells_per_rmax != j0.cells_per_rmax || j.lds_copies_log2 != j0.lds_copies_log2 ||
-        j.blocks_per_cu != j0.blocks_per_cu)
-      return fail(c, PCF_ERR_ARG, "jobs of a batch must share type, bins, dmax, box, precision");
+    if (j.type != j0.type || j.bpc != j0.bpc || (j0.bpc && j.box != j0.box) ||
+        j.precision != j0.precision || j.cells_per_rmax != j0.cells_per_rmax ||
+        j.lds_copies_log2 != j0.lds_copies_log2 || j.blocks_per_cu != j0.blocks_per_cu)
+      return fail(c, PCF_ERR_ARG, "jobs of a batch must share type, box and precision");
     if (j.hist != PCF_XX && j.hist != PCF_XY) return fail(c, PCF_ERR_ARG, "pair histogram kind");
     if (!get_cat(c, j.cat_x) || (j.hist == PCF_XY && !get_cat(c, j.cat_y)))
       return fail(c, PCF_ERR_ARG, "unknown catalog id");
+    if (j.bins <= 0 || pcf_hist_size(j.type, j.bins) > kMaxFlatBins)
+      return fail(c, PCF_ERR_ARG, "bins out of range");
+    if (!(j.dmax > 0) || !std::isfinite(j.dmax)) return fail(c, PCF_ERR_ARG, "dmax must be > 0");
   }
   apply_tuning(c, tuned.data(), njobs);
-  const int64_t hs = pcf_hist_size(j0.type, j0.bins);
+  // Each job's output: pcf_hist_size values, one job after the other.
+  std::vector<int64_t> out_off(njobs + 1, 0);
+  for (int k = 0; k < njobs; ++k) out_off[k + 1] = out_off[k] + pcf_hist_size(j0.type, jobs[k].bins);
   // The plain FP64 path and out-of-range arguments go through the single-job
   // driver (same kernels, one launch per job).
   // Histograms the mixed kernel does not serve (bin counts past its table or
@@ -429,14 +526,15 @@ int run_pairs_batch(pcf_ctx* c, const pcf_job* jobs_in, int njobs, uint64_t* cou
       if (X->n >= kRecheckMaxPoints) single = true;
       if (!X->all_w1) any_weighted = true;
     }
-  if (!single && !pair_mixed_fits(j0.bins, j0.type == PCF_2ANI, any_weighted)) single = true;
+  for (int k = 0; k < njobs && !single; ++k)
+    if (!pair_mixed_fits(jobs[k].bins, j0.type == PCF_2ANI, any_weighted)) single = true;
   if (!single) {
     // Argument checks of the single-job driver (range, box) on each job.
-    if (!(j0.dmax > 0) || !std::isfinite(j0.dmax)) return fail(c, PCF_ERR_ARG, "dmax must be > 0");
     if (j0.bpc) {
       if (!(j0.box > 0)) return fail(c, PCF_ERR_ARG, "box side required for periodic boundaries");
-      if (!(j0.dmax < j0.box))
-        return fail(c, PCF_ERR_UNSUPPORTED, "periodic boundaries need dmax < box side");
+      for (int k = 0; k < njobs; ++k)
+        if (!(jobs[k].dmax < j0.box))
+          return fail(c, PCF_ERR_UNSUPPORTED, "periodic boundaries need dmax < box side");
       for (int k = 0; k < njobs; ++k)
         for (pcf_catalog id : {jobs[k].cat_x, jobs[k].hist == PCF_XY ? jobs[k].cat_y : jobs[k].cat_x}) {
           Catalog* X = get_cat(c, id);
@@ -449,8 +547,8 @@ int run_pairs_batch(pcf_ctx* c, const pcf_job* jobs_in, int njobs, uint64_t* cou
   if (single) {
     pcf_stats agg{};
     for (int k = 0; k < njobs; ++k) {
-      int rc = pcf_histogram(c, &jobs[k], counts ? counts + hs * k : nullptr,
-                             weights ? weights + hs * k : nullptr);
+      int rc = pcf_histogram(c, &jobs[k], counts ? counts + out_off[k] : nullptr,
+                             weights ? weights + out_off[k] : nullptr);
       if (rc != PCF_OK) return rc;
       agg.evaluations += c->last.evaluations;
       agg.rechecks += c->last.rechecks;
@@ -496,7 +594,7 @@ int run_pairs_batch(pcf_ctx* c, const pcf_job* jobs_in, int njobs, uint64_t* cou
     HIPCHK(c, enqueue_plan(c, p, (size_t)njobs, false, false, 1));
   }
   HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-  const size_t nf = (size_t)p->nflat * njobs;
+  const size_t nf = (size_t)p->total_rows;
   std::vector<unsigned long long> hc(nf + 8);
   std::vector<double> hw(nf);
   const size_t hist_bytes = sizeof(unsigned long long) * nf;
@@ -509,11 +607,11 @@ int run_pairs_batch(pcf_ctx* c, const pcf_job* jobs_in, int njobs, uint64_t* cou
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (int k = 0; k < njobs; ++k) {
     const uint64_t mult = (uint64_t)p->mult[k];
-    for (int b = 0; b < p->nflat; ++b) {
-      const size_t q = (size_t)k * p->nflat + b;
-      if (counts) counts[(size_t)k * hs + b] = (uint64_t)hc[q] * mult;
+    for (int b = 0; b < p->job_nflat[k]; ++b) {
+      const size_t q = (size_t)p->job_row[k] + b;
+      if (counts) counts[out_off[k] + b] = (uint64_t)hc[q] * mult;
       if (weights)
-        weights[(size_t)k * hs + b] =
+        weights[out_off[k] + b] =
             p->weighted ? hw[q] * (double)mult : (double)((uint64_t)hc[q] * mult);
     }
   }

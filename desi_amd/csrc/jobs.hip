@@ -11,8 +11,10 @@
 //     scratch buffers, sharing the context's resident catalogs and grids (the
 //     grid cache pins grids in use and orders cross-stream builds by events);
 //   * continuous batching: a lane that takes a pair job also takes the queued
-//     pair jobs compatible with it (same type, bins, dmax, box, precision) and
-//     runs them as one persistent launch (pcf_histogram_batch's path);
+//     pair jobs compatible with it (same type, periodicity, box and
+//     precision; bins and dmax may differ, each job keeping its own bin
+//     table, range and scale) and runs them as one persistent launch
+//     (pcf_histogram_batch's path);
 //   * per-ticket host timestamps, so callers can report queueing and latency
 //     percentiles.
 // Results are the same as pcf_histogram's for the job alone (counts bit-exact,
@@ -58,10 +60,10 @@ bool batchable(const pcf_job& a) {
 }
 
 bool compatible(const pcf_job& a, const pcf_job& b) {
-  return batchable(a) && batchable(b) && a.type == b.type && a.bins == b.bins &&
-         a.dmax == b.dmax && a.bpc == b.bpc && (!a.bpc || a.box == b.box) &&
-         a.cells_per_rmax == b.cells_per_rmax && a.lds_copies_log2 == b.lds_copies_log2 &&
-         a.blocks_per_cu == b.blocks_per_cu && a.pair_kernel == b.pair_kernel;
+  return batchable(a) && batchable(b) && a.type == b.type && a.bpc == b.bpc &&
+         (!a.bpc || a.box == b.box) && a.cells_per_rmax == b.cells_per_rmax &&
+         a.lds_copies_log2 == b.lds_copies_log2 && a.blocks_per_cu == b.blocks_per_cu &&
+         a.pair_kernel == b.pair_kernel;
 }
 
 }  // namespace
@@ -150,10 +152,13 @@ struct JobEngine {
       return;
     }
     std::vector<pcf_job> jobs;
-    for (Ticket* t : take) jobs.push_back(t->job);
-    const int64_t hs = pcf_hist_size(jobs[0].type, jobs[0].bins);
-    std::vector<uint64_t> c((size_t)hs * jobs.size());
-    std::vector<double> w((size_t)hs * jobs.size());
+    std::vector<int64_t> off(1, 0);
+    for (Ticket* t : take) {
+      jobs.push_back(t->job);
+      off.push_back(off.back() + pcf_hist_size(t->job.type, t->job.bins));
+    }
+    std::vector<uint64_t> c((size_t)off.back());
+    std::vector<double> w((size_t)off.back());
     const int rc = run_pairs_batch(lane, jobs.data(), (int)jobs.size(), c.data(), w.data());
     if (rc != PCF_OK) {
       // One job of the batch is refused (its catalog outside the box, say):
@@ -171,8 +176,8 @@ struct JobEngine {
       t->rc = PCF_OK;
       t->st = lane->last;
       t->st.batched_jobs = (int32_t)take.size();
-      std::copy(c.begin() + hs * k, c.begin() + hs * (k + 1), t->counts.begin());
-      std::copy(w.begin() + hs * k, w.begin() + hs * (k + 1), t->weights.begin());
+      std::copy(c.begin() + off[k], c.begin() + off[k + 1], t->counts.begin());
+      std::copy(w.begin() + off[k], w.begin() + off[k + 1], t->weights.begin());
     }
   }
 

@@ -208,16 +208,20 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
     const PairItem* __restrict__ items, int* __restrict__ counter,
     const uint8_t* __restrict__ table, const BinRules* __restrict__ rules,
     unsigned long long* __restrict__ hist, double* __restrict__ whist,
-    unsigned long long* __restrict__ stats, KArgs k, const BatchJob* __restrict__ jobs,
+    unsigned long long* __restrict__ stats, KArgs k_in, const BatchJob* __restrict__ jobs,
     const BatchChunk* __restrict__ chunks) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ uint8_t tab[kMaxTable];
-  const int nrows = k.nflat + 2;
+  // The job parameters in force; a batched launch switches them per chunk
+  // (jobs of one batch may differ in bins, range and scale).  k_in's nflat and
+  // bins are the largest of the launch and size the LDS layout.
+  KArgs k = k_in;
+  const int nrows = k_in.nflat + 2;
   constexpr int ncopy = 1 << CS;
   uint32_t* hc = reinterpret_cast<uint32_t*>(smem);
   const int hc_bytes = (nrows * ncopy * 4 + 15) & ~15;
   double* sT = reinterpret_cast<double*>(smem + hc_bytes);
-  double* hw = sT + ((k.bins + 2) & ~1);
+  double* hw = sT + ((k_in.bins + 2) & ~1);
   // Per-wave queue of the pairs whose table cell straddles an edge.
   int2* const wq =
       reinterpret_cast<int2*>(hw + (WEIGHTED ? nrows * ncopy : 0)) + (threadIdx.x >> 6) * kRecheckCap;
@@ -237,10 +241,10 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
   const float sel_z = grp == 2 ? 1.0f : 0.0f, sel_w = grp == 3 ? 1.0f : 0.0f;
   uint32_t* const hc_lane = hc + (lane & (ncopy - 1));
   double* const hw_lane = hw + (lane & (ncopy - 1));
-  const unsigned ntm1 = (unsigned)(k.nt - 1);
-  const unsigned out_row = (unsigned)k.nflat;
-  const unsigned amb_row = (unsigned)k.nflat + 1;
-  const unsigned bins = (unsigned)k.bins;
+  unsigned ntm1 = (unsigned)(k.nt - 1);
+  unsigned out_row = (unsigned)k.nflat;
+  unsigned amb_row = (unsigned)k.nflat + 1;
+  unsigned bins = (unsigned)k.bins;
   unsigned long long evals = 0;
   unsigned long long nre = 0;
   unsigned long long ndirect = 0;  // Gram path: items run by the direct path
@@ -690,6 +694,7 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
     flush(0, false);
   } else {
     __shared__ int s_chunk, s_next;
+    const uint8_t* loaded = table;  // the table (and rules) now in LDS
     for (;;) {
       if (threadIdx.x == 0) {
         s_chunk = atomicAdd(counter, 1);
@@ -700,6 +705,31 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
       if (c >= k.nitems) break;  // nitems = number of chunks in batch mode
       const int cb = chunks[c].begin, ce = chunks[c].end;
       const BatchJob jd = jobs[chunks[c].job];
+      // The chunk's job parameters (the same for every thread of the
+      // workgroup); a job with another table reloads it and its thresholds.
+      if (jd.table != loaded) {
+        for (int t = threadIdx.x; t < jd.nt; t += kBlockMixed) tab[t] = jd.table[t];
+        for (int t = threadIdx.x; t <= jd.bins; t += kBlockMixed) sT[t] = jd.rules->T[t];
+        loaded = jd.table;
+        __syncthreads();
+      }
+      k.nt = jd.nt;
+      k.bins = jd.bins;
+      k.nflat = jd.nflat;
+      k.R = jd.rules->R;
+      k.le = jd.rules->le;
+      k.L = jd.Ls;
+      k.tflip = jd.tflip;
+      k.rho2 = jd.rho2;
+      for (int a = 0; a < 3; ++a) {
+        k.glo[a] = jd.glo[a];
+        k.gcs[a] = jd.gcs[a];
+        k.ginv[a] = jd.ginv[a];
+      }
+      ntm1 = (unsigned)(k.nt - 1);
+      out_row = (unsigned)k.nflat;
+      amb_row = (unsigned)k.nflat + 1;
+      bins = (unsigned)k.bins;
       for (;;) {
         int it = 0;
         if (lane == 0) it = cb + atomicAdd(&s_next, 1);

@@ -121,6 +121,7 @@ int build_table(const BinRules& r, double A, int nt_max, std::vector<uint8_t>* t
 // Largest tile half-diagonal (unscaled units) the Gram pair path's error
 // bound is built for on geometry g with n points per catalog.
 double gram_rho(const GridGeom& g, int64_t n);
+double pair_reach(const double lo[3], const double hi[3], float dmax, int bpc, float box);
 int choose_geometry(const Catalog* X, const Catalog* Y, float dmax, int bpc, float box,
                     int cells_per_rmax, GridGeom* g, int K[3], float* reach_out, int* m_out,
                     int zsub = 1);

@@ -159,13 +159,22 @@ hipError_t convert_grid64(const float4* pts, int64_t n, double4* out, hipStream_
 hipError_t build_grid64(DevGrid* grid, hipStream_t s);
 
 // One histogram of a batched pair launch (k_pairs_mixed<..., BATCH = 1>).
+// Jobs of one batch share the histogram type, periodicity, box and grid; each
+// has its own bins, range (dmax), d2 -> bin table, exact rules and scale.
 struct BatchJob {
-  const float4* xs;  // i side, scaled copy
+  const float4* xs;  // i side, scaled copy (this job's scale)
   const float4* ys;  // j side, scaled copy
   const float4* xo;  // i side, cell-ordered originals
   const float4* yo;  // j side, cell-ordered originals
   int hoff;          // first row of this job's histogram in the batch output
   int zero;          // zero-separation rule of this job (BinRules::zero)
+  const uint8_t* table;    // this job's d2 -> bin table (device)
+  const BinRules* rules;   // and exact rules (device)
+  int nt, bins, nflat;
+  float Ls;          // box side in this job's scaled units
+  float tflip;       // 2iso: flipped-table top
+  float rho2;        // Gram path bound (scaled units)
+  float glo[3], gcs[3], ginv[3];  // grid geometry in this job's scaled units
 };
 // A run of work items [begin, end) that all belong to job `job`.
 struct BatchChunk {

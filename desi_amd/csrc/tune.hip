@@ -77,8 +77,9 @@ int run_trial(pcf_ctx* c, const std::vector<pcf_job>& base, const Trial& t,
     j.blocks_per_cu = t.per_cu;
     j.pair_kernel = t.kernel;
   }
-  const int64_t hs = pcf_hist_size(jobs[0].type, jobs[0].bins);
-  counts->assign((size_t)hs * jobs.size(), 0);
+  int64_t total = 0;
+  for (const auto& j : jobs) total += pcf_hist_size(j.type, j.bins);
+  counts->assign((size_t)total, 0);
   int rc;
   if (jobs.size() == 1) rc = pcf_histogram(c, jobs.data(), counts->data(), nullptr);
   else rc = run_pairs_batch(c, jobs.data(), (int)jobs.size(), counts->data(), nullptr);
@@ -102,7 +103,9 @@ extern "C" int pcf_autotune(pcf_ctx* c, const pcf_job* jobs, int32_t njobs, pcf_
   if (jobs[0].precision != PCF_PREC_MIXED)
     return fail(c, PCF_ERR_UNSUPPORTED, "autotuning covers the mixed-precision path");
   bool weighted = false;
+  int bins_max = 0;  // the batch's largest bins sizes the LDS layout
   for (int i = 0; i < njobs; ++i) {
+    bins_max = std::max(bins_max, jobs[i].bins);
     Catalog* X = get_cat(c, jobs[i].cat_x);
     Catalog* Y = jobs[i].hist == PCF_XY ? get_cat(c, jobs[i].cat_y) : X;
     if (!X || !Y) return fail(c, PCF_ERR_ARG, "unknown catalog id");
@@ -179,8 +182,8 @@ extern "C" int pcf_autotune(pcf_ctx* c, const pcf_job* jobs, int32_t njobs, pcf_
   }
   // 2. LDS copies of the histogram rows (log2 + 1 in the job)
   {
-    const int nflat = type == PCF_2ANI ? jobs[0].bins * jobs[0].bins : jobs[0].bins;
-    const int cs_def = pair_shape(nflat, jobs[0].bins, weighted, false, 0, 0).cshift;
+    const int nflat = type == PCF_2ANI ? bins_max * bins_max : bins_max;
+    const int cs_def = pair_shape(nflat, bins_max, weighted, false, 0, 0).cshift;
     for (int cs = std::max(0, cs_def - 3); cs <= std::min(6, cs_def + 1); ++cs) {
       if (cs == cs_def) continue;
       double tm;
@@ -194,10 +197,10 @@ extern "C" int pcf_autotune(pcf_ctx* c, const pcf_job* jobs, int32_t njobs, pcf_
   }
   // 3. resident workgroups per CU
   {
-    const int nflat = type == PCF_2ANI ? jobs[0].bins * jobs[0].bins : jobs[0].bins;
+    const int nflat = type == PCF_2ANI ? bins_max * bins_max : bins_max;
     const int cs = best.lds ? best.lds - 1
-                            : pair_shape(nflat, jobs[0].bins, weighted, false, 0, 0).cshift;
-    const int pc_def = pair_shape(nflat, jobs[0].bins, weighted, false, cs + 1, 0).per_cu;
+                            : pair_shape(nflat, bins_max, weighted, false, 0, 0).cshift;
+    const int pc_def = pair_shape(nflat, bins_max, weighted, false, cs + 1, 0).per_cu;
     for (int pc = 1; pc < pc_def; ++pc) {
       double tm;
       Trial tr{best.m, best.lds, pc, best.kernel};

@@ -159,11 +159,13 @@ int pcf_histogram(pcf_ctx* ctx, const pcf_job* job, uint64_t* counts, double* we
 /* Several pair histograms in one persistent launch.  Replaces the per-random
  * loops of the reference's many-random drivers, which launch XX and XY once
  * per random file (cuda/pcf2iso.cuh:96-104, cuda/pcf2ani.cuh:90-98).  All jobs
- * share type (PCF_2ISO or PCF_2ANI), bins, dmax, bpc, box, precision and
- * cells_per_rmax; hist is PCF_XX or PCF_XY; the catalogs may differ in size.
- * counts / weights receive njobs consecutive histograms of pcf_hist_size
- * values (either may be NULL), each equal to what pcf_histogram returns for
- * that job (counts bit-exact, weights to 1e-12 relative).  The plan and its
+ * share type (PCF_2ISO or PCF_2ANI), bpc, box, precision and the tuning
+ * knobs; hist is PCF_XX or PCF_XY; the catalogs may differ in size, and bins
+ * and dmax may differ between jobs (each keeps its own bin table and range;
+ * the batch's grid is cut for its largest dmax).  counts / weights receive
+ * the njobs histograms one after the other, pcf_hist_size(type, bins of that
+ * job) values each (either may be NULL), each equal to what pcf_histogram
+ * returns for that job (counts bit-exact, weights to 1e-12 relative).  The plan and its
  * HIP graph stay on the context: repeating the same batch over the same
  * resident catalogs replays the graph (pcf_last_stats: graph_replay, chunks).
  * A batch whose jobs set rebuild_grid re-runs the device grid build (counting
@@ -264,8 +266,8 @@ typedef int64_t pcf_ticket;
 int pcf_set_hbm_budget(pcf_ctx* ctx, int64_t bytes, int32_t max_inflight);
 /* Continuous batching: a lane taking a pair job (PCF_2ISO / PCF_2ANI, mixed
  * precision) also takes up to max_batch - 1 queued compatible pair jobs (same
- * type, bins, dmax, box and tuning knobs) into one persistent launch.
- * 1 (the default) disables it. */
+ * type, periodicity, box and tuning knobs; any bins and dmax) into one
+ * persistent launch.  1 (the default) disables it. */
 int pcf_set_max_batch(pcf_ctx* ctx, int32_t max_batch);
 int pcf_submit(pcf_ctx* ctx, const pcf_job* job, pcf_ticket* ticket);
 int pcf_poll(pcf_ctx* ctx, pcf_ticket ticket, int32_t* done);

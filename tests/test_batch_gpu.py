@@ -151,8 +151,14 @@ def test_batch_argument_errors(engine):
     d = synth.uniform(200, 100.0, seed=91)
     cd = engine.upload(d)
     try:
+        # jobs of one batch share the histogram type and periodicity
         jobs = [da.make_job(da.PCF_2ISO, da.PCF_XX, bins=10, dmax=20.0, cat_x=cd),
-                da.make_job(da.PCF_2ISO, da.PCF_XX, bins=12, dmax=20.0, cat_x=cd)]
+                da.make_job(da.PCF_2ANI, da.PCF_XX, bins=10, dmax=20.0, cat_x=cd)]
+        with pytest.raises(da.PcfError):
+            engine.histogram_batch(jobs)
+        jobs = [da.make_job(da.PCF_2ISO, da.PCF_XX, bins=10, dmax=20.0, cat_x=cd),
+                da.make_job(da.PCF_2ISO, da.PCF_XX, bins=10, dmax=20.0, cat_x=cd, bpc=True,
+                            box=100.0)]
         with pytest.raises(da.PcfError):
             engine.histogram_batch(jobs)
         jobs = [da.make_job(da.PCF_3ISO, da.PCF_XXX, bins=10, dmax=20.0, cat_x=cd)]
@@ -160,3 +166,42 @@ def test_batch_argument_errors(engine):
             engine.histogram_batch(jobs)
     finally:
         engine.release(cd)
+
+
+@pytest.mark.parametrize("type_", [da.PCF_2ISO, da.PCF_2ANI])
+@pytest.mark.parametrize("bpc", [False, True])
+def test_heterogeneous_batch(engine, type_, bpc):
+    """Jobs of one launch with different bins and dmax (each with its own bin
+    table, range and scale on the grid of the largest dmax), weighted and
+    not: every histogram equals its single run, and the oracle."""
+    d = synth.neyman_scott(60, 32, 6.0, 300.0, seed=81, weights="uniform")
+    r1 = synth.uniform(2500, 300.0, seed=82)
+    r2 = synth.uniform(1800, 300.0, seed=83, weights="uniform")
+    cd, c1, c2 = engine.upload(d), engine.upload(r1), engine.upload(r2)
+    print(f"case heterogeneous batch type={type_} bpc={bpc}", flush=True)
+    try:
+        J = da.make_job
+        kw = dict(bpc=bpc, box=300.0)
+        jobs = [J(type_, da.PCF_XX, bins=20, dmax=40.0, cat_x=cd, **kw),
+                J(type_, da.PCF_XY, bins=30, dmax=80.0, cat_x=cd, cat_y=c1, **kw),
+                J(type_, da.PCF_XX, bins=40, dmax=110.0, cat_x=c2, **kw),
+                J(type_, da.PCF_XY, bins=20, dmax=40.0, cat_x=c1, cat_y=c2, **kw),
+                J(type_, da.PCF_XY, bins=30, dmax=65.5, cat_x=cd, cat_y=c2, **kw)]
+        res = engine.histogram_batch(jobs)
+        assert [len(c) for c, _ in res] == [da.hist_size(j.type, j.bins) for j in jobs]
+        _same(res, _singles(engine, jobs))
+        cats = {cd: d, c1: r1, c2: r2}
+        ani = type_ == da.PCF_2ANI
+        for j, (c, w) in zip(jobs, res):
+            b = None if j.hist == da.PCF_XX else cats[j.cat_y]
+            co, wo = O.pairs(cats[j.cat_x], b, ani=ani, bpc=bpc, box=300.0, bins=j.bins,
+                             dmax=j.dmax)
+            assert np.array_equal(c, co)
+            assert np.all(np.abs(w - wo) <= 1e-12 * np.maximum(np.abs(wo), 1e-300))
+        # the plan replays
+        res2 = engine.histogram_batch(jobs)
+        assert engine.stats()["graph_replay"] == 1
+        _same(res2, res)
+    finally:
+        for c in (cd, c1, c2):
+            engine.release(c)

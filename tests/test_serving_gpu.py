@@ -83,6 +83,29 @@ def test_mix_equals_alone(served, lanes, max_batch):
     _check_mix(e, jobs, results)
 
 
+def test_heterogeneous_batches_in_the_mix(served):
+    """Pair requests of one type and box with different bins and dmax queue
+    behind a long job on one lane and are served as one launch: each result
+    equals the request alone."""
+    e = served
+    ids = _catalogs(e)
+    e.set_hbm_budget(0, 1)
+    e.set_max_batch(8)
+    J = da.make_job
+    first = J(da.PCF_2ISO, da.PCF_XY, bins=30, dmax=80.0, cat_x=ids["big"], cat_y=ids["big2"])
+    jobs = [J(da.PCF_2ISO, da.PCF_XX, bins=b, dmax=dm, cat_x=ids[x], bpc=True, box=250.0)
+            for b, dm, x in ((20, 45.0, "u4k"), (30, 60.0, "ns3k"), (40, 97.5, "u9k"),
+                             (30, 120.0, "u1k"))]
+    jobs += [J(da.PCF_2ISO, da.PCF_XY, bins=b, dmax=dm, cat_x=ids["ns3k"], cat_y=ids["u4k"],
+               bpc=True, box=250.0) for b, dm in ((20, 70.0), (40, 52.0))]
+    t0 = e.submit(first)
+    tickets = [e.submit(j) for j in jobs]
+    e.wait(t0, first.type, first.bins)
+    results = [e.wait(t, j.type, j.bins) for t, j in zip(tickets, jobs)]
+    assert max(r[2]["batched_jobs"] for r in results) > 1
+    _check_mix(e, jobs, results)
+
+
 def test_tiny_budget_serialises(served):
     e = served
     ids = _catalogs(e)
