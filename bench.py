@@ -301,84 +301,123 @@ def batch_leg(e, steps, no_tune=False):
                          "unit": "TFLOP/s", "frac": ach / PEAK_FP32, "traffic": None}}
 
 
-def serving_leg(requests=400, rate=50.0, lanes=4, max_batch=8):
-    """Open-loop serving: Poisson arrivals of a request mix over resident
-    catalogs, answered by the engine (pcf_submit / pcf_wait) with `lanes`
-    concurrent lanes and continuous batching.  Mix: 50% 2iso, 30% 2ani, 20%
-    3iso; bins in {20, 30, 40}; periodic for half the requests; catalogs from a
-    pool of 24 (sizes log-uniform in [1024, 1048576], uniform or Neyman-Scott,
-    box 250 or 1000); rmax U[40, 120] for pairs, U[20, 40] and N <= 16384 for
-    triplets (a 3iso request at N = 1e6 and rmax 120 in box 250 would be
-    ~1e17 triplets).  Latency = submit -> result ready, per request.  With
-    rmax continuous, queued requests rarely share a setting, so launches are
-    seldom shared (the serving tests cover batched launches)."""
+# Expected ordered triplets above which a 3iso request of the serving mix is
+# capped: the mix's N (log-uniform up to 1,048,576) with rmax up to 120 in a box
+# of 250 would ask for ~1e17 triplets (about a day of GPU time) in one
+# request.  The cap lowers N until N (N V / L^3)^2 <= TRIPLET_CAP, V the
+# sphere of radius rmax: at most ~1e10 triplet evaluations (~10 ms).
+TRIPLET_CAP = 1e10
+
+
+def serving_leg(requests=2000, rate=100.0, lanes=4, max_batch=8, pool_size=48, check=40):
+    """Open-loop serving of the serving configuration: a Poisson stream of
+    `requests` requests; each picks its type (2iso 50%, 2ani 30%, 3iso 20%),
+    N log-uniform in [1024, 1048576], bins in {20, 30, 40}, rmax U[40, 120],
+    box 250 or 1000, periodic 50%, uniform or Neyman-Scott data, all from its
+    own seed.  The catalogs are drawn from a pool of `pool_size` resident
+    catalogs (sizes log-uniform over the same range, uniform or Neyman-Scott,
+    both boxes): a request's seed picks the pool catalog of its box nearest
+    its N, so grids stay resident and are reused across requests (DR and RR
+    of the same catalogs).  3iso requests are capped by TRIPLET_CAP.  The
+    engine runs `lanes` concurrent lanes with HBM-budget admission and
+    continuous batching of up to `max_batch` compatible pair requests
+    (different bins and rmax allowed).  Latency = submit -> result ready.  A
+    sample of `check` requests is recomputed alone and must match."""
     rng = np.random.default_rng(31)
+    lo, hi = np.log(1024), np.log(1048576)
     pool = []  # (box, n, xyzw)
-    for k in range(24):
+    for k in range(pool_size):
         box = 250.0 if k % 2 == 0 else 1000.0
-        n = int(round(np.exp(rng.uniform(np.log(1024), np.log(1048576)))))
-        if k % 3 == 0:
-            xyzw = synth.neyman_scott(max(1, n // 32), 32, 5.0, box, seed=300 + k)
-        else:
+        n = int(round(np.exp(lo + (hi - lo) * (k // 2 + 0.5) / (pool_size // 2))))
+        if k % 4 < 2:
             xyzw = synth.uniform(n, box, seed=300 + k)
+        else:
+            xyzw = synth.neyman_scott(max(1, n // 32), 32, 5.0, box, seed=300 + k)
         pool.append((box, len(xyzw), xyzw))
+    by_box = {b: sorted((i for i, (bb, _, _) in enumerate(pool) if bb == b),
+                        key=lambda i: pool[i][1]) for b in (250.0, 1000.0)}
+
+    def pick(box, n):
+        cand = by_box[box]
+        sizes = np.array([pool[i][1] for i in cand])
+        return cand[int(np.argmin(np.abs(np.log(sizes) - np.log(n))))]
+
+    jobs, capped = [], 0
+    for q in range(requests):
+        rq = np.random.default_rng(1000 + q)
+        u = rq.random()
+        type_ = da.PCF_2ISO if u < 0.5 else (da.PCF_2ANI if u < 0.8 else da.PCF_3ISO)
+        n = float(np.exp(rq.uniform(lo, hi)))
+        bins = int(rq.choice([20, 30, 40]))
+        dmax = float(rq.uniform(40.0, 120.0))
+        box = 250.0 if rq.random() < 0.5 else 1000.0
+        bpc = bool(rq.random() < 0.5)
+        cross = rq.random() < 0.5
+        if type_ == da.PCF_3ISO:
+            vol = 4.0 / 3.0 * np.pi * dmax ** 3
+            nmax = (TRIPLET_CAP * box ** 6 / vol ** 2) ** (1.0 / 3.0)
+            if n > nmax:
+                n, capped = nmax, capped + 1
+        x = pick(box, n)
+        y = pick(box, n * float(np.exp(rq.uniform(-1.0, 1.0))))
+        if type_ == da.PCF_3ISO:
+            hist = da.PCF_XXY if cross else da.PCF_XXX
+        else:
+            hist = da.PCF_XY if cross else da.PCF_XX
+        jobs.append((type_, hist, bins, dmax, box, bpc, x, y if cross else -1))
     with da.Engine(0) as e:
-        ids = [e.upload(x) for _, _, x in pool]
+        ids = [e.upload(xyzw) for _, _, xyzw in pool]
+        made = [da.make_job(t, h, bins=b, dmax=dm, cat_x=ids[x], cat_y=ids[y] if y >= 0 else -1,
+                            bpc=bp, box=bx) for t, h, b, dm, bx, bp, x, y in jobs]
         e.set_hbm_budget(0, lanes)
         e.set_max_batch(max_batch)
-        jobs = []
-        for _ in range(requests):
-            u = rng.random()
-            box = 250.0 if rng.random() < 0.5 else 1000.0
-            cand = [i for i, (b, n, _) in enumerate(pool) if b == box]
-            bpc = bool(rng.random() < 0.5)
-            bins = int(rng.choice([20, 30, 40]))
-            if u < 0.8:
-                type_ = da.PCF_2ISO if u < 0.5 else da.PCF_2ANI
-                dmax = float(rng.uniform(40.0, 120.0))
-                x = int(rng.choice(cand))
-                if rng.random() < 0.5:
-                    jobs.append(da.make_job(type_, da.PCF_XX, bins=bins, dmax=dmax, cat_x=ids[x],
-                                            bpc=bpc, box=box))
-                else:
-                    y = int(rng.choice(cand))
-                    jobs.append(da.make_job(type_, da.PCF_XY, bins=bins, dmax=dmax, cat_x=ids[x],
-                                            cat_y=ids[y], bpc=bpc, box=box))
-            else:
-                small = [i for i in cand if pool[i][1] <= 16384] or cand[:1]
-                x = int(rng.choice(small))
-                dmax = float(rng.uniform(20.0, 40.0))
-                jobs.append(da.make_job(da.PCF_3ISO, da.PCF_XXX, bins=bins, dmax=dmax,
-                                        cat_x=ids[x], bpc=bpc, box=box))
-        gaps = rng.exponential(1.0 / rate, requests)
+        gaps = np.random.default_rng(32).exponential(1.0 / rate, requests)
         tickets = []
         t0 = time.perf_counter()
         t_next = t0
-        for j, g in zip(jobs, gaps):
+        for j, g in zip(made, gaps):
             t_next += g
             d = t_next - time.perf_counter()
             if d > 0:
                 time.sleep(d)
             tickets.append(e.submit(j))
-        lat, queue, useful, batched = [], [], 0, 0
-        for t, j in zip(tickets, jobs):
-            c, _, st = e.wait(t, j.type, j.bins)
+        lat, queue, useful_pairs, useful_trip, batched = [], [], 0, 0, 0
+        results = []
+        for t, j in zip(tickets, made):
+            c, w, st = e.wait(t, j.type, j.bins)
             lat.append(st["latency_ms"])
             queue.append(st["queue_ms"])
             batched += st["batched_jobs"] > 1
             if j.type in (da.PCF_2ISO, da.PCF_2ANI):
-                useful += int(c.sum())
+                useful_pairs += int(c.sum())
+            else:
+                useful_trip += int(c.sum())
+            results.append((c, w))
         t1 = time.perf_counter()
+        # Spot check: a sample of requests recomputed alone.
+        srng = np.random.default_rng(33)
+        for k in srng.choice(requests, size=min(check, requests), replace=False):
+            c1, w1 = e.histogram(made[k])
+            c, w = results[k]
+            if not np.array_equal(c, c1) or not np.all(np.abs(w - w1) <= 1e-12 * np.abs(w1)):
+                raise SystemExit(f"serving: request {k} differs from the same request alone")
     lat = np.array(lat)
-    return {"workload": f"{requests} Poisson requests at {rate:g}/s: 50% 2iso, 30% 2ani, "
-                        "20% 3iso over 24 resident catalogs (N 1024..1048576, box 250/1000, "
-                        "uniform or clustered), bins 20/30/40, rmax U[40,120] (3iso U[20,40], "
-                        f"N <= 16384), 50% periodic; {lanes} lanes, batching up to {max_batch}",
+    types = [t for t, *_ in jobs]
+    return {"workload": f"{requests} Poisson requests at {rate:g}/s: 2iso/2ani/3iso "
+                        f"{types.count(da.PCF_2ISO)}/{types.count(da.PCF_2ANI)}/"
+                        f"{types.count(da.PCF_3ISO)}, N log-uniform [1024, 1048576] over a pool "
+                        f"of {pool_size} resident catalogs (uniform or Neyman-Scott, box 250 or "
+                        f"1000), bins 20/30/40, rmax U[40,120], 50% periodic, 50% cross; "
+                        f"{capped} 3iso requests capped at {TRIPLET_CAP:g} expected triplets; "
+                        f"{lanes} lanes, batching up to {max_batch}",
             "requests": requests, "wall_s": t1 - t0, "requests_per_s": requests / (t1 - t0),
             "p50_ms": float(np.percentile(lat, 50)), "p90_ms": float(np.percentile(lat, 90)),
             "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
             "mean_queue_ms": float(np.mean(queue)), "batched_requests": batched,
-            "pair_requests_useful_pairs_per_s": useful / (t1 - t0)}
+            "capped_3iso_requests": capped,
+            "useful_pairs_per_s": useful_pairs / (t1 - t0),
+            "useful_triplets_per_s": useful_trip / (t1 - t0),
+            "checked_alone": min(check, requests)}
 
 
 def analytic_leg(e):
@@ -479,7 +518,9 @@ def main():
     ap.add_argument("--no-analytic", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--serving-requests", type=int, default=400)
+    ap.add_argument("--serving-requests", type=int, default=2000)
+    ap.add_argument("--serving-rate", type=float, default=100.0,
+                    help="Poisson arrival rate of the serving leg, requests/s")
     ap.add_argument("--serving-max-batch", type=int, default=8,
                     help="most queued pair requests one serving launch may take")
     ap.add_argument("--no-tune", action="store_true",
@@ -540,6 +581,7 @@ def main():
         bat = None if args.no_batch else batch_leg(e, args.batch_steps, args.no_tune)
 
     serving = None if args.no_serving else serving_leg(args.serving_requests,
+                                                       rate=args.serving_rate,
                                                        max_batch=args.serving_max_batch)
     achieved = evals * FLOPS_PER_PAIR / (kms * 1e-3)
     # The dominant kernel: the Gram form forms d2 with four chained FMAs on the
