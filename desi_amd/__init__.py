@@ -36,7 +36,8 @@ EXPORTED_SYMBOLS = (
     "pcf_catalog_size", "pcf_histogram", "pcf_last_stats", "pcf_analytic_rr",
     "pcf_analytic_3iso", "pcf_histogram_batch", "pcf_autotune", "pcf_set_max_batch",
     "pcf_set_hbm_budget", "pcf_submit", "pcf_poll", "pcf_wait", "pcf_estimator",
-    "pcf_histogram_noalloc", "pcf_selftest_scan",
+    "pcf_histogram_noalloc", "pcf_selftest_scan", "pcf_tune_save", "pcf_tune_load",
+    "pcf_tune_lookup",
 )
 # Bytes of the static device array pcf_histogram_noalloc works in (per GPU).
 PCF_NOALLOC_SCRATCH_BYTES = 128 * 1024 * 1024
@@ -114,6 +115,9 @@ def load_library(path=LIB_PATH):
     L.pcf_analytic_rr.argtypes = [vp, c_i32, c_i64, ctypes.c_float, ctypes.c_float, vp]
     L.pcf_histogram_batch.argtypes = [vp, ctypes.POINTER(PcfJob), c_i32, vp, vp]
     L.pcf_autotune.argtypes = [vp, ctypes.POINTER(PcfJob), c_i32, ctypes.POINTER(PcfTuning)]
+    L.pcf_tune_save.argtypes = [vp, ctypes.c_char_p]
+    L.pcf_tune_load.argtypes = [vp, ctypes.c_char_p]
+    L.pcf_tune_lookup.argtypes = [vp, ctypes.POINTER(PcfJob), c_i32, ctypes.POINTER(PcfTuning)]
     L.pcf_analytic_3iso.argtypes = [vp, c_i32, c_i32, ctypes.c_float, ctypes.c_float, c_i64,
                                     c_i32, c_i32, vp, vp, vp, vp]
     L.pcf_estimator.argtypes = [vp, c_i32, c_i64, vp, vp, vp, vp, ctypes.c_double,
@@ -270,6 +274,25 @@ class Engine:
         t = PcfTuning()
         self._check(self.lib.pcf_autotune(self.ctx, arr, len(jobs), ctypes.byref(t)))
         return t.as_dict()
+
+    def tune_save(self, path):
+        """Write the context's tuning cache to a file (pcf_tune_save)."""
+        self._check(self.lib.pcf_tune_save(self.ctx, os.fsencode(path)))
+
+    def tune_load(self, path):
+        """Merge a tuning cache file into the context (pcf_tune_load)."""
+        self._check(self.lib.pcf_tune_load(self.ctx, os.fsencode(path)))
+
+    def tune_lookup(self, jobs):
+        """The cached tuning of one job or a batch, or None (pcf_tune_lookup)."""
+        if isinstance(jobs, PcfJob):
+            jobs = [jobs]
+        arr = (PcfJob * len(jobs))(*jobs)
+        t = PcfTuning()
+        rc = self.lib.pcf_tune_lookup(self.ctx, arr, len(jobs), ctypes.byref(t))
+        if rc < 0:
+            self._check(rc)
+        return t.as_dict() if rc == 1 else None
 
     def analytic_3iso(self, cat, bins, dmax, box, np_=0, ptt=0, bn_ref=0):
         """Analytic DDR (= DRR) and RRR of the periodic 3iso case plus the

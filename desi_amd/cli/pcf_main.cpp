@@ -21,6 +21,12 @@
 //     full bins^5 histogram with the reference's save_histogram5D format.
 // Environment: PCF_DATA_DIR / PCF_RESULTS_DIR override ../data/ and
 // ../results/; PCF_PRECISION=fp64 selects the plain FP64 path.
+// PCF_TUNE_CACHE=<file> loads the pair kernels' tuning cache from the file at
+// start (a missing file is an empty cache; one from another device model is
+// reported and ignored) and writes it back at the end; with PCF_AUTOTUNE=1
+// every pair histogram whose workload the cache does not hold is autotuned
+// first (pcf_autotune), so the second run of a workload reuses the first's
+// tuning.
 // Extension: -e also writes the Landy-Szalay estimator of each random file,
 // formed on the device by pcf_estimator as the reference's analysis scripts
 // form it (cpp/2PCFisotropic/normal/graph_2DPF.py:8-14,
@@ -32,6 +38,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
@@ -129,10 +136,20 @@ struct Run {
     return j;
   }
 
+  bool autotune = false;  // PCF_AUTOTUNE=1
+
   int hist(int type, int h, pcf_catalog x, pcf_catalog y, std::vector<double>* out) {
     int64_t n = pcf_hist_size(type, o.bn);
     out->assign((size_t)n, 0.0);
     pcf_job j = job(type, h, x, y);
+    if (autotune && (type == PCF_2ISO || type == PCF_2ANI) && precision == PCF_PREC_MIXED) {
+      const int found = pcf_tune_lookup(ctx, &j, 1, nullptr);
+      if (found < 0) return found;
+      if (found == 0) {
+        const int r = pcf_autotune(ctx, &j, 1, nullptr);
+        if (r != PCF_OK) return r;
+      }
+    }
     return pcf_histogram(ctx, &j, nullptr, out->data());
   }
 
@@ -288,6 +305,16 @@ int main(int argc, char** argv) {
 
   int rc = pcf_create(&run.ctx, 0);
   if (rc != PCF_OK) return die(nullptr, rc, "pcf_create");
+  const string tune_cache = env_or("PCF_TUNE_CACHE", "");
+  run.autotune = env_or("PCF_AUTOTUNE", "0") == "1";
+  if (!tune_cache.empty()) {
+    FILE* tf = std::fopen(tune_cache.c_str(), "r");
+    if (tf) {
+      std::fclose(tf);
+      if (pcf_tune_load(run.ctx, tune_cache.c_str()) != PCF_OK)
+        std::cerr << "PCF: ignoring the tuning cache: " << pcf_last_error(run.ctx) << endl;
+    }
+  }
   pcf_catalog cd = -1;
   std::vector<pcf_catalog> cr(R.size(), -1);
   if ((rc = pcf_catalog_upload(run.ctx, D.data(), o.np, &cd)) != PCF_OK)
@@ -384,6 +411,8 @@ int main(int argc, char** argv) {
   auto t1 = std::chrono::steady_clock::now();
   cout << "Spent " << std::chrono::duration<double, std::milli>(t1 - t0).count()
        << " miliseconds to compute and save all the histograms." << endl;
+  if (!tune_cache.empty() && pcf_tune_save(run.ctx, tune_cache.c_str()) != PCF_OK)
+    std::cerr << "PCF: " << pcf_last_error(run.ctx) << endl;
   pcf_destroy(run.ctx);
   cout << "Program terminated..." << endl;
   return 0;

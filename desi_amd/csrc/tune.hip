@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -224,5 +225,134 @@ extern "C" int pcf_autotune(pcf_ctx* c, const pcf_job* jobs, int32_t njobs, pcf_
     c->store->tuned[key] = res;
   }
   if (out) *out = res;
+  return PCF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Tuning cache file.  Text, one tuned workload per line after a two-line
+// header naming the format and the device it was measured on:
+//   pcf-tuning-cache 1
+//   device <gcnArchName> <compute units>
+//   <workload signature>\t<cells_per_rmax> <lds_copies_log2> <blocks_per_cu>
+//       <pair_kernel> <trials> <best_ms> <default_ms>
+// The signature is tune_key's (type, periodicity, bins, dmax, box, precision
+// and catalog sizes); it holds no tab or newline.  A file from another device
+// model is refused: its timings do not carry over.
+// ---------------------------------------------------------------------------
+namespace pcf {
+namespace {
+
+int device_tag(pcf_ctx* c, std::string* tag) {
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, c->device) != hipSuccess) return -1;
+  char buf[320];
+  std::snprintf(buf, sizeof(buf), "%s %d", p.gcnArchName, p.multiProcessorCount);
+  for (char* q = buf; *q; ++q)
+    if (*q == '\t' || *q == '\n') *q = ' ';
+  *tag = buf;
+  return 0;
+}
+
+}  // namespace
+}  // namespace pcf
+
+extern "C" int pcf_tune_lookup(pcf_ctx* c, const pcf_job* jobs, int32_t njobs, pcf_tuning* out) {
+  using namespace pcf;
+  if (!c || !jobs || njobs <= 0) return fail(c, PCF_ERR_ARG, "bad tuning lookup arguments");
+  c->err.clear();
+  std::vector<pcf_job> base(jobs, jobs + njobs);
+  for (auto& j : base) {
+    if (!get_cat(c, j.cat_x)) return fail(c, PCF_ERR_ARG, "unknown catalog id");
+    j.cells_per_rmax = j.lds_copies_log2 = j.blocks_per_cu = j.pair_kernel = 0;
+  }
+  const std::string key = tune_key(c, base.data(), njobs);
+  std::lock_guard<std::mutex> lk(c->store->mu);
+  auto it = c->store->tuned.find(key);
+  if (it == c->store->tuned.end()) return 0;
+  if (out) *out = it->second;
+  return 1;
+}
+
+extern "C" int pcf_tune_save(pcf_ctx* c, const char* path) {
+  using namespace pcf;
+  if (!c || !path || !*path) return fail(c, PCF_ERR_ARG, "bad tuning cache path");
+  c->err.clear();
+  std::string tag;
+  if (device_tag(c, &tag) != 0) return fail(c, PCF_ERR_HIP, "hipGetDeviceProperties failed");
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE* f = std::fopen(tmp.c_str(), "w");
+  if (!f) return fail(c, PCF_ERR_ARG, std::string("cannot write ") + tmp);
+  bool ok = std::fprintf(f, "pcf-tuning-cache 1\ndevice %s\n", tag.c_str()) > 0;
+  {
+    std::lock_guard<std::mutex> lk(c->store->mu);
+    for (const auto& kv : c->store->tuned) {
+      const pcf_tuning& t = kv.second;
+      ok = ok && std::fprintf(f, "%s\t%d %d %d %d %d %.17g %.17g\n", kv.first.c_str(),
+                              t.cells_per_rmax, t.lds_copies_log2, t.blocks_per_cu,
+                              t.pair_kernel, t.trials, t.best_ms, t.default_ms) > 0;
+    }
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok || std::rename(tmp.c_str(), path) != 0) {
+    std::remove(tmp.c_str());
+    return fail(c, PCF_ERR_ARG, std::string("cannot write ") + path);
+  }
+  return PCF_OK;
+}
+
+extern "C" int pcf_tune_load(pcf_ctx* c, const char* path) {
+  using namespace pcf;
+  if (!c || !path || !*path) return fail(c, PCF_ERR_ARG, "bad tuning cache path");
+  c->err.clear();
+  FILE* f = std::fopen(path, "r");
+  if (!f) return fail(c, PCF_ERR_ARG, std::string("cannot read ") + path);
+  std::string tag;
+  if (device_tag(c, &tag) != 0) {
+    std::fclose(f);
+    return fail(c, PCF_ERR_HIP, "hipGetDeviceProperties failed");
+  }
+  std::vector<std::pair<std::string, pcf_tuning>> got;
+  std::string err;
+  char line[2048];
+  int ln = 0;
+  while (err.empty() && std::fgets(line, sizeof(line), f)) {
+    ++ln;
+    std::string s(line);
+    if (s.empty() || s.back() != '\n') {
+      err = "line " + std::to_string(ln) + " is too long or unterminated";
+      break;
+    }
+    s.pop_back();
+    if (ln == 1) {
+      if (s != "pcf-tuning-cache 1") err = "not a tuning cache file (format line)";
+      continue;
+    }
+    if (ln == 2) {
+      if (s.rfind("device ", 0) != 0) err = "missing device line";
+      else if (s.substr(7) != tag)
+        err = "tuning cache was measured on device '" + s.substr(7) + "', this one is '" + tag +
+              "'";
+      continue;
+    }
+    const size_t tab = s.find('\t');
+    pcf_tuning t{};
+    int used = 0;
+    if (tab == std::string::npos || tab == 0 ||
+        std::sscanf(s.c_str() + tab + 1, "%d %d %d %d %d %lf %lf%n", &t.cells_per_rmax,
+                    &t.lds_copies_log2, &t.blocks_per_cu, &t.pair_kernel, &t.trials, &t.best_ms,
+                    &t.default_ms, &used) != 7 ||
+        s[tab + 1 + used] != '\0' || t.cells_per_rmax < 0 || t.cells_per_rmax > 64 ||
+        t.lds_copies_log2 < 0 || t.lds_copies_log2 > 7 || t.blocks_per_cu < 0 ||
+        t.blocks_per_cu > 64 || t.pair_kernel < PCF_PAIR_AUTO || t.pair_kernel > PCF_PAIR_GRAM) {
+      err = "malformed entry on line " + std::to_string(ln);
+      break;
+    }
+    got.emplace_back(s.substr(0, tab), t);
+  }
+  std::fclose(f);
+  if (err.empty() && ln < 2) err = "truncated tuning cache (no device line)";
+  if (!err.empty()) return fail(c, PCF_ERR_ARG, std::string(path) + ": " + err);
+  std::lock_guard<std::mutex> lk(c->store->mu);
+  for (auto& kv : got) c->store->tuned[kv.first] = kv.second;
   return PCF_OK;
 }

@@ -213,6 +213,17 @@ typedef struct pcf_tuning {
   int32_t reserved;
 } pcf_tuning;
 int pcf_autotune(pcf_ctx* ctx, const pcf_job* jobs, int32_t njobs, pcf_tuning* out);
+/* The autotuner's cache as a file (no counterpart in the reference), so a
+ * tuning measured once is reused by later processes.  pcf_tune_save writes
+ * every tuned workload of the context (text: a format line, the device model
+ * and compute-unit count, then one line per workload signature with its
+ * knobs and timings); pcf_tune_load merges a file's entries into the context
+ * and refuses (PCF_ERR_ARG) a malformed file or one measured on another device
+ * model.  pcf_tune_lookup returns 1 and the tuning (out may be NULL) when the
+ * workload of jobs is cached, 0 when it is not, or a negative status. */
+int pcf_tune_save(pcf_ctx* ctx, const char* path);
+int pcf_tune_load(pcf_ctx* ctx, const char* path);
+int pcf_tune_lookup(pcf_ctx* ctx, const pcf_job* jobs, int32_t njobs, pcf_tuning* out);
 
 /* Statistics of the last pcf_histogram / pcf_wait on this context. */
 int pcf_last_stats(const pcf_ctx* ctx, pcf_stats* out);

@@ -1,0 +1,194 @@
+// Host-code sanitizer driver.  Built by test_sanitize_cpu.py (and by
+// `make -C oracle sanitize`) with AddressSanitizer + UndefinedBehaviorSanitizer
+// (-fno-sanitize-recover: the first report aborts), it runs every entry of the
+// CPU oracle (oracle/pcf_oracle.cpp, compiled into this program) over small
+// seeded catalogs in all modes -- auto and cross, periodic and open, 2*dmax
+// above the box where the entry accepts it, empty and one-point catalogs,
+// duplicate points -- and the command line's catalog reader and histogram
+// writers (desi_amd/cli/pcf_io.h) on files it writes itself, including
+// malformed and short catalogs.  The results are not checked here (the
+// oracle's own tests do that); the program exists so that the sanitizers see
+// every code path run.
+//   sanitize_check <scratch dir>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "../desi_amd/cli/pcf_io.h"
+
+extern "C" {
+int pcf_oracle_pairs(const float*, int64_t, const float*, int64_t, int, int, int, float, int,
+                     float, int, uint64_t*, double*);
+int pcf_oracle_pairs_brute(const float*, int64_t, const float*, int64_t, int, int, int, float,
+                           int, float, uint64_t*, double*);
+int pcf_oracle_cpp32_2iso(const float*, int64_t, const float*, int64_t, int, int, float, int,
+                          double*);
+void pcf_oracle_analytic_rr(int, int64_t, float, float, double*);
+int pcf_oracle_triplets(const float*, int64_t, const float*, int64_t, int, int, int, float, int,
+                        float, int, uint64_t*, double*);
+int pcf_oracle_triplets_brute(const float*, int64_t, const float*, int64_t, int, int, int, float,
+                              int, float, int, uint64_t*, double*);
+int pcf_oracle_cpp32_3iso(const float*, int64_t, const float*, int64_t, int, int, float, int,
+                          double*);
+int pcf_oracle_analytic_3iso(const float*, int64_t, int, float, float, int64_t, int, int, int,
+                             double*, double*, double*, double*);
+}
+
+namespace {
+
+uint64_t g_state = 0x9e3779b97f4a7c15ull;
+uint64_t next_u64() {  // splitmix64
+  uint64_t z = (g_state += 0x9e3779b97f4a7c15ull);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+float uniform(float L) { return (float)((next_u64() >> 40) * (1.0 / 16777216.0)) * L; }
+
+std::vector<float> catalog(int64_t n, float L, bool weighted) {
+  std::vector<float> v((size_t)n * 4);
+  for (int64_t i = 0; i < n; ++i) {
+    for (int a = 0; a < 3; ++a) v[i * 4 + a] = std::fmin(uniform(L), std::nextafter(L, 0.f));
+    v[i * 4 + 3] = weighted ? 0.5f + uniform(1.f) : 1.f;
+  }
+  // A duplicate point (d = 0 pairs) when there is room for one.
+  if (n >= 2)
+    for (int a = 0; a < 4; ++a) v[4 + a] = v[a];
+  return v;
+}
+
+int64_t hsize(int order, int ani, int bins) {
+  const int64_t b = bins;
+  if (order == 2) return ani ? b * b : b;
+  return ani ? b * b * b * b * b : b * b * b;
+}
+
+int fails = 0;
+void expect(bool ok, const char* what) {
+  if (!ok) {
+    std::fprintf(stderr, "unexpected result: %s\n", what);
+    ++fails;
+  }
+}
+
+void pairs_cases() {
+  const float L = 100.f;
+  for (int64_t n : {0, 1, 2, 97, 400}) {
+    std::vector<float> a = catalog(n, L, true), b = catalog(n + 3, L, false);
+    for (int ani = 0; ani < 2; ++ani)
+      for (int bpc = 0; bpc < 2; ++bpc)
+        for (int auto_ = 0; auto_ < 2; ++auto_)
+          for (float dmax : {17.f, 60.f}) {  // 60: 2*dmax above the box
+            const int bins = ani ? 9 : 13;
+            std::vector<uint64_t> c((size_t)hsize(2, ani, bins));
+            std::vector<double> w(c.size());
+            const int rc = pcf_oracle_pairs(a.data(), n, b.data(), n + 3, auto_, ani, bpc, L,
+                                            bins, dmax, 2, c.data(), w.data());
+            expect(rc == 0, "pcf_oracle_pairs");
+            if (n <= 97) {
+              const int rb = pcf_oracle_pairs_brute(a.data(), n, b.data(), n + 3, auto_, ani,
+                                                    bpc, L, bins, dmax, c.data(), w.data());
+              expect(rb == 0, "pcf_oracle_pairs_brute");
+            }
+          }
+    for (int auto_ = 0; auto_ < 2; ++auto_) {
+      std::vector<double> h(20);
+      expect(pcf_oracle_cpp32_2iso(a.data(), n, b.data(), n + 3, auto_, 20, 30.f, 2, h.data()) == 0,
+             "pcf_oracle_cpp32_2iso");
+    }
+  }
+  // Refused arguments return an error code, not a crash.
+  std::vector<uint64_t> c(8);
+  std::vector<double> w(8);
+  expect(pcf_oracle_pairs(nullptr, 0, nullptr, 0, 1, 0, 0, 0.f, 0, 10.f, 1, c.data(), w.data()) != 0,
+         "bins = 0 refused");
+  expect(pcf_oracle_pairs(nullptr, 0, nullptr, 0, 1, 0, 1, 10.f, 8, 10.f, 1, c.data(), w.data()) != 0,
+         "periodic dmax >= box refused");
+  std::vector<double> rr(30);
+  pcf_oracle_analytic_rr(30, 100000, 80.f, 250.f, rr.data());
+  pcf_oracle_analytic_rr(30, 3000000, 80.f, 1000.f, rr.data());
+}
+
+void triplet_cases() {
+  const float L = 60.f;
+  for (int64_t n : {0, 1, 3, 40, 120}) {
+    std::vector<float> a = catalog(n, L, true), b = catalog(n + 5, L, true);
+    for (int ani = 0; ani < 2; ++ani)
+      for (int bpc = 0; bpc < 2; ++bpc)
+        for (int auto_ = 0; auto_ < 2; ++auto_) {
+          const int bins = ani ? 4 : 7;
+          const float dmax = 14.f;
+          std::vector<uint64_t> c((size_t)hsize(3, ani, bins));
+          std::vector<double> w(c.size());
+          expect(pcf_oracle_triplets(a.data(), n, b.data(), n + 5, auto_, ani, bpc, L, bins, dmax,
+                                     2, c.data(), w.data()) == 0,
+                 "pcf_oracle_triplets");
+          if (n <= 40)
+            expect(pcf_oracle_triplets_brute(a.data(), n, b.data(), n + 5, auto_, ani, bpc, L,
+                                             bins, dmax, 1, c.data(), w.data()) == 0,
+                   "pcf_oracle_triplets_brute");
+        }
+    for (int auto_ = 0; auto_ < 2; ++auto_) {
+      std::vector<double> h(6 * 6 * 6);
+      expect(pcf_oracle_cpp32_3iso(a.data(), n, b.data(), n + 5, auto_, 6, 14.f, 2, h.data()) == 0,
+             "pcf_oracle_cpp32_3iso");
+    }
+    if (n > 0) {
+      const int bins = 5, ptt = 4, bref = 6;
+      std::vector<double> ddr(bins * bins * bins), rrr(ddr.size()), ff(bins), ffr(bins * bref);
+      expect(pcf_oracle_analytic_3iso(a.data(), n, bins, 14.f, L, 0, ptt, bref, 2, ddr.data(),
+                                      rrr.data(), ff.data(), ffr.data()) == 0,
+             "pcf_oracle_analytic_3iso");
+    }
+  }
+}
+
+void io_cases(const std::string& dir) {
+  const std::string good = dir + "/cat.dat", bad = dir + "/bad.dat", shortf = dir + "/short.dat";
+  {
+    std::ofstream f(good);
+    for (int i = 0; i < 257; ++i)
+      f << uniform(250.f) << (i % 3 ? " " : "\t") << uniform(250.f) << "  " << uniform(250.f)
+        << " " << 1.0 << (i % 5 ? "\n" : "\r\n");
+    std::ofstream g(bad);
+    g << "1 2 3 1\n4 5 x 1\n";
+    std::ofstream s(shortf);
+    s << "1 2 3 1\n4 5 6";
+  }
+  for (int threads : {1, 3, 8}) {
+    std::vector<float> v;
+    double mx = 0;
+    std::string err;
+    expect(pcfcli::read_catalog(good, 257, &v, &mx, &err, threads), "read_catalog good");
+    // Malformed input: only that the reader returns (its messages are tested
+    // elsewhere).
+    (void)pcfcli::read_catalog(good, 300, &v, &mx, &err, threads);
+    (void)pcfcli::read_catalog(bad, 2, &v, &mx, &err, threads);
+    (void)pcfcli::read_catalog(shortf, 2, &v, &mx, &err, threads);
+    expect(!pcfcli::read_catalog(dir + "/missing.dat", 2, &v, &mx, &err, threads),
+           "read_catalog missing file");
+  }
+  const int bins = 4;
+  std::vector<double> h(bins * bins * bins * bins * bins);
+  for (size_t i = 0; i < h.size(); ++i) h[i] = (double)i * 0.37;
+  expect(pcfcli::save_histogram1D(dir + "/h1.txt", bins, h.data()), "save_histogram1D");
+  expect(pcfcli::save_histogram2D(dir + "/h2.txt", bins, h.data()), "save_histogram2D");
+  expect(pcfcli::save_histogram3D(dir + "/h3.txt", bins, h.data()), "save_histogram3D");
+  expect(pcfcli::save_histogram5D(dir + "/h5.txt", bins, h.data()), "save_histogram5D");
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc != 2) return 2;
+  pairs_cases();
+  triplet_cases();
+  io_cases(argv[1]);
+  if (fails) return 1;
+  std::printf("sanitize_check ok\n");
+  return 0;
+}

@@ -153,3 +153,35 @@ def test_2ani_bpc_2dmax_ge_box_files(tmp_path):
                       ("DRani_BPC_rand0.dat", "DR")):
         want = np.loadtxt(os.path.join(fix, f"2ani_bpc_d150_s250_b50_{ref}.txt"))
         assert np.array_equal(_load(res / name), want), name
+
+
+def test_tuning_cache_env(tmp_path):
+    """PCF_TUNE_CACHE + PCF_AUTOTUNE=1: the first run tunes its pair
+    histograms and writes the cache, the second reuses it; both runs write the
+    same histograms as a run without tuning."""
+    cache = tmp_path / "tune.cache"
+    args = ("2iso", "-n", "5000", "-f", "data_5K.dat", "-r", "rand0_5K.dat", "-b", "20", "-d",
+            "60")
+    (tmp_path / "plain").mkdir()
+    base = _run(tmp_path / "plain", *args)
+    outs = []
+    for k in range(2):
+        res = tmp_path / f"t{k}" / "results"
+        res.mkdir(parents=True)
+        env = dict(os.environ, PCF_DATA_DIR=CATS, PCF_RESULTS_DIR=str(res),
+                   PCF_TUNE_CACHE=str(cache), PCF_AUTOTUNE="1")
+        print("cli tuned run", k, flush=True)
+        p = subprocess.run([da.CLI_PATH, *args], capture_output=True, text=True, env=env,
+                           timeout=300)
+        assert p.returncode == 0, p.stdout + p.stderr
+        assert "ignoring" not in p.stderr, p.stderr
+        outs.append(res)
+        lines = cache.read_text().splitlines()
+        assert lines[0] == "pcf-tuning-cache 1"
+        # DD, RR and DR: three workloads (RR and DD share sizes and differ in
+        # nothing the signature holds, so they may share one entry).
+        assert 2 <= len(lines) - 2 <= 3, lines
+    for name in os.listdir(base):
+        ref = _load(base / name)
+        for res in outs:
+            assert np.array_equal(_load(res / name), ref), name
