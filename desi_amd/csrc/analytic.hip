@@ -257,8 +257,8 @@ int run_analytic_3iso(pcf_ctx* c, pcf_catalog data, int bins, float dmax, float 
   if (bins <= 0 || bins > 1024) return fail(c, PCF_ERR_ARG, "bins out of range");
   if (!(dmax > 0) || !std::isfinite(dmax)) return fail(c, PCF_ERR_ARG, "dmax must be > 0");
   if (!(box > 0)) return fail(c, PCF_ERR_ARG, "box side required for periodic boundaries");
-  if (!(2.0 * dmax < box))
-    return fail(c, PCF_ERR_UNSUPPORTED, "periodic boundaries need 2*dmax < box side");
+  if (!(dmax < box))
+    return fail(c, PCF_ERR_UNSUPPORTED, "periodic boundaries need dmax < box side");
   if (np <= 0) np = X->n;
   const int64_t nb1 = (int64_t)ptt * bins, nb2 = (int64_t)ptt * bn_ref * bins;
   if (nb2 > (1ll << 27) || (int64_t)bins * bn_ref > (1 << 24))

@@ -31,7 +31,11 @@
 //      both orders and every third point is taken.
 //
 // 3iso accumulates in LDS (u32 counts, FP64 weights) over slices of b12 so the
-// slice fits; 3ani's 5-D histogram (bins^5) goes to global atomics.
+// slice fits (or, when one b12 row does not, in global memory); 3ani's 5-D
+// histogram (bins^5) is privatised in LDS per (b12, t) slice where one t value
+// of a b12 row fits, and goes to global atomics otherwise.
+#include <cstring>
+
 #include "exact.h"
 #include "pcf_ctx.h"
 #include "worklist.h"
@@ -295,6 +299,7 @@ struct TripArgs {
   int bpc;
   double R, L, ds_th;
   int lo, hi;     // b12 slice
+  int glob;       // 3iso: V rows in global memory (one b12 row exceeds LDS)
   float reach;    // rmax plus slack (unscaled)
   int K[3];       // neighbour reach in cells
   GridGeom g;
@@ -425,7 +430,7 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
   __shared__ double sT[kMaxTripBins + 2];
   const int bins = a.bins;
   const int bins2 = bins * bins;
-  const int nrows = ANI ? 0 : (a.hi - a.lo) * bins2;
+  const int nrows = (ANI || a.glob) ? 0 : (a.hi - a.lo) * bins2;
   uint32_t* hc = reinterpret_cast<uint32_t*>(smem);
   const size_t hc_bytes = ((size_t)nrows * 4 + 15) & ~(size_t)15;
   double* hw = reinterpret_cast<double*>(smem + hc_bytes);
@@ -454,6 +459,18 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
   const unsigned ntm1 = (unsigned)(a.nt - 1);
   const unsigned ubins = (unsigned)bins;
   unsigned long long evals = 0, nre = 0;
+  // 3iso deposit of V row vrow = (b12 * bins + b23) * bins + b31: into the
+  // LDS slice [lo, hi) of b12 rows, or (glob) straight into global memory.
+  const unsigned vrow0 = a.glob ? 0u : (unsigned)a.lo * (unsigned)bins2;
+  auto deposit_v = [&](unsigned vrow, double w) {
+    if (a.glob) {
+      atomicAdd(&hist[vrow], 1ull);
+      if (WEIGHTED) atomicAdd(&whist[vrow], w);
+    } else {
+      atomicAdd(&hc[vrow - vrow0], 1u);
+      if (WEIGHTED) atomicAdd(&hw[vrow - vrow0], w);
+    }
+  };
 
   for (;;) {
     int task = 0;
@@ -494,7 +511,7 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
         const double w12 = wi * (double)r.wj;
         const unsigned nact = (unsigned)__popcll(__ballot(active));
         evals += (unsigned long long)nact * (unsigned long long)n;
-        const unsigned rbase = (unsigned)(b12 - a.lo) * (unsigned)bins2;
+        const unsigned rbase = (unsigned)b12 * (unsigned)bins2;
         const int64_t abase = ((int64_t)b12 * bins * bins * bins) * bins + (int64_t)t12 * bins;
         float4 jo = make_float4(0.f, 0.f, 0.f, 0.f);
         bool jo_loaded = false;
@@ -540,9 +557,8 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
           }
           if (active && c23 < ubins) {
             if (!ANI) {
-              const unsigned row = rbase + c23 * ubins + (unsigned)b31;
-              atomicAdd(&hc[row], 1u);
-              if (WEIGHTED) atomicAdd(&hw[row], __dmul_rn(w12, (double)kq.w));
+              deposit_v(rbase + c23 * ubins + (unsigned)b31,
+                        WEIGHTED ? __dmul_rn(w12, (double)kq.w) : 0.0);
             } else {
               const int64_t idx = abase + ((int64_t)b31 * bins * bins * bins +
                                            (int64_t)c23 * bins * bins) + pb;
@@ -587,7 +603,7 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
           bop[b] = __builtin_fmaf(sel_x, -2.0f * x, __builtin_fmaf(sel_y, -2.0f * y,
                                                                    __builtin_fmaf(sel_z, -2.0f * z, sel_w)));
           cin[b] = act ? __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x)) : 1e30f;
-          rb[b] = (unsigned)(((r.meta >> 8) & 0xfff) - a.lo) * (unsigned)bins2;
+          rb[b] = (unsigned)((r.meta >> 8) & 0xfff) * (unsigned)bins2;
           wj[b] = r.wj;
           npact += (unsigned)__popcll(__ballot(act && grp == 0));
         }
@@ -635,9 +651,8 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
               for (int v = 0; v < 4; ++v) {
                 const unsigned cc = c23[4 * u + v];
                 if (cc < ubins) {
-                  const unsigned row = rb[h + u] + cc * ubins + (unsigned)(mr[v] & 0xff);
-                  atomicAdd(&hc[row], 1u);
-                  if (WEIGHTED) atomicAdd(&hw[row], __dmul_rn(w12, (double)wr[v]));
+                  deposit_v(rb[h + u] + cc * ubins + (unsigned)(mr[v] & 0xff),
+                            WEIGHTED ? __dmul_rn(w12, (double)wr[v]) : 0.0);
                 }
               }
             }
@@ -674,12 +689,9 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
                   if (!(a.auto_ && !(s23 > 0.0))) {
                     const int e23 = class_lds(s23, sT, bins, a.R, 0);
                     if (e23 >= 0) {
-                      const unsigned row =
-                          (unsigned)(((r.meta >> 8) & 0xfff) - a.lo) * (unsigned)bins2 +
-                          (unsigned)e23 * ubins + (unsigned)(meta & 0xff);
-                      atomicAdd(&hc[row], 1u);
-                      if (WEIGHTED)
-                        atomicAdd(&hw[row], __dmul_rn(wi * (double)r.wj, (double)Lw[ent]));
+                      deposit_v((unsigned)((r.meta >> 8) & 0xfff) * (unsigned)bins2 +
+                                    (unsigned)e23 * ubins + (unsigned)(meta & 0xff),
+                                WEIGHTED ? __dmul_rn(wi * (double)r.wj, (double)Lw[ent]) : 0.0);
                     }
                   }
                 }
@@ -1132,15 +1144,18 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   Catalog* K = auto_ ? X : get_cat(c, job->cat_y);
   if (!X || !K) return fail(c, PCF_ERR_ARG, "unknown catalog id");
   const int bins = job->bins;
-  if (bins <= 0 || bins > kMaxTripBins || (ani && bins > 40))
+  if (bins <= 0 || bins > kMaxTripBins)
     return fail(c, PCF_ERR_ARG, "bins out of range");
   if (!(job->dmax > 0) || !std::isfinite(job->dmax)) return fail(c, PCF_ERR_ARG, "dmax must be > 0");
   const int bpc = job->bpc ? 1 : 0;
   const float L = job->box;
   if (bpc) {
     if (!(L > 0)) return fail(c, PCF_ERR_ARG, "box side required for periodic boundaries");
-    if (!(2.0 * job->dmax < L))
-      return fail(c, PCF_ERR_UNSUPPORTED, "periodic boundaries need 2*dmax < box side");
+    // 2*dmax >= L is handled (a point may count under its regular and its
+    // projected image on an axis); dmax >= L would reach far images, which no
+    // reference term is.
+    if (!(job->dmax < L))
+      return fail(c, PCF_ERR_UNSUPPORTED, "periodic boundaries need dmax < box side");
     for (Catalog* cc : {X, K})
       for (int a = 0; a < 3; ++a)
         if (cc->n && (cc->lo[a] < 0.f || !(cc->hi[a] < L)))
@@ -1149,7 +1164,11 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   const int64_t b64 = bins;
   const int64_t nflat = ani ? b64 * b64 * b64 * b64 * b64 : b64 * b64 * b64;
   const bool weighted = !(X->all_w1 && K->all_w1);
-  const bool plain = job->precision == PCF_PREC_FP64;
+  // With 2*dmax >= L the reference's 2-3 side, |x3 - x2| - L*|f2 - f3| per
+  // axis, is no longer the separation of the two images in point 1's frame
+  // (both points projected toward opposite walls, say), which the FP32 bulk
+  // forms; such jobs take the plain FP64 kernels, which evaluate that rule.
+  const bool plain = job->precision == PCF_PREC_FP64 || (bpc && !(2.0 * job->dmax < L));
 
   GridGeom g;
   int Kc[3];
@@ -1302,6 +1321,7 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   t.L = (double)L;
   t.ds_th = (double)ds_th_f;
   t.reach = reach;
+  t.glob = 0;
   for (int q = 0; q < 3; ++q) t.K[q] = Kc[q];
   t.g = g;
   HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
@@ -1312,18 +1332,20 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
       const size_t lists = (size_t)(kTripBlock / kWave) * ListCap<0>::value * (gram ? 28 : 24);
       const size_t budget = 160 * 1024 - kMixedTableMax - 8 * (kMaxTripBins + 2) - 1024 - lists;
       const size_t per_b12 = (size_t)bins * bins * (4 + (weighted ? 8 : 0));
-      if (per_b12 + 16 > budget)
-        return fail(c, PCF_ERR_UNSUPPORTED,
-                    "3iso: one b12 row of the histogram does not fit the LDS budget "
-                    "(about 90 bins weighted, 160 unweighted)");
+      // A b12 row larger than the LDS budget (about 90 bins weighted, 160
+      // unweighted) deposits straight into the global V histogram instead.
+      const bool glob = per_b12 + 16 > budget;
+      t.glob = glob ? 1 : 0;
       // XXX takes one b12 row per launch: its third points are then cut to
       // b31 >= b12 (each unordered triplet deposited once, k_sym_cyc).
-      const int S = auto_ ? 1
-                          : (int)std::max<size_t>(1, std::min<size_t>(bins, (budget - 16) / per_b12));
+      const int S =
+          auto_ ? 1
+                : (glob ? bins
+                        : (int)std::max<size_t>(1, std::min<size_t>(bins, (budget - 16) / per_b12)));
       for (int lo = 0; lo < bins; lo += S) {
         t.lo = lo;
         t.hi = std::min(bins, lo + S);
-        const int nrows = (t.hi - t.lo) * bins * bins;
+        const int nrows = glob ? 0 : (t.hi - t.lo) * bins * bins;
         const size_t lds =
             ((((size_t)nrows * 4 + 15) / 16 * 16 + (weighted ? (size_t)nrows * 8 : 0) + 15) &
              ~(size_t)15) + lists;
@@ -1485,21 +1507,26 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
     outH = d_H;
     outW = d_HW;
   }
-  std::vector<unsigned long long> hc((size_t)nflat + 4);
-  std::vector<double> hw((size_t)nflat);
-  HIPCHK(c, hipMemcpyAsync(hc.data(), outH, hbytes, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(hw.data(), outW, wbytes, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(hc.data() + nflat, d_stats, 2 * sizeof(unsigned long long),
-                           hipMemcpyDeviceToHost, c->stream));
+  // Straight into the caller's arrays (3ani histograms reach gigabytes): an
+  // unweighted job's weights are its counts, copied as integers and converted
+  // in place.
+  unsigned long long st2[2] = {0ull, 0ull};
+  if (counts) HIPCHK(c, hipMemcpyAsync(counts, outH, hbytes, hipMemcpyDeviceToHost, c->stream));
+  if (weights)
+    HIPCHK(c, hipMemcpyAsync(weights, weighted ? (const void*)outW : (const void*)outH,
+                             weighted ? wbytes : hbytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(st2, d_stats, sizeof(st2), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  for (int64_t b = 0; b < nflat; ++b) {
-    if (counts) counts[b] = (uint64_t)hc[b];
-    if (weights) weights[b] = weighted ? hw[b] : (double)hc[b];
-  }
+  if (weights && !weighted)
+    for (int64_t b = 0; b < nflat; ++b) {
+      uint64_t v;
+      std::memcpy(&v, weights + b, sizeof(v));
+      weights[b] = (double)v;
+    }
   pcf_stats st{};
-  st.evaluations = hc[nflat];
-  st.rechecks = hc[nflat + 1];
+  st.evaluations = st2[0];
+  st.rechecks = st2[1];
   st.work_items = (uint64_t)npairs;
   float ms = 0.f;
   HIPCHK(c, hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));

@@ -567,7 +567,11 @@ int pcf_oracle_triplets(const float* a_, int64_t na, const float* b_, int64_t nb
                         int ani, int bpc, float box, int bins, float dmax, int nthreads,
                         uint64_t* counts, double* wsum) {
   if (bins <= 0 || !(dmax > 0)) return -1;
-  if (bpc && !(2.0 * dmax < box)) return -2;
+  // Periodic: dmax < box (2*dmax >= box included: a point may then count
+  // under its regular and its projected image on an axis, as the reference's
+  // 64 (f2, f3) combinations allow; the cells are then the whole box and the
+  // neighbour images enumerate both).
+  if (bpc && !(dmax < box)) return -2;
   const Pt* a = reinterpret_cast<const Pt*>(a_);
   const Pt* b = auto_ ? a : reinterpret_cast<const Pt*>(b_);
   if (auto_) nb = na;
@@ -774,7 +778,7 @@ extern "C" {
 int pcf_oracle_analytic_3iso(const float* a_, int64_t na, int bins, float dmax, float box,
                              int64_t np, int ptt, int bn_ref, int nthreads, double* ddr,
                              double* rrr, double* ff_av_out, double* ff_av_ref_out) {
-  if (bins <= 0 || ptt <= 0 || bn_ref <= 0 || !(2.0 * dmax < box)) return -1;
+  if (bins <= 0 || ptt <= 0 || bn_ref <= 0 || !(dmax < box)) return -1;
   const Pt* a = reinterpret_cast<const Pt*>(a_);
   const int64_t nb1 = (int64_t)ptt * bins, nb2 = (int64_t)ptt * bn_ref * bins;
   const float dsf1 = (float)nb1 / dmax, dsf2 = (float)nb2 / dmax;

@@ -101,8 +101,21 @@ def test_argument_errors(engine):
     c = engine.upload(d)
     try:
         with pytest.raises(da.PcfError):
-            engine.analytic_3iso(c, bins=10, dmax=130.0, box=L)  # 2*dmax >= box
+            engine.analytic_3iso(c, bins=10, dmax=260.0, box=L)  # dmax >= box
         with pytest.raises(da.PcfError):
             engine.analytic_3iso(c, bins=10, dmax=60.0, box=100.0)  # points outside box
     finally:
         engine.release(c)
+
+
+def test_parity_wide_periodic(engine):
+    """2*dmax >= box: the fine pair histograms count a pair under both of its
+    images where both are in range, as the pair kernels do."""
+    d = synth.uniform(900, 100.0, seed=9)
+    print("case uniform 900 box 100 bins 6 dmax 60 ptt 10 bn_ref 4", flush=True)
+    g = _gpu(engine, d, bins=6, dmax=60.0, box=100.0, ptt=10, bn_ref=4)
+    o = O.analytic_3iso(d, bins=6, dmax=60.0, box=100.0, ptt=10, bn_ref=4)
+    assert np.array_equal(g[2], o[2]), "ff_av"
+    assert np.array_equal(g[3], o[3]), "ff_av_ref"
+    _close(g[1], o[1], 1e-12)
+    _close(g[0], o[0], 1e-12)

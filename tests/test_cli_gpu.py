@@ -185,3 +185,24 @@ def test_tuning_cache_env(tmp_path):
         ref = _load(base / name)
         for res in outs:
             assert np.array_equal(_load(res / name), ref), name
+
+
+@pytest.mark.parametrize("kind", ["3iso", "3ani"])
+def test_triplets_bpc_2dmax_ge_box_files(tmp_path, kind):
+    """`PCF 3iso|3ani -bpc -d 150 -s 250` (2*dmax > box): the written
+    histograms equal the oracle's (a triplet counts under every combination of
+    regular and projected images in range)."""
+    bins = 6 if kind == "3iso" else 4
+    res = _run(tmp_path, kind, "-n", "100", "-f", "data_500.dat", "-r", "rand0_500.dat",
+               "-b", str(bins), "-d", "150", "-bpc", "-s", "250")
+    d, r = catalog("data_500.dat")[:100], catalog("rand0_500.dat")[:100]
+    tag = "iso" if kind == "3iso" else "ani"
+    for name, (a, b) in ((f"DDD{tag}_BPC_data_500.dat", (d, None)),
+                         (f"DDR{tag}_BPC_rand0_500.dat", (d, r)),
+                         (f"DRR{tag}_BPC_rand0_500.dat", (r, d)),
+                         (f"RRR{tag}_BPC_rand0_500.dat", (r, None))):
+        c, _ = O.triplets(a, b, ani=kind == "3ani", bpc=True, box=250.0, bins=bins,
+                          dmax=150.0)
+        got = _load(res / name)
+        # 3ani files hold the first bins^3 values of the bins^5 histogram.
+        assert np.array_equal(got, c[:got.size].astype(np.float64)), name

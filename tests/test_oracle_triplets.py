@@ -110,3 +110,51 @@ def test_cpu_layout_relations():
     third = float(np.float32(1.0) / np.float32(3.0))
     expect = (np.round(sym * 3) * third).astype(np.float32).astype(np.float64)
     assert np.array_equal(cpu_xxy, expect)
+
+
+@pytest.mark.parametrize("ani", [False, True])
+@pytest.mark.parametrize("auto", [True, False])
+def test_triplet_grid_equals_brute_wide_periodic(ani, auto):
+    """Periodic with 2*dmax >= box (dmax < box): a point can count under its
+    regular and its projected image on one axis, and the grid enumeration
+    (one cell spanning the box, neighbour images -1, 0, +1) must reproduce
+    the literal 64-combination formula."""
+    a = synth.uniform(26, 20.0, seed=43, weights="uniform")
+    b = synth.uniform(21, 20.0, seed=44, weights="uniform")
+    bins = 4 if ani else 6
+    for dmax in (10.5, 13.0, 19.0):
+        c1, w1 = O.triplets(a, None if auto else b, ani=ani, bpc=True, box=20.0, bins=bins,
+                            dmax=dmax)
+        c2, w2 = O.triplets(a, None if auto else b, ani=ani, bpc=True, box=20.0, bins=bins,
+                            dmax=dmax, brute=True)
+        assert c1.sum() > 0
+        assert np.array_equal(c1, c2)
+        assert np.allclose(w1, w2, rtol=1e-13, atol=0)
+
+
+def test_wide_periodic_triplet_by_hand():
+    """Three points on a line in a box of side 10 with dmax 8 (> box / 2):
+    A = 5, B = 2, C = 8.  Per axis the reference takes |dx| - L*f for the 1-2
+    and 1-3 sides (f = 1: the projected image) and |x3 - x2| - L*|f2 - f3| for
+    the 2-3 side (3isoBPC_k.cuh:56-75), so from A the pair (B, C) counts four
+    ways: (f2, f3) = (0,0) sides 3, 6, 3; (1,0) 7, 4, 3; (0,1) 3, 4, 7; and
+    (1,1) 7, 6, 7, where the 2-3 side is |8 - 2| although the two projected
+    images lie 14 apart.  Worked out for all six orders by hand, the 24 ordered
+    (triplet, combination) entries are each permutation of {3, 6, 3}, of
+    {7, 4, 3} and of {7, 6, 7}, twice each (bins = floor(d), ds = 1)."""
+    pts = np.array([[5, 1, 1, 1], [2, 1, 1, 1], [8, 1, 1, 1]], np.float32)
+    c, w = O.triplets(pts, bpc=True, box=10.0, bins=8, dmax=8.0)
+    h = c.reshape(8, 8, 8)
+    expect = np.zeros((8, 8, 8), np.uint64)
+    for tri in ((3, 6, 3), (7, 4, 3), (7, 6, 7)):
+        for p in set(itertools.permutations(tri)):
+            expect[p] += 2
+    assert h.sum() == 24
+    assert np.array_equal(h, expect), np.argwhere(h != expect)
+    assert np.array_equal(w.reshape(8, 8, 8), expect.astype(np.float64))
+    # At dmax 4.5 (< box / 2) the three points form no triplet, although B
+    # and C are 4 apart through the wall: from A both are at their regular
+    # images (f = 0), so the 2-3 side is |8 - 2| = 6; from B (or C) the third
+    # point is projected (f3 = 1) and the 2-3 side becomes |8 - 5| - 10.
+    c2, _ = O.triplets(pts, bpc=True, box=10.0, bins=9, dmax=4.5)
+    assert c2.sum() == 0

@@ -73,19 +73,24 @@ def test_empty_and_tiny(engine):
     assert nonzero == 16
 
 
-def test_3iso_too_many_bins_is_unsupported(engine):
-    """A weighted 3iso histogram whose single b12 row exceeds the LDS budget
-    (120 x 120 bins x 12 bytes) is refused with PCF_ERR_UNSUPPORTED before any
-    launch; the same bins unweighted fit and match the oracle."""
+@pytest.mark.parametrize("cross", [False, True])
+@pytest.mark.parametrize("prec", [da.PCF_PREC_MIXED, da.PCF_PREC_FP64])
+def test_3iso_many_bins_global_histogram(engine, cross, prec):
+    """Weighted 3iso with 120 bins: one b12 row of the histogram (120 x 120
+    bins x 12 bytes) exceeds the LDS budget, so the kernel deposits straight
+    into the global histogram; counts and weights equal the oracle's."""
     a = synth.uniform(400, 50.0, seed=5, weights="uniform")
-    with pytest.raises(da.PcfError) as ei:
-        _run(engine, a, None, ani=False, bpc=False, box=50.0, bins=120, dmax=12.0)
-    assert ei.value.rc == da.PCF_ERR_UNSUPPORTED
-    a1 = a.copy()
-    a1[:, 3] = 1.0
-    c, _, _ = _run(engine, a1, None, ani=False, bpc=False, box=50.0, bins=120, dmax=12.0)
-    co, _ = O.triplets(a1, None, bins=120, dmax=12.0)
-    assert np.array_equal(c, co) and co.sum() > 0
+    b = synth.uniform(300, 50.0, seed=6, weights="uniform")
+    _check(engine, a, b if cross else None, ani=False, bpc=False, box=50.0, bins=120,
+           dmax=12.0, prec=prec)
+
+
+def test_3ani_more_than_40_bins(engine):
+    """3ani with 41 bins per dimension (41^5 = 115,856,201 bins per
+    histogram, about 1.9 GB of counts and weights): the global-atomic kernel,
+    against the oracle."""
+    a = synth.neyman_scott(4, 10, 3.0, 60.0, seed=7, weights="uniform")
+    _check(engine, a, None, ani=True, bpc=True, box=60.0, bins=41, dmax=20.0)
 
 
 @pytest.mark.parametrize("bpc", [False, True])
@@ -208,3 +213,55 @@ def test_3ani_many_bins_global_histogram(engine, cross):
     d = synth.neyman_scott(10, 30, 5.0, 250.0, seed=61, weights="uniform")
     r = synth.uniform(300, 250.0, seed=62, weights="uniform")
     _check(engine, d, r if cross else None, ani=True, bpc=True, box=250.0, bins=24, dmax=60.0)
+
+
+@pytest.mark.parametrize("bins,weighted", [(4, True), (15, True), (21, True), (22, True),
+                                           (15, False), (30, False)])
+@pytest.mark.parametrize("cross", [False, True])
+def test_3ani_kernel_domains(engine, bins, weighted, cross):
+    """3ani mixed precision across the bin counts where the LDS kernel's slice
+    (one t value of a b12 row, bins^3 rows of 4 + 8 bytes) goes from many t
+    per slice down to one (22 bins weighted) and, unweighted, up to 30 bins;
+    against the oracle, periodic, auto and cross."""
+    wk = "uniform" if weighted else None
+    d = synth.neyman_scott(12, 24, 4.0, 120.0, seed=70 + bins, weights=wk)
+    r = synth.uniform(260, 120.0, seed=90 + bins, weights=wk)
+    _check(engine, d, r if cross else None, ani=True, bpc=True, box=120.0, bins=bins,
+           dmax=30.0)
+
+
+@pytest.mark.parametrize("ani", [False, True])
+@pytest.mark.parametrize("cross", [False, True])
+def test_wide_periodic_triplets(engine, ani, cross):
+    """Periodic boxes with 2*dmax >= box (dmax < box): a point can count
+    under its regular and its projected image on an axis, and the reference's
+    2-3 side |x3 - x2| - L*|f2 - f3| is no longer a distance in point 1's
+    frame.  Requested in mixed precision (the library runs such jobs on its
+    plain FP64 kernels) and in FP64, against the oracle."""
+    d = synth.uniform(90, 20.0, seed=101, weights="uniform")
+    r = synth.uniform(70, 20.0, seed=102, weights="uniform")
+    bins = 5 if ani else 8
+    for dmax in (10.5, 17.0):
+        for prec in (da.PCF_PREC_MIXED, da.PCF_PREC_FP64):
+            _check(engine, d, r if cross else None, ani=ani, bpc=True, box=20.0, bins=bins,
+                   dmax=dmax, prec=prec)
+
+
+def test_wide_periodic_triplet_by_hand_gpu(engine):
+    """The three points of the oracle's hand-worked case (A = 5, B = 2,
+    C = 8 on a line, box 10, dmax 8): 24 ordered entries, among them the
+    (7, 6, 7) sides whose 2-3 side is |8 - 2| although both projected images
+    lie 14 apart."""
+    pts = np.array([[5, 1, 1, 1], [2, 1, 1, 1], [8, 1, 1, 1]], np.float32)
+    c, _, _ = _run(engine, pts, None, ani=False, bpc=True, box=10.0, bins=8, dmax=8.0)
+    co, _ = O.triplets(pts, bpc=True, box=10.0, bins=8, dmax=8.0)
+    assert np.array_equal(c, co) and c.sum() == 24
+    assert c.reshape(8, 8, 8)[7, 6, 7] == 2
+
+
+def test_periodic_dmax_at_box_is_unsupported(engine):
+    """dmax >= box would reach far images, which no reference term is."""
+    d = synth.uniform(50, 20.0, seed=103)
+    with pytest.raises(da.PcfError) as ei:
+        _run(engine, d, None, ani=False, bpc=True, box=20.0, bins=5, dmax=20.0)
+    assert ei.value.rc == da.PCF_ERR_UNSUPPORTED
