@@ -37,7 +37,7 @@ EXPORTED_SYMBOLS = (
     "pcf_analytic_3iso", "pcf_histogram_batch", "pcf_autotune", "pcf_set_max_batch",
     "pcf_set_hbm_budget", "pcf_submit", "pcf_poll", "pcf_wait", "pcf_estimator",
     "pcf_histogram_noalloc", "pcf_selftest_scan", "pcf_tune_save", "pcf_tune_load",
-    "pcf_tune_lookup",
+    "pcf_tune_lookup", "pcf_set_ref_bugs",
 )
 # Bytes of the static device array pcf_histogram_noalloc works in (per GPU).
 PCF_NOALLOC_SCRATCH_BYTES = 128 * 1024 * 1024
@@ -115,6 +115,7 @@ def load_library(path=LIB_PATH):
     L.pcf_analytic_rr.argtypes = [vp, c_i32, c_i64, ctypes.c_float, ctypes.c_float, vp]
     L.pcf_histogram_batch.argtypes = [vp, ctypes.POINTER(PcfJob), c_i32, vp, vp]
     L.pcf_autotune.argtypes = [vp, ctypes.POINTER(PcfJob), c_i32, ctypes.POINTER(PcfTuning)]
+    L.pcf_set_ref_bugs.argtypes = [vp, c_i32]
     L.pcf_tune_save.argtypes = [vp, ctypes.c_char_p]
     L.pcf_tune_load.argtypes = [vp, ctypes.c_char_p]
     L.pcf_tune_lookup.argtypes = [vp, ctypes.POINTER(PcfJob), c_i32, ctypes.POINTER(PcfTuning)]
@@ -274,6 +275,10 @@ class Engine:
         t = PcfTuning()
         self._check(self.lib.pcf_autotune(self.ctx, arr, len(jobs), ctypes.byref(t)))
         return t.as_dict()
+
+    def set_ref_bugs(self, on=True):
+        """Reproduce the reference's defects (pcf_set_ref_bugs)."""
+        self._check(self.lib.pcf_set_ref_bugs(self.ctx, 1 if on else 0))
 
     def tune_save(self, path):
         """Write the context's tuning cache to a file (pcf_tune_save)."""

@@ -19,6 +19,14 @@
 //     reference's writer reads a bins^3 stride into a bins^5 array for random
 //     files after the first, cuda/pcf3ani.cuh:123-138); PCF_SAVE_5D=1 writes the
 //     full bins^5 histogram with the reference's save_histogram5D format.
+// --ref-bugs reproduces the reference's defects instead (for comparison with
+// result files it wrote; pcf_set_ref_bugs for the library's part): the
+// cumulative triplet weight and the XY2ani weight on the -p node grid, the
+// 32-bit int overflows of the analytic normalisations, 3PCF output names that
+// accumulate across random files (cuda/pcf3iso.cuh:127), -rd in readdir
+// order (cuda/main.cu:164-185), and 3ani files of random file i holding values
+// i*bins^3 .. (i+1)*bins^3 - 1 of the random files' concatenated bins^5
+// histograms (save_histogram3D with nhistos = i, cuda/pcf3ani.cuh:123-138).
 // Environment: PCF_DATA_DIR / PCF_RESULTS_DIR override ../data/ and
 // ../results/; PCF_PRECISION=fp64 selects the plain FP64 path.
 // PCF_TUNE_CACHE=<file> loads the pair kernels' tuning cache from the file at
@@ -85,7 +93,10 @@ void show_help() {
           "                       compatibility, the engine sizes its own grid.\n"
           "-s size_box:           (optional) side of the box holding the points.\n"
           "-e:                    (optional) also write the Landy-Szalay estimator per random\n"
-          "                       file (LS... for 2PCF, LS3... for 3PCF).\n\n";
+          "                       file (LS... for 2PCF, LS3... for 3PCF).\n"
+          "--ref-bugs:            (optional) reproduce the reference program's defects (weighted\n"
+          "                       3PCF and 2ani DR weights, analytic overflows, file names), to\n"
+          "                       compare with result files it wrote; -p then sets its node grid.\n\n";
 }
 
 struct Opts {
@@ -93,6 +104,7 @@ struct Opts {
   int np = 0, bn = 0, partitions = 35;
   bool bpc = false, analytic = false, rand_dir = false, size_box_provided = false;
   bool estimator = false;
+  bool ref_bugs = false;
   float size_box = 0, dmax = 0;
   string data_name, rand_name;
 };
@@ -217,6 +229,7 @@ int main(int argc, char** argv) {
       else if (a == "-bpc") o.bpc = true;
       else if (a == "-a") o.analytic = true;
       else if (a == "-e") o.estimator = true;
+      else if (a == "--ref-bugs") o.ref_bugs = true;
       else if (a == "-p") o.partitions = (int)std::stof(next());
       else if (a == "-s") {
         o.size_box = std::stof(next());
@@ -279,7 +292,7 @@ int main(int argc, char** argv) {
         if (nm != "." && nm != "..") names.push_back(nm);
       }
       closedir(folder);
-      std::sort(names.begin(), names.end());
+      if (!o.ref_bugs) std::sort(names.begin(), names.end());
       for (auto& nm : names) {
         histo_names.push_back(nm);
         rand_files.push_back(o.rand_name + nm);
@@ -305,6 +318,8 @@ int main(int argc, char** argv) {
 
   int rc = pcf_create(&run.ctx, 0);
   if (rc != PCF_OK) return die(nullptr, rc, "pcf_create");
+  if (o.ref_bugs && (rc = pcf_set_ref_bugs(run.ctx, 1)) != PCF_OK)
+    return die(run.ctx, rc, "pcf_set_ref_bugs");
   const string tune_cache = env_or("PCF_TUNE_CACHE", "");
   run.autotune = env_or("PCF_AUTOTUNE", "0") == "1";
   if (!tune_cache.empty()) {
@@ -395,13 +410,45 @@ int main(int argc, char** argv) {
     }
     if ((rc = H(type, PCF_XXX, cd, -1, "DDD" + tag + dn, &h0)) != PCF_OK)
       return die(run.ctx, rc, "DDD");
+    // --ref-bugs: the names accumulate the random files' names, and 3ani
+    // random file i writes the slice i*bins^3 of the concatenated histograms.
+    string acc_rrr = "RRR" + tag, acc_ddr = "DDR" + tag, acc_drr = "DRR" + tag;
+    std::vector<std::vector<double>> all_rrr, all_ddr, all_drr;
+    const size_t b3 = (size_t)o.bn * o.bn * o.bn;
+    auto slab = [&](const std::vector<std::vector<double>>& all, size_t i) {
+      const size_t off = i * b3, per = all[0].size();
+      std::vector<double> v(b3, 0.0);
+      for (size_t k = 0; k < b3; ++k) {
+        const size_t g = off + k;
+        if (g / per < all.size()) v[k] = all[g / per][g % per];
+      }
+      return v;
+    };
+    auto H3 = [&](int hist, pcf_catalog x, pcf_catalog y, const string& name,
+                  std::vector<double>* keep, std::vector<std::vector<double>>* all,
+                  size_t i) -> int {
+      if (!(o.ref_bugs && type == PCF_3ANI)) return H(type, hist, x, y, name, keep);
+      int r = run.hist(type, hist, x, y, &h);
+      if (r != PCF_OK) return r;
+      *keep = h;
+      all->push_back(h);
+      // Random file i's slice lies in a histogram already computed (i*bins^3
+      // < (i+1)*bins^5).
+      return run.save(PCF_3ISO, name, slab(*all, i)) ? PCF_OK : PCF_ERR_ARG;
+    };
     for (size_t i = 0; i < cr.size(); ++i) {
       const string& rn = histo_names[i + 1];
-      if ((rc = H(type, PCF_XXX, cr[i], -1, "RRR" + tag + rn, &h3)) != PCF_OK)
+      acc_rrr += rn;
+      acc_ddr += rn;
+      acc_drr += rn;
+      const string n_rrr = o.ref_bugs ? acc_rrr : "RRR" + tag + rn;
+      const string n_ddr = o.ref_bugs ? acc_ddr : "DDR" + tag + rn;
+      const string n_drr = o.ref_bugs ? acc_drr : "DRR" + tag + rn;
+      if ((rc = H3(PCF_XXX, cr[i], -1, n_rrr, &h3, &all_rrr, i)) != PCF_OK)
         return die(run.ctx, rc, "RRR");
-      if ((rc = H(type, PCF_XXY, cd, cr[i], "DDR" + tag + rn, &h1)) != PCF_OK)
+      if ((rc = H3(PCF_XXY, cd, cr[i], n_ddr, &h1, &all_ddr, i)) != PCF_OK)
         return die(run.ctx, rc, "DDR");
-      if ((rc = H(type, PCF_XXY, cr[i], cd, "DRR" + tag + rn, &h2)) != PCF_OK)
+      if ((rc = H3(PCF_XXY, cr[i], cd, n_drr, &h2, &all_drr, i)) != PCF_OK)
         return die(run.ctx, rc, "DRR");
       if (o.estimator && (rc = run.estimate(type, {&h0, &h1, &h2, &h3}, "LS3" + tag + rn)) !=
                              PCF_OK)

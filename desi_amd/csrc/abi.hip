@@ -854,10 +854,13 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights,
 
 namespace {
 
-__global__ void k_analytic_rr(double alpha, int bins, double* rr) {
+// wrap32: the reference's int shell count, which wraps from i = 26755 on
+// (pcf_set_ref_bugs); else 64-bit, with no overflow for large bin counts.
+__global__ void k_analytic_rr(double alpha, int bins, double* rr, int wrap32) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < bins) {
-    long long k = 3LL * i * i + 3LL * i + 1;  // 64-bit: no overflow for large bin counts
+    const unsigned u = (unsigned)i;
+    long long k = wrap32 ? (long long)(int)(3u * u * u + 3u * u + 1u) : 3LL * i * i + 3LL * i + 1;
     rr[i] = alpha * (double)k;
   }
 }
@@ -1024,6 +1027,15 @@ int pcf_histogram(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weig
 
 namespace pcf {
 
+// The reference's XY2ani weight defect applies: 2ani DR without periodic
+// boundaries under pcf_set_ref_bugs, with some weight other than 1.
+bool ref_bugs_alias_needed(pcf_ctx* c, const pcf_job* job) {
+  if (!ref_bugs(c) || job->type != PCF_2ANI || job->hist != PCF_XY || job->bpc) return false;
+  Catalog* X = get_cat(c, job->cat_x);
+  Catalog* Y = get_cat(c, job->cat_y);
+  return X && Y && !(X->all_w1 && Y->all_w1);
+}
+
 // One histogram on this context's stream (also the engine lanes' entry).
 int dispatch_histogram(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights) {
   if (!job) return fail(c, PCF_ERR_ARG, "null argument");
@@ -1036,6 +1048,19 @@ int dispatch_histogram(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double*
         pcf_job tuned = *job;
         if (get_cat(c, job->cat_x) && (job->hist == PCF_XX || get_cat(c, job->cat_y)))
           apply_tuning(c, &tuned, 1);
+        if (ref_bugs_alias_needed(c, job)) {
+          // The reference's XY2ani weight (cuda/kernels/2ani_k.cuh:128): the
+          // random points take the data weights of their ordered slots.
+          pcf_catalog alias = -1;
+          int rc = ref_alias_catalog(c, job->cat_x, job->cat_y, job->box, job->partitions, &alias);
+          if (rc != PCF_OK) return rc;
+          tuned.cat_y = alias;
+          rc = run_pairs(c, &tuned, counts, weights, nullptr, 0);
+          std::string keep = c->err;
+          (void)pcf_catalog_release(c, alias);
+          c->err = keep;
+          return rc;
+        }
         return run_pairs(c, &tuned, counts, weights, nullptr, 0);
       }
     case PCF_3ISO:
@@ -1083,17 +1108,20 @@ int pcf_analytic_rr(pcf_ctx* c, int32_t bins, int64_t np, float dmax, float box,
     return fail(c, PCF_ERR_ARG, "bad analytic RR arguments");
   HIPCHK(c, hipSetDevice(c->device));
   // Host constants of cuda/pcf2isoBPC.cuh:199-202 with FP32 dr and beta as the
-  // reference forms them, np*np in 64-bit.
+  // reference forms them, np*np in 64-bit (its 32-bit wrap under
+  // pcf_set_ref_bugs).
+  const bool wrap = ref_bugs(c);
   float drf = dmax / (float)bins;
   double dr = drf;
-  float np2 = (float)((double)np * (double)np);
+  float np2 = wrap ? (float)(int32_t)(uint32_t)((uint64_t)np * (uint64_t)np)
+                   : (float)((double)np * (double)np);
   float L3 = box * box * box;
   float betaf = np2 / L3;
   double beta = betaf;
   double alpha = 8 * dr * dr * dr * std::acos(0.0) * beta / 3;
   HIPCHK(c, c->misc_buf.ensure(sizeof(double) * bins));
   hipLaunchKernelGGL(k_analytic_rr, dim3((bins + 255) / 256), dim3(256), 0, c->stream, alpha,
-                     (int)bins, c->misc_buf.as<double>());
+                     (int)bins, c->misc_buf.as<double>(), wrap ? 1 : 0);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(rr, c->misc_buf.p, sizeof(double) * bins, hipMemcpyDeviceToHost,
                            c->stream));

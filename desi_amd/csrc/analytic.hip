@@ -93,10 +93,21 @@ __global__ __launch_bounds__(kFineBlock) void k_fine_dd(const float4* __restrict
   }
 }
 
+// Shell count 3q^2+3q+1 of the fine RR: 64-bit, or (wrap32, under
+// pcf_set_ref_bugs) the reference's int, which wraps from q = 26755 on
+// (cuda/kernels/2isoBPC_k.cuh:572).
+__device__ __forceinline__ double shell_count(long long q, int wrap32) {
+  if (wrap32) {
+    const unsigned u = (unsigned)q;
+    return (double)(int)(3u * u * u + 3u * u + 1u);
+  }
+  return (double)(3 * q * q + 3 * q + 1);
+}
+
 // ff_av[i] = sum_j (ri + rj) (DD/RR - 1) / ptt over the ptt fine bins of coarse
-// bin i; RR = alpha*(3q^2+3q+1) (64-bit).  cuda/kernels/3isoBPC_k.cuh:1984.
+// bin i; RR = alpha*(3q^2+3q+1).  cuda/kernels/3isoBPC_k.cuh:1984.
 __global__ void k_ff_av(const double* __restrict__ dd, double alpha, float dmax, int bins,
-                        int ptt, double* __restrict__ ff) {
+                        int ptt, double* __restrict__ ff, int wrap32) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= bins) return;
   const int nb1 = ptt * bins;
@@ -105,7 +116,7 @@ __global__ void k_ff_av(const double* __restrict__ dd, double alpha, float dmax,
   for (int j = 0; j < ptt; ++j) {
     const double rj = __ddiv_rn(__dmul_rn((double)j + 0.5, (double)dmax), (double)nb1);
     const long long q = (long long)i * ptt + j;
-    const double rr = __dmul_rn(alpha, (double)(3 * q * q + 3 * q + 1));
+    const double rr = __dmul_rn(alpha, shell_count(q, wrap32));
     s = __dadd_rn(s, __ddiv_rn(__dmul_rn(__dadd_rn(ri, rj), __dsub_rn(__ddiv_rn(dd[q], rr), 1.0)),
                                (double)ptt));
   }
@@ -115,7 +126,7 @@ __global__ void k_ff_av(const double* __restrict__ dd, double alpha, float dmax,
 // ff_av_ref[i*bn_ref + j] over the ptt finest bins of sub-bin (i, j);
 // cuda/kernels/3isoBPC_k.cuh:2001.
 __global__ void k_ff_av_ref(const double* __restrict__ dd, double alpha, float dmax, int bins,
-                            int bn_ref, int ptt, double* __restrict__ ffr) {
+                            int bn_ref, int ptt, double* __restrict__ ffr, int wrap32) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= bins * bn_ref) return;
   const int i = t / bn_ref, j = t % bn_ref;
@@ -127,7 +138,7 @@ __global__ void k_ff_av_ref(const double* __restrict__ dd, double alpha, float d
   for (int k = 0; k < ptt; ++k) {
     const double rk = __dmul_rn((double)k + 0.5, dr_ptt);
     const long long q = ((long long)i * bn_ref) * ptt + (long long)j * ptt + k;
-    const double rr = __dmul_rn(alpha, (double)(3 * q * q + 3 * q + 1));
+    const double rr = __dmul_rn(alpha, shell_count(q, wrap32));
     s = __dadd_rn(s, __ddiv_rn(__dmul_rn(__dadd_rn(__dadd_rn(ri, rj), rk),
                                          __dsub_rn(__ddiv_rn(dd[q], rr), 1.0)),
                                (double)ptt));
@@ -236,10 +247,11 @@ __global__ __launch_bounds__(kCubeBlock) void k_cubes(const double* __restrict__
 
 // alpha of the fine analytic RR with nb bins (cuda/pcf3isoBPC.cuh:233-238):
 // FP32 dr = dmax/nb and beta = np^2/L^3 as the reference forms them, np^2 in
-// 64-bit.
-double fine_alpha(float dmax, int64_t nb, int64_t np, float box) {
+// 64-bit (wrap32: the reference's 32-bit int, pcf_set_ref_bugs).
+double fine_alpha(float dmax, int64_t nb, int64_t np, float box, bool wrap32) {
   const double dr = (double)(float)(dmax / (float)nb);
-  const float np2 = (float)((double)np * (double)np);
+  const float np2 = wrap32 ? (float)(int32_t)(uint32_t)((uint64_t)np * (uint64_t)np)
+                           : (float)((double)np * (double)np);
   const float L3 = box * box * box;
   const double beta = (double)(float)(np2 / L3);
   return 8 * dr * dr * dr * std::acos(0.0) * beta / 3;
@@ -327,13 +339,14 @@ int run_analytic_3iso(pcf_ctx* c, pcf_catalog data, int bins, float dmax, float 
   // The work list holds each unordered pair once and the ordered sums are
   // twice that.  Halving alpha instead is exact (a power of two), so
   // DD/(RR/2) equals (2 DD)/RR bit for bit.
-  const double a1 = fine_alpha(dmax, nb1, np, box) * 0.5;
-  const double a2 = fine_alpha(dmax, nb2, np, box) * 0.5;
+  const bool wrap = ref_bugs(c);
+  const double a1 = fine_alpha(dmax, nb1, np, box, wrap) * 0.5;
+  const double a2 = fine_alpha(dmax, nb2, np, box, wrap) * 0.5;
   hipLaunchKernelGGL(k_ff_av, dim3((bins + 127) / 128), dim3(128), 0, c->stream, d_dd1, a1, dmax,
-                     bins, ptt, d_ff);
+                     bins, ptt, d_ff, wrap ? 1 : 0);
   HIPCHK(c, hipGetLastError());
   hipLaunchKernelGGL(k_ff_av_ref, dim3((bins * bn_ref + 127) / 128), dim3(128), 0, c->stream,
-                     d_dd2, a2, dmax, bins, bn_ref, ptt, d_ffr);
+                     d_dd2, a2, dmax, bins, bn_ref, ptt, d_ffr, wrap ? 1 : 0);
   HIPCHK(c, hipGetLastError());
 
   // Cube constants (cuda/pcf3isoBPC.cuh:240-245).

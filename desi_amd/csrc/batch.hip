@@ -499,6 +499,9 @@ int run_pairs_batch(pcf_ctx* c, const pcf_job* jobs_in, int njobs, uint64_t* cou
     return fail(c, PCF_ERR_UNSUPPORTED, "batched launches are for pair histograms");
   for (int k = 0; k < njobs; ++k) {
     const pcf_job& j = jobs[k];
+    if (ref_bugs_alias_needed(c, &j))
+      return fail(c, PCF_ERR_UNSUPPORTED,
+                  "reference 2ani DR weights (pcf_set_ref_bugs) take pcf_histogram calls");
     if (j.type != j0.type || j.bpc != j0.bpc || (j0.bpc && j.box != j0.box) ||
         j.precision != j0.precision || j.cells_per_rmax != j0.cells_per_rmax ||
         j.lds_copies_log2 != j0.lds_copies_log2 || j.blocks_per_cu != j0.blocks_per_cu)

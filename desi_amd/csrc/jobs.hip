@@ -59,6 +59,8 @@ bool batchable(const pcf_job& a) {
          (a.hist == PCF_XX || a.hist == PCF_XY) && !a.rebuild_grid;
 }
 
+// (Batching is also off under pcf_set_ref_bugs, whose 2ani DR weights are
+// formed per call.)
 bool compatible(const pcf_job& a, const pcf_job& b) {
   return batchable(a) && batchable(b) && a.type == b.type && a.bpc == b.bpc &&
          (!a.bpc || a.box == b.box) && a.cells_per_rmax == b.cells_per_rmax &&
@@ -104,7 +106,7 @@ struct JobEngine {
         take.push_back(t);
         bytes += t->bytes;
         // Continuous batching: compatible pair jobs queued behind it.
-        if (max_batch > 1 && batchable(t->job)) {
+        if (max_batch > 1 && batchable(t->job) && !ref_bugs(parent)) {
           for (auto it = queue.begin(); it != queue.end() && (int)take.size() < max_batch;) {
             if (compatible(t->job, (*it)->job) && inflight_bytes + bytes + (*it)->bytes <= budget) {
               bytes += (*it)->bytes;

@@ -69,6 +69,9 @@ extern "C" int pcf_histogram_noalloc(pcf_ctx* c, const pcf_job* job, uint64_t* c
     return fail(c, PCF_ERR_UNSUPPORTED, "pcf_histogram_noalloc computes pair histograms only");
   if (job->hist != PCF_XX && job->hist != PCF_XY)
     return fail(c, PCF_ERR_ARG, "pair types take PCF_XX or PCF_XY");
+  if (ref_bugs_alias_needed(c, job))
+    return fail(c, PCF_ERR_UNSUPPORTED,
+                "reference 2ani DR weights (pcf_set_ref_bugs) allocate; use pcf_histogram");
   HIPCHK(c, hipSetDevice(c->device));
   if (c->device < 0 || c->device >= kMaxDevices)
     return fail(c, PCF_ERR_ARG, "device index out of range");

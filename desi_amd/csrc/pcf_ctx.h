@@ -2,6 +2,7 @@
 // shared by the pair, triplet and serving translation units.
 #pragma once
 
+#include <atomic>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -32,6 +33,9 @@ struct CatalogStore {
   std::map<std::string, pcf_tuning> tuned;
   uint64_t next_serial = 1;
   std::mutex mu;
+  // pcf_set_ref_bugs: reproduce the reference defects that depend on its node
+  // order and 32-bit arithmetic (read by the context and its serving lanes).
+  std::atomic<bool> ref_bugs{false};
 };
 }  // namespace pcf
 
@@ -147,10 +151,23 @@ int run_pairs_batch(pcf_ctx* c, const pcf_job* jobs, int njobs, uint64_t* counts
 // Defined in abi.hip: one histogram on c's stream (pcf_histogram's body, also
 // run by the serving lanes), and a serving lane of a context.
 int dispatch_histogram(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights);
+bool ref_bugs_alias_needed(pcf_ctx* c, const pcf_job* job);
 int create_lane(pcf_ctx* parent, pcf_ctx** out);
 
 // Defined in tune.hip.
 void apply_tuning(pcf_ctx* c, pcf_job* jobs, int njobs);
+
+// Defined in refbugs.hip: the reference's node order of a catalog (stable sort
+// of its node ids on partitions^3 nodes of side box / partitions): order[q]
+// is the catalog index in slot q, node_start[n] the first slot of node n
+// (nodes + 1 entries), sorted_pts (may be null) the points in slot order;
+// and a catalog whose weights are the ones XY2ani reads for `rand`.
+int ref_partitions(int partitions);
+int build_ref_order(pcf_ctx* c, const Catalog* X, float box, int partitions, DevBuf* order,
+                    DevBuf* node_start, DevBuf* sorted_pts);
+int ref_alias_catalog(pcf_ctx* c, pcf_catalog data, pcf_catalog rand, float box, int partitions,
+                      pcf_catalog* out);
+inline bool ref_bugs(const pcf_ctx* c) { return c->store && c->store->ref_bugs.load(); }
 
 // Defined in analytic.hip.
 int run_analytic_3iso(pcf_ctx* c, pcf_catalog data, int bins, float dmax, float box, int64_t np,

@@ -121,6 +121,15 @@ struct DevBuf {
   T* as() const { return reinterpret_cast<T*>(p); }
 };
 
+// A DevBuf freed when it leaves scope (per-call temporaries; hipFree waits
+// for the device work that may still use it).
+struct ScopedDevBuf : DevBuf {
+  ScopedDevBuf() = default;
+  ScopedDevBuf(const ScopedDevBuf&) = delete;
+  ScopedDevBuf& operator=(const ScopedDevBuf&) = delete;
+  ~ScopedDevBuf() { release(); }
+};
+
 // Page-locked host staging memory (small uploads and result read-backs that
 // stay asynchronous on the stream).
 struct PinnedBuf {

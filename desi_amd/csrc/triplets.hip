@@ -1079,6 +1079,159 @@ __global__ __launch_bounds__(kTripBlock) void k_trip_ani_lds(
   if (lane == 0 && nre) atomicAdd(&stats[1], nre);
 }
 
+// Reference-defect triplet weights (pcf_set_ref_bugs): the reference's triplet
+// kernels set v = w1 * w2 (an FP32 product) for each (point 1, point 2) and,
+// walking node 3's points in order, multiply v by w3 before each deposit and
+// deposit v (cuda/kernels/3iso_k.cuh:61,79,168, 3isoBPC_k.cuh:60,88,179,
+// 3ani_k.cuh:94,197, 3aniBPC_k.cuh:98), so the weight of a triplet is the
+// product over the third points deposited before it in its node (and
+// projection) -- a chain that restarts at each node.  Counts are unchanged.
+// A wave owns point 1 and walks the reference nodes within reach of it (the
+// third-point catalog in the reference's node order, refbugs.hip) in
+// lockstep; its lanes are point 1's pairs, each with its own chain, in FP64
+// exact arithmetic (the plain path's rules).  3iso pairs are unordered:
+// (j, i) deposits the same values into the (b23, b31)-transposed bins (the
+// chains of the two orders hold the same third points: relabelling 1 and 2
+// maps projection (f2, f3) to (f2, f2 ^ f3)), so k_sym_bc completes the
+// histogram; 3ani's list holds both orders.
+struct RefArgs {
+  int nx;          // points 1
+  int kstride;     // pair-list keys per point 1
+  int bins, auto_, bpc, P;
+  float size_node, reach;
+  double R, L, ds_th;
+};
+
+template <int ANI>
+__global__ __launch_bounds__(256) void k_trip_refbugs(
+    const float4* __restrict__ xo, const float4* __restrict__ kref, const int* __restrict__ ns,
+    const PairRec* __restrict__ pairs, const int* __restrict__ poff, int* __restrict__ counter,
+    const double* __restrict__ T, unsigned long long* __restrict__ hist,
+    double* __restrict__ whist, unsigned long long* __restrict__ stats, RefArgs a) {
+  __shared__ double sT[kMaxTripBins + 2];
+  for (int t = threadIdx.x; t <= a.bins; t += blockDim.x) sT[t] = T[t];
+  __syncthreads();
+  const int lane = threadIdx.x & (kWave - 1);
+  const int B = a.bins, P = a.P;
+  const float sn = a.size_node;
+  unsigned long long evals = 0;
+  auto node_axis = [&](float v) {
+    const int c = (int)(v / sn);
+    return c < 0 ? 0 : (c >= P ? P - 1 : c);
+  };
+  for (;;) {
+    int task = 0;
+    if (lane == 0) task = atomicAdd(counter, 1);
+    const int i = __builtin_amdgcn_readfirstlane(task);
+    if (i >= a.nx) break;
+    const int pb = __builtin_amdgcn_readfirstlane(poff[(int64_t)i * a.kstride]);
+    const int pe = __builtin_amdgcn_readfirstlane(poff[(int64_t)(i + 1) * a.kstride]);
+    if (pe <= pb) continue;
+    const float4 p1 = xo[i];
+    const float a1[3] = {p1.x, p1.y, p1.z};
+    const int cx = node_axis(p1.x), cy = node_axis(p1.y), cz = node_axis(p1.z);
+    int K = (int)ceilf(a.reach / sn) + 1;
+    if (a.bpc) K = min(K, P);
+    const float r2 = a.reach * a.reach;
+    for (int pc = pb; pc < pe; pc += kWave) {
+      const bool act = pc + lane < pe;
+      PairRec r;
+      r.j = i;
+      r.meta = 13;  // image code of (0, 0, 0)
+      r.wj = 0.f;
+      if (act) r = pairs[pc + lane];
+      const float4 p2 = xo[r.j];
+      const int f2 = act ? image_flags(p1, p2, r.meta & 31, false) : 0;
+      const int b12 = (r.meta >> 8) & 0xfff, t12 = (r.meta >> 20) & 0xfff;
+      const float w12 = __fmul_rn(p1.w, p2.w);  // FP32, as the reference forms v
+      const float a2[3] = {p2.x, p2.y, p2.z};
+      const unsigned long long nact = (unsigned long long)__popcll(__ballot(act));
+      for (int dx = -K; dx <= K; ++dx) {
+        const int vx = cx + dx;
+        int ccx = vx, sx = 0;
+        if (vx < 0 || vx >= P) {
+          if (!a.bpc) continue;
+          sx = vx < 0 ? -1 : 1;
+          ccx = vx - sx * P;
+        }
+        const float x0 = (float)vx * sn;
+        const float gx = fmaxf(0.f, fmaxf(x0 - p1.x, p1.x - (x0 + sn))) * 0.99999f;
+        for (int dy = -K; dy <= K; ++dy) {
+          const int vy = cy + dy;
+          int ccy = vy, sy = 0;
+          if (vy < 0 || vy >= P) {
+            if (!a.bpc) continue;
+            sy = vy < 0 ? -1 : 1;
+            ccy = vy - sy * P;
+          }
+          const float y0 = (float)vy * sn;
+          const float gy = fmaxf(0.f, fmaxf(y0 - p1.y, p1.y - (y0 + sn))) * 0.99999f;
+          const float lat = gx * gx + gy * gy;
+          if (lat >= r2) continue;
+          const float hz = sqrtf(r2 - lat) * 1.00001f;
+          const int zlo = max(cz - K, (int)floorf((p1.z - hz) / sn) - 1);
+          const int zhi = min(cz + K, (int)floorf((p1.z + hz) / sn) + 1);
+          for (int vz = zlo; vz <= zhi; ++vz) {
+            int ccz = vz, sz = 0;
+            if (vz < 0 || vz >= P) {
+              if (!a.bpc) continue;
+              sz = vz < 0 ? -1 : 1;
+              ccz = vz - sz * P;
+            }
+            const int node = (ccx * P + ccy) * P + ccz;
+            const int q0 = ns[node], q1 = ns[node + 1];
+            if (q1 <= q0) continue;
+            const int img3 = (sx + 1) * 9 + (sy + 1) * 3 + (sz + 1);
+            double v = (double)w12;  // the chain restarts at each node (and projection)
+            for (int q = q0; q < q1; ++q) {
+              const float4 p3 = kref[q];
+              const int f3 = image_flags(p1, p3, img3, false);
+              if (f3 < 0) continue;
+              const float a3[3] = {p3.x, p3.y, p3.z};
+              double t31[3], t23[3];
+              for (int ax = 0; ax < 3; ++ax) {
+                const double d = __dsub_rn((double)a3[ax], (double)a1[ax]);
+                t31[ax] = a.bpc ? proj_term(d, a.L, (f3 >> ax) & 1) : d;
+              }
+              const double s31 = __dadd_rn(
+                  __dadd_rn(__dmul_rn(t31[0], t31[0]), __dmul_rn(t31[1], t31[1])),
+                  __dmul_rn(t31[2], t31[2]));
+              if (a.auto_ && !(s31 > 0.0)) continue;
+              const int b31 = class_lds(s31, sT, B, a.R, 0);
+              if (b31 < 0) continue;
+              int pbin = 0;
+              if (ANI) {
+                const double pp = __dmul_rn(__dadd_rn(1.0, t31[2] / sqrt(s31)), a.ds_th);
+                pbin = isnan(pp) ? 0 : (int)pp;
+                if (pbin > B - 1) continue;
+              }
+              evals += nact;
+              for (int ax = 0; ax < 3; ++ax) {
+                const double d = __dsub_rn((double)a3[ax], (double)a2[ax]);
+                t23[ax] = a.bpc ? proj_term(d, a.L, ((f2 >> ax) & 1) != ((f3 >> ax) & 1)) : d;
+              }
+              const double s23 = __dadd_rn(
+                  __dadd_rn(__dmul_rn(t23[0], t23[0]), __dmul_rn(t23[1], t23[1])),
+                  __dmul_rn(t23[2], t23[2]));
+              int b23 = -1;
+              if (act && !(a.auto_ && !(s23 > 0.0))) b23 = class_lds(s23, sT, B, a.R, 0);
+              if (b23 >= 0) {
+                v = __dmul_rn(v, (double)p3.w);
+                const int64_t idx =
+                    ANI ? ((((int64_t)b12 * B + b31) * B + b23) * B + t12) * B + pbin
+                        : ((int64_t)b12 * B + b23) * B + b31;
+                atomicAdd(&hist[idx], 1ull);
+                atomicAdd(&whist[idx], v);
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  if (lane == 0) atomicAdd(&stats[0], evals);
+}
+
 // XXX 3iso: point 1 is the lowest index of its triplet (pairs i < j, third
 // points k > i).  An unordered triplet {p < q < r} with sides A = b(pq),
 // B = b(qr), C = b(rp) is met as (A, B, C) (pair (p, q), third r) and as
@@ -1168,7 +1321,14 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   // axis, is no longer the separation of the two images in point 1's frame
   // (both points projected toward opposite walls, say), which the FP32 bulk
   // forms; such jobs take the plain FP64 kernels, which evaluate that rule.
-  const bool plain = job->precision == PCF_PREC_FP64 || (bpc && !(2.0 * job->dmax < L));
+  // pcf_set_ref_bugs with weights: the reference's cumulative triplet weight
+  // (k_trip_refbugs, FP64 exact rules, reference node order of K).
+  const bool refchain = ref_bugs(c) && weighted;
+  if (refchain && !(L > 0))
+    return fail(c, PCF_ERR_ARG,
+                "reference triplet weights need the box side (the node grid is box / partitions)");
+  const bool plain = job->precision == PCF_PREC_FP64 || (bpc && !(2.0 * job->dmax < L)) ||
+                     refchain;
 
   GridGeom g;
   int Kc[3];
@@ -1324,8 +1484,37 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   t.glob = 0;
   for (int q = 0; q < 3; ++q) t.K[q] = Kc[q];
   t.g = g;
+  // The reference's node order of the third-point catalog (refchain only).
+  ScopedDevBuf ref_order, ref_start, ref_pts;
+  if (refchain) {
+    int rr = build_ref_order(c, K, L, job->partitions, &ref_order, &ref_start, &ref_pts);
+    if (rr != PCF_OK) return rr;
+  }
   HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-  if (npairs > 0) {
+  if (npairs > 0 && refchain) {
+    RefArgs ra;
+    ra.nx = (int)nx;
+    ra.kstride = ani_lds ? bins * bins : bins;
+    ra.bins = bins;
+    ra.auto_ = auto_ ? 1 : 0;
+    ra.bpc = bpc;
+    ra.P = ref_partitions(job->partitions);
+    ra.size_node = L / (float)ra.P;
+    ra.reach = reach;
+    ra.R = kr.R;
+    ra.L = (double)L;
+    ra.ds_th = (double)ds_th_f;
+    HIPCHK(c, launch_zero(d_counter, sizeof(int), c->stream));
+    if (ani)
+      hipLaunchKernelGGL(k_trip_refbugs<1>, dim3(c->ncu * 8), dim3(256), 0, c->stream, gx->pts,
+                         ref_pts.as<float4>(), ref_start.as<int>(), d_pairs, d_off, d_counter,
+                         d_T, d_V, d_VW, d_stats, ra);
+    else
+      hipLaunchKernelGGL(k_trip_refbugs<0>, dim3(c->ncu * 8), dim3(256), 0, c->stream, gx->pts,
+                         ref_pts.as<float4>(), ref_start.as<int>(), d_pairs, d_off, d_counter,
+                         d_T, d_V, d_VW, d_stats, ra);
+    HIPCHK(c, hipGetLastError());
+  } else if (npairs > 0) {
     if (!ani) {
       // LDS budget of one 1024-thread workgroup per CU: the slice of b12 rows
       // whose counts (and FP64 weights) fit beside the table.
@@ -1497,7 +1686,9 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   const unsigned long long* outH = d_V;
   const double* outW = d_VW;
   if (!ani) {
-    if (auto_)
+    // (The reference-weight kernel takes every third point: its V is
+    // completed by the (b23, b31) transpose even for XXX.)
+    if (auto_ && !refchain)
       hipLaunchKernelGGL(k_sym_cyc, dim3((unsigned)((nflat + 255) / 256)), dim3(256), 0,
                          c->stream, d_V, d_VW, bins, d_H, d_HW);
     else

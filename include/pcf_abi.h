@@ -84,7 +84,8 @@ typedef struct pcf_job {
   int32_t precision;   /* PCF_PREC_MIXED or PCF_PREC_FP64                 */
   pcf_catalog cat_x;   /* X: the auto catalog, or D of a cross histogram  */
   pcf_catalog cat_y;   /* Y: R of a cross histogram; -1 for XX / XXX      */
-  int32_t partitions;  /* -p; informational                               */
+  int32_t partitions;  /* -p; informational, except under pcf_set_ref_bugs */
+                       /* (the reference's node grid, partitions per side) */
   int32_t cells_per_rmax; /* cell edge = rmax / cells_per_rmax; 0 = the    */
                           /* autotuned value (pcf_autotune) or a default  */
   int32_t rebuild_grid;   /* 1: re-grid the catalogs even if resident      */
@@ -225,6 +226,25 @@ int pcf_tune_save(pcf_ctx* ctx, const char* path);
 int pcf_tune_load(pcf_ctx* ctx, const char* path);
 int pcf_tune_lookup(pcf_ctx* ctx, const pcf_job* jobs, int32_t njobs, pcf_tuning* out);
 
+/* Reproduce reference defects (on != 0; the PCF command line's --ref-bugs),
+ * for drop-in comparison with result files the reference wrote.  No
+ * counterpart call in the reference, whose GPU code has these defects:
+ *   - weighted triplet histograms take the cumulative weight of the triplet
+ *     kernels (cuda/kernels/3iso_k.cuh:79,168, 3isoBPC_k.cuh:88,179,
+ *     3ani_k.cuh:94,197, 3aniBPC_k.cuh:98): v = w1*w2 (FP32) per (point 1,
+ *     point 2), multiplied by w3 of each third point deposited before it in its
+ *     node, in the node's point order, the node grid being pcf_job.partitions
+ *     (default 35) nodes per side of pcf_job.box (needed, also without -bpc);
+ *   - 2ani DR (no periodic boundaries) weights each pair with the data weight
+ *     at the random point's slot of the ordered arrays (cuda/kernels/2ani_k.cuh:128);
+ *     refused when the random catalog is longer than the data catalog (the
+ *     reference reads past its data array), and by pcf_histogram_batch and
+ *     pcf_histogram_noalloc;
+ *   - the analytic normalisations use 32-bit int np*np (cuda/pcf2isoBPC.cuh:201,
+ *     cuda/pcf3isoBPC.cuh:236) and 3i^2+3i+1 (cuda/kernels/2isoBPC_k.cuh:572).
+ * Counts are unchanged.  Applies to the context and its serving lanes, whose
+ * continuous batching is then off. */
+int pcf_set_ref_bugs(pcf_ctx* ctx, int32_t on);
 /* Statistics of the last pcf_histogram / pcf_wait on this context. */
 int pcf_last_stats(const pcf_ctx* ctx, pcf_stats* out);
 

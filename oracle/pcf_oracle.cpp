@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 #ifdef _OPENMP
@@ -245,6 +246,18 @@ int nthreads_or(int nt) {
 
 }  // namespace
 
+// The value of a 64-bit integer product or sum as the reference's 32-bit
+// signed int holds it (two's complement wrap, what nvcc and g++ produce for
+// the reference's overflowing int expressions).
+inline int32_t wrap_i32(uint64_t v) { return (int32_t)(uint32_t)(v & 0xffffffffull); }
+
+// 3i^2 + 3i + 1, the shell count of the analytic RR, exact or wrapped.
+inline int64_t shell_int(int64_t i, bool wrap32) {
+  if (!wrap32) return 3 * i * i + 3 * i + 1;
+  const uint32_t u = (uint32_t)i;
+  return (int64_t)(int32_t)(3u * u * u + 3u * u + 1u);
+}
+
 extern "C" {
 
 // Ordered-pair 2PCF histogram in the FP64 canonical mode.
@@ -415,18 +428,27 @@ int pcf_oracle_cpp32_2iso(const float* a_, int64_t na, const float* b_, int64_t 
 // The reference's int overflows (np*np, 3i^2+3i+1) are replaced by 64-bit
 // arithmetic.  Pinned by the shell-volume closed form (4pi/3) N^2 / L^3
 // ((i+1)^3 - i^3) dr^3 and by the sum over bins.
-void pcf_oracle_analytic_rr(int bins, int64_t np, float dmax, float box, double* rr) {
+static void analytic_rr(int bins, int64_t np, float dmax, float box, bool wrap32, double* rr) {
   float drf = dmax / (float)bins;
   double dr = drf;
-  float np2 = (float)((double)np * (double)np);
+  float np2 = wrap32 ? (float)wrap_i32((uint64_t)np * (uint64_t)np)
+                     : (float)((double)np * (double)np);
   float L3 = box * box * box;
   float betaf = np2 / L3;
   double beta = betaf;
   double alpha = 8 * dr * dr * dr * std::acos(0.0) * beta / 3;
-  for (int i = 0; i < bins; ++i) {
-    int64_t k = 3 * (int64_t)i * i + 3 * (int64_t)i + 1;
-    rr[i] = alpha * (double)k;
-  }
+  for (int i = 0; i < bins; ++i) rr[i] = alpha * (double)shell_int(i, wrap32);
+}
+
+void pcf_oracle_analytic_rr(int bins, int64_t np, float dmax, float box, double* rr) {
+  analytic_rr(bins, np, dmax, box, false, rr);
+}
+
+// The same with the reference's 32-bit int arithmetic (its defects): np*np
+// (cuda/pcf2isoBPC.cuh:201) and 3i^2+3i+1 (cuda/kernels/2isoBPC_k.cuh:572)
+// wrap modulo 2^32 into a signed int.  Pinned by hand-computed wrapped values.
+void pcf_oracle_analytic_rr_refbugs(int bins, int64_t np, float dmax, float box, double* rr) {
+  analytic_rr(bins, np, dmax, box, true, rr);
 }
 
 
@@ -775,9 +797,9 @@ extern "C" {
 // ff_av (cpp/3PCFisotropic/BPC, FP32 distances: agreement to the fine-bin
 // edge-flip noise); GPU parity at reduced ptt/bn_ref.  The sub-cube refinement
 // itself is formula-pinned (no runnable reference output exists for it).
-int pcf_oracle_analytic_3iso(const float* a_, int64_t na, int bins, float dmax, float box,
-                             int64_t np, int ptt, int bn_ref, int nthreads, double* ddr,
-                             double* rrr, double* ff_av_out, double* ff_av_ref_out) {
+static int analytic_3iso(const float* a_, int64_t na, int bins, float dmax, float box,
+                         int64_t np, int ptt, int bn_ref, int nthreads, double* ddr, double* rrr,
+                         double* ff_av_out, double* ff_av_ref_out, bool wrap32) {
   if (bins <= 0 || ptt <= 0 || bn_ref <= 0 || !(dmax < box)) return -1;
   const Pt* a = reinterpret_cast<const Pt*>(a_);
   const int64_t nb1 = (int64_t)ptt * bins, nb2 = (int64_t)ptt * bn_ref * bins;
@@ -821,14 +843,16 @@ int pcf_oracle_analytic_3iso(const float* a_, int64_t na, int bins, float dmax, 
       for (int64_t k = 0; k < nb2; ++k) DD2[k] += L2[k];
     }
   }
-  // Fine analytic RR (cuda/pcf3isoBPC.cuh:233-238).
-  const float np2 = (float)((double)np * (double)np);
+  // Fine analytic RR (cuda/pcf3isoBPC.cuh:233-238); wrap32: the
+  // reference's 32-bit np*np and 3i^2+3i+1.
+  const float np2 = wrap32 ? (float)wrap_i32((uint64_t)np * (uint64_t)np)
+                           : (float)((double)np * (double)np);
   const float L3 = box * box * box;
   const double beta = (double)(float)(np2 / L3);
   auto rr_of = [&](int64_t nb, int64_t i) {
     const double dr = (double)(float)(dmax / (float)nb);
     const double alpha = 8 * dr * dr * dr * std::acos(0.0) * beta / 3;
-    return alpha * (double)(3 * i * i + 3 * i + 1);
+    return alpha * (double)shell_int(i, wrap32);
   };
   std::vector<double> ff(bins, 0.0), ffr((size_t)bins * bn_ref, 0.0);
   for (int i = 0; i < bins; ++i) {
@@ -919,6 +943,125 @@ int pcf_oracle_analytic_3iso(const float* a_, int64_t na, int bins, float dmax, 
     }
   }
   return 0;
+}
+
+
+int pcf_oracle_analytic_3iso(const float* a_, int64_t na, int bins, float dmax, float box,
+                             int64_t np, int ptt, int bn_ref, int nthreads, double* ddr,
+                             double* rrr, double* ff_av_out, double* ff_av_ref_out) {
+  return analytic_3iso(a_, na, bins, dmax, box, np, ptt, bn_ref, nthreads, ddr, rrr, ff_av_out,
+                       ff_av_ref_out, false);
+}
+
+// The same with the reference's 32-bit np*np (cuda/pcf3isoBPC.cuh:236) and
+// 3i^2+3i+1 of its fine RR (cuda/kernels/2isoBPC_k.cuh:572).
+int pcf_oracle_analytic_3iso_refbugs(const float* a_, int64_t na, int bins, float dmax,
+                                     float box, int64_t np, int ptt, int bn_ref, int nthreads,
+                                     double* ddr, double* rrr, double* ff_av_out,
+                                     double* ff_av_ref_out) {
+  return analytic_3iso(a_, na, bins, dmax, box, np, ptt, bn_ref, nthreads, ddr, rrr, ff_av_out,
+                       ff_av_ref_out, true);
+}
+
+// ---------------------------------------------------------------------------
+// Reference defects reproduced on request (--ref-bugs).  Both depend on the
+// reference's node grid: partitions^3 nodes of side size_box / partitions
+// (FP32, cuda/main.cu:216), a point in node ((int)(x / size_node), ...)
+// (cuda/create_grid.cuh:133-135; clamped to the last node where FP32 rounding
+// of a coordinate just below the box side would reach past it), the nodes in
+// row-major (x, y, z) order and a node's points in file order (add() appends,
+// create_grid.cuh:81-102; cuda/main.cu:276-300 copies node by node).
+// ---------------------------------------------------------------------------
+}  // extern "C"
+namespace {
+int64_t ref_node_of(const Pt& p, float size_node, int P) {
+  auto ax = [&](float v) {
+    int c = (int)(v / size_node);
+    return c < 0 ? 0 : (c >= P ? P - 1 : c);
+  };
+  return ((int64_t)ax(p.x) * P + ax(p.y)) * P + ax(p.z);
+}
+// Indices of the catalog in the reference's ordered array (node order, file
+// order inside a node).
+std::vector<int64_t> ref_order(const Pt* p, int64_t n, float box, int partitions) {
+  const float size_node = box / (float)partitions;
+  std::vector<int64_t> node(n), ord(n);
+  for (int64_t i = 0; i < n; ++i) node[i] = ref_node_of(p[i], size_node, partitions);
+  for (int64_t i = 0; i < n; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) { return node[a] < node[b]; });
+  return ord;
+}
+}  // namespace
+extern "C" {
+
+// Triplet histogram with the reference GPU kernels' cumulative weight: in
+// count123 and the XXX/XXY kernels (cuda/kernels/3iso_k.cuh:61,79,168,
+// 3isoBPC_k.cuh:60,88,179, 3ani_k.cuh:94,197, 3aniBPC_k.cuh:98) v = w1*w2 (an
+// FP32 product) is set for each (point 1, point 2) and multiplied by w3 of
+// every third point deposited so far in the same node 3 under the same
+// projection pair (f2, f3), in the node's point order, before each deposit;
+// the deposit is v, not w1*w2*w3.  Counts are those of pcf_oracle_triplets.
+// Literal form: every ordered (i, j), every (f2, f3) and every node of the
+// third-point catalog's reference grid walked in order.  O(na^2 nb 64); small
+// inputs.  Pinned by unit weights (equal to pcf_oracle_triplets) and
+// hand-worked chains (tests/test_oracle_refbugs.py).
+int pcf_oracle_triplets_refbugs(const float* a_, int64_t na, const float* b_, int64_t nb,
+                                int auto_, int ani, int bpc, float box, int partitions,
+                                int bins, float dmax, uint64_t* counts, double* wsum) {
+  if (bins <= 0 || !(dmax > 0) || !(box > 0) || partitions <= 0) return -1;
+  if (bpc && !(dmax < box)) return -2;
+  const Pt* a = reinterpret_cast<const Pt*>(a_);
+  const Pt* b = auto_ ? a : reinterpret_cast<const Pt*>(b_);
+  if (auto_) nb = na;
+  TripRule r = make_trip_rule(ani, bpc, auto_, bins, dmax, box);
+  const int64_t bn = bins;
+  const int64_t nout = ani ? bn * bn * bn * bn * bn : bn * bn * bn;
+  std::memset(counts, 0, sizeof(uint64_t) * nout);
+  std::memset(wsum, 0, sizeof(double) * nout);
+  const std::vector<int64_t> ord = ref_order(b, nb, box, partitions);
+  const float size_node = box / (float)partitions;
+  // Node boundaries in the ordered array.
+  std::vector<int64_t> bounds;
+  for (int64_t q = 0; q < nb; ++q)
+    if (q == 0 || ref_node_of(b[ord[q]], size_node, partitions) !=
+                      ref_node_of(b[ord[q - 1]], size_node, partitions))
+      bounds.push_back(q);
+  bounds.push_back(nb);
+  const int ncomb = bpc ? 8 : 1;
+  for (int64_t i = 0; i < na; ++i)
+    for (int64_t j = 0; j < na; ++j)
+      for (int f2 = 0; f2 < ncomb; ++f2) {
+        const float w12 = a[i].w * a[j].w;  // FP32 product, as the reference forms it
+        for (size_t s = 0; s + 1 < bounds.size(); ++s)
+          for (int f3 = 0; f3 < ncomb; ++f3) {
+            double v = (double)w12;
+            for (int64_t q = bounds[s]; q < bounds[s + 1]; ++q) {
+              const Pt& p3 = b[ord[q]];
+              const int64_t bin = triplet_bin(a[i], a[j], p3, f2, f3, r);
+              if (bin < 0) continue;
+              v *= (double)p3.w;
+              counts[bin] += 1;
+              wsum[bin] += v;
+            }
+          }
+      }
+  return 0;
+}
+
+// Weights the reference's XY2ani reads for the random points
+// (cuda/kernels/2ani_k.cuh:128: v = w1 * elementsD[j].w, j an index of the
+// random catalog's ordered array): out[j] = weight of the data point at the
+// random point j's position in the data catalog's ordered array, or NaN where
+// that position lies past the data catalog (the reference reads out of
+// bounds there).
+void pcf_oracle_xy2ani_alias_weights(const float* d_, int64_t nd, const float* r_, int64_t nr,
+                                     float box, int partitions, float* out) {
+  const Pt* d = reinterpret_cast<const Pt*>(d_);
+  const Pt* rp = reinterpret_cast<const Pt*>(r_);
+  const std::vector<int64_t> od = ref_order(d, nd, box, partitions);
+  const std::vector<int64_t> orr = ref_order(rp, nr, box, partitions);
+  for (int64_t q = 0; q < nr; ++q)
+    out[orr[q]] = q < nd ? d[od[q]].w : std::numeric_limits<float>::quiet_NaN();
 }
 
 }  // extern "C"

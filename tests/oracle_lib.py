@@ -28,6 +28,19 @@ def _configure(L):
                                         _i32, _f64p]
     L.pcf_oracle_analytic_rr.argtypes = [_i32, _i64, ctypes.c_float, ctypes.c_float, _f64p]
     L.pcf_oracle_analytic_rr.restype = None
+    if hasattr(L, "pcf_oracle_analytic_rr_refbugs"):
+        L.pcf_oracle_analytic_rr_refbugs.argtypes = [_i32, _i64, ctypes.c_float, ctypes.c_float,
+                                                     _f64p]
+        L.pcf_oracle_analytic_rr_refbugs.restype = None
+        L.pcf_oracle_analytic_3iso_refbugs.argtypes = [_f32p, _i64, _i32, ctypes.c_float,
+                                                       ctypes.c_float, _i64, _i32, _i32, _i32,
+                                                       _f64p, _f64p, _f64p, _f64p]
+        L.pcf_oracle_triplets_refbugs.argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32, _i32,
+                                                  ctypes.c_float, _i32, _i32, ctypes.c_float,
+                                                  _u64p, _f64p]
+        L.pcf_oracle_xy2ani_alias_weights.argtypes = [_f32p, _i64, _f32p, _i64, ctypes.c_float,
+                                                      _i32, _f32p]
+        L.pcf_oracle_xy2ani_alias_weights.restype = None
     L.pcf_oracle_cpp32_3iso.argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32, ctypes.c_float,
                                         _i32, _f64p]
     L.pcf_oracle_analytic_3iso.argtypes = [_f32p, _i64, _i32, ctypes.c_float, ctypes.c_float,
@@ -114,25 +127,28 @@ def cpp32_3iso(a, b=None, *, bins=30, dmax=50.0, threads=0):
     return h
 
 
-def analytic_rr(bins, np_, dmax, box):
+def analytic_rr(bins, np_, dmax, box, ref_bugs=False):
+    """Analytic periodic RR; ref_bugs: the reference's 32-bit int overflows."""
     h = np.zeros(bins, np.float64)
-    lib().pcf_oracle_analytic_rr(bins, np_, dmax, box, _p(h, _f64p))
+    f = lib().pcf_oracle_analytic_rr_refbugs if ref_bugs else lib().pcf_oracle_analytic_rr
+    f(bins, np_, dmax, box, _p(h, _f64p))
     return h
 
 
-def analytic_3iso(a, *, bins, dmax, box, np_=0, ptt=100, bn_ref=200, threads=0, cubes=True):
+def analytic_3iso(a, *, bins, dmax, box, np_=0, ptt=100, bn_ref=200, threads=0, cubes=True,
+                  ref_bugs=False):
     """Analytic periodic 3iso: (DDR, RRR, ff_av, ff_av_ref); cubes=False stops
-    after ff_av / ff_av_ref (DDR and RRR are then None)."""
+    after ff_av / ff_av_ref (DDR and RRR are then None); ref_bugs: the
+    reference's 32-bit int overflows in the fine RR."""
     a = _cat(a)
     ddr = np.zeros(bins ** 3, np.float64) if cubes else None
     rrr = np.zeros(bins ** 3, np.float64) if cubes else None
     ff = np.zeros(bins, np.float64)
     ffr = np.zeros(bins * bn_ref, np.float64)
-    rc = lib().pcf_oracle_analytic_3iso(_p(a, _f32p), len(a), bins, dmax, box,
-                                        int(np_ if np_ > 0 else len(a)), ptt, bn_ref, threads,
-                                        _p(ddr, _f64p) if cubes else None,
-                                        _p(rrr, _f64p) if cubes else None, _p(ff, _f64p),
-                                        _p(ffr, _f64p))
+    f = lib().pcf_oracle_analytic_3iso_refbugs if ref_bugs else lib().pcf_oracle_analytic_3iso
+    rc = f(_p(a, _f32p), len(a), bins, dmax, box, int(np_ if np_ > 0 else len(a)), ptt, bn_ref,
+           threads, _p(ddr, _f64p) if cubes else None, _p(rrr, _f64p) if cubes else None,
+           _p(ff, _f64p), _p(ffr, _f64p))
     if rc != 0:
         raise ValueError(f"oracle analytic 3iso rc={rc}")
     return ddr, rrr, ff, ffr
@@ -161,6 +177,32 @@ def triplets(a, b=None, *, ani=False, bpc=False, box=0.0, bins=30, dmax=80.0, th
     if rc != 0:
         raise ValueError(f"oracle triplets rc={rc}")
     return c, w
+
+
+def triplets_refbugs(a, b=None, *, ani=False, bpc=False, box, partitions=35, bins, dmax):
+    """Triplet histogram with the reference GPU's cumulative triplet weight
+    (literal form, small inputs)."""
+    a = _cat(a)
+    auto = b is None
+    b = a if auto else _cat(b)
+    n = bins ** 5 if ani else bins ** 3
+    c = np.zeros(n, np.uint64)
+    w = np.zeros(n, np.float64)
+    rc = lib().pcf_oracle_triplets_refbugs(_p(a, _f32p), len(a), _p(b, _f32p), len(b), int(auto),
+                                           int(ani), int(bpc), box, partitions, bins, dmax,
+                                           _p(c, _u64p), _p(w, _f64p))
+    if rc != 0:
+        raise ValueError(f"oracle triplets_refbugs rc={rc}")
+    return c, w
+
+
+def xy2ani_alias_weights(d, r, *, box, partitions=35):
+    """The data weights the reference's XY2ani reads for each random point."""
+    d, r = _cat(d), _cat(r)
+    out = np.zeros(len(r), np.float32)
+    lib().pcf_oracle_xy2ani_alias_weights(_p(d, _f32p), len(d), _p(r, _f32p), len(r), box,
+                                          partitions, _p(out, _f32p))
+    return out
 
 
 def estimator(hists, n_data=0.0, n_rand=0.0, normalize=False):

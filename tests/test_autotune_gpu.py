@@ -120,48 +120,48 @@ def test_autotune_rejects_triplets(engine):
         engine.release(cd)
 
 
-def test_tuning_cache_file(engine, tmp_path):
+def test_tuning_cache_file(tmp_path):
     """pcf_tune_save / pcf_tune_load / pcf_tune_lookup: a tuning written by
     one context is found by another after loading the file, and the histogram
     under the loaded tuning has the tuned run's counts; malformed files and
-    files from another device model are refused."""
+    files from another device model are refused.  (Fresh contexts: a
+    context's cache holds every workload it tuned.)"""
     d = synth.uniform(3000, 250.0, seed=151)
-    cd = engine.upload(d)
     path = str(tmp_path / "tune.cache")
-    try:
+    with da.Engine(0) as e1:
+        cd = e1.upload(d)
         job = da.make_job(da.PCF_2ISO, da.PCF_XX, bins=20, dmax=40.0, cat_x=cd)
-        t = engine.autotune(job)
-        assert engine.tune_lookup(job) == t
-        c_ref, _ = engine.histogram(job)
-        engine.tune_save(path)
-        lines = open(path).read().splitlines()
-        assert lines[0] == "pcf-tuning-cache 1" and lines[1].startswith("device ")
-        assert len(lines) == 3 and "\t" in lines[2]
-        with da.Engine(0) as e2:
-            cd2 = e2.upload(d)
-            job2 = da.make_job(da.PCF_2ISO, da.PCF_XX, bins=20, dmax=40.0, cat_x=cd2)
-            assert e2.tune_lookup(job2) is None
-            e2.tune_load(path)
-            got = e2.tune_lookup(job2)
-            assert got is not None
-            for k in ("cells_per_rmax", "lds_copies_log2", "blocks_per_cu", "pair_kernel",
-                      "trials"):
-                assert got[k] == t[k], k
-            c2, _ = e2.histogram(job2)
-            assert np.array_equal(c2, c_ref)
-            assert e2.stats()["cells_per_rmax"] == t["cells_per_rmax"]
-            # Another job's workload is not matched by the entry.
-            other = da.make_job(da.PCF_2ISO, da.PCF_XX, bins=21, dmax=40.0, cat_x=cd2)
-            assert e2.tune_lookup(other) is None
-            bad = tmp_path / "bad.cache"
-            bad.write_text(lines[0] + "\n" + lines[1] + "\nkey-without-tab 1 2 3\n")
-            with pytest.raises(da.PcfError):
-                e2.tune_load(str(bad))
-            alien = tmp_path / "alien.cache"
-            alien.write_text(lines[0] + "\ndevice gfx000 1\n" + lines[2] + "\n")
-            with pytest.raises(da.PcfError):
-                e2.tune_load(str(alien))
-            with pytest.raises(da.PcfError):
-                e2.tune_load(str(tmp_path / "missing.cache"))
-    finally:
-        engine.release(cd)
+        assert e1.tune_lookup(job) is None
+        t = e1.autotune(job)
+        assert e1.tune_lookup(job) == t
+        c_ref, _ = e1.histogram(job)
+        e1.tune_save(path)
+    lines = open(path).read().splitlines()
+    assert lines[0] == "pcf-tuning-cache 1" and lines[1].startswith("device ")
+    assert len(lines) == 3 and "\t" in lines[2]
+    with da.Engine(0) as e2:
+        cd2 = e2.upload(d)
+        job2 = da.make_job(da.PCF_2ISO, da.PCF_XX, bins=20, dmax=40.0, cat_x=cd2)
+        assert e2.tune_lookup(job2) is None
+        e2.tune_load(path)
+        got = e2.tune_lookup(job2)
+        assert got is not None
+        for k in ("cells_per_rmax", "lds_copies_log2", "blocks_per_cu", "pair_kernel",
+                  "trials"):
+            assert got[k] == t[k], k
+        c2, _ = e2.histogram(job2)
+        assert np.array_equal(c2, c_ref)
+        assert e2.stats()["cells_per_rmax"] == t["cells_per_rmax"]
+        # Another job's workload is not matched by the entry.
+        other = da.make_job(da.PCF_2ISO, da.PCF_XX, bins=21, dmax=40.0, cat_x=cd2)
+        assert e2.tune_lookup(other) is None
+        bad = tmp_path / "bad.cache"
+        bad.write_text(lines[0] + "\n" + lines[1] + "\nkey-without-tab 1 2 3\n")
+        with pytest.raises(da.PcfError):
+            e2.tune_load(str(bad))
+        alien = tmp_path / "alien.cache"
+        alien.write_text(lines[0] + "\ndevice gfx000 1\n" + lines[2] + "\n")
+        with pytest.raises(da.PcfError):
+            e2.tune_load(str(alien))
+        with pytest.raises(da.PcfError):
+            e2.tune_load(str(tmp_path / "missing.cache"))
