@@ -4,9 +4,9 @@
 // CPU oracle (oracle/pcf_oracle.cpp, compiled into this program) over small
 // seeded catalogs in all modes -- auto and cross, periodic and open, 2*dmax
 // above the box where the entry accepts it, empty and one-point catalogs,
-// duplicate points -- and the command line's catalog reader and histogram
-// writers (desi_amd/cli/pcf_io.h) on files it writes itself, including
-// malformed and short catalogs.  The results are not checked here (the
+// duplicate points; the reference-defect reproductions -- and the command
+// line's catalog reader and histogram writers (desi_amd/cli/pcf_io.h) on
+// files it writes itself, including malformed and short catalogs.  The results are not checked here (the
 // oracle's own tests do that); the program exists so that the sanitizers see
 // every code path run.
 //   sanitize_check <scratch dir>
@@ -36,6 +36,13 @@ int pcf_oracle_cpp32_3iso(const float*, int64_t, const float*, int64_t, int, int
                           double*);
 int pcf_oracle_analytic_3iso(const float*, int64_t, int, float, float, int64_t, int, int, int,
                              double*, double*, double*, double*);
+void pcf_oracle_analytic_rr_refbugs(int, int64_t, float, float, double*);
+int pcf_oracle_analytic_3iso_refbugs(const float*, int64_t, int, float, float, int64_t, int, int,
+                                     int, double*, double*, double*, double*);
+int pcf_oracle_triplets_refbugs(const float*, int64_t, const float*, int64_t, int, int, int, float,
+                                int, int, float, uint64_t*, double*);
+void pcf_oracle_xy2ani_alias_weights(const float*, int64_t, const float*, int64_t, float, int,
+                                     float*);
 }
 
 namespace {
@@ -147,6 +154,41 @@ void triplet_cases() {
   }
 }
 
+// The reference-defect reproductions: the signed 32-bit wraps are formed
+// without signed overflow, the chain walk and the alias read stay in bounds
+// (a random catalog longer than the data catalog included).
+void refbugs_cases() {
+  std::vector<double> rr(26800);
+  pcf_oracle_analytic_rr_refbugs(30, 46341, 80.f, 250.f, rr.data());
+  pcf_oracle_analytic_rr_refbugs(30, 65536, 80.f, 250.f, rr.data());
+  pcf_oracle_analytic_rr_refbugs(26800, 1000, 80.f, 250.f, rr.data());
+  const float L = 60.f;
+  for (int64_t n : {0, 1, 3, 40}) {
+    std::vector<float> a = catalog(n, L, true), b = catalog(n + 5, L, true);
+    for (int ani = 0; ani < 2; ++ani)
+      for (int bpc = 0; bpc < 2; ++bpc)
+        for (int auto_ = 0; auto_ < 2; ++auto_)
+          for (int partitions : {1, 3, 35}) {
+            const int bins = ani ? 4 : 7;
+            std::vector<uint64_t> c((size_t)hsize(3, ani, bins));
+            std::vector<double> w(c.size());
+            expect(pcf_oracle_triplets_refbugs(a.data(), n, b.data(), n + 5, auto_, ani, bpc, L,
+                                               partitions, bins, 14.f, c.data(), w.data()) == 0,
+                   "pcf_oracle_triplets_refbugs");
+          }
+    std::vector<float> alias((size_t)n + 5);
+    pcf_oracle_xy2ani_alias_weights(a.data(), n, b.data(), n + 5, L, 4, alias.data());
+    pcf_oracle_xy2ani_alias_weights(b.data(), n + 5, a.data(), n, L, 4, alias.data());
+    if (n > 0) {
+      const int bins = 5, ptt = 4, bref = 6;
+      std::vector<double> ddr(bins * bins * bins), rrr(ddr.size()), ff(bins), ffr(bins * bref);
+      expect(pcf_oracle_analytic_3iso_refbugs(a.data(), n, bins, 14.f, L, 46341, ptt, bref, 2,
+                                              ddr.data(), rrr.data(), ff.data(), ffr.data()) == 0,
+             "pcf_oracle_analytic_3iso_refbugs");
+    }
+  }
+}
+
 void io_cases(const std::string& dir) {
   const std::string good = dir + "/cat.dat", bad = dir + "/bad.dat", shortf = dir + "/short.dat";
   {
@@ -187,6 +229,7 @@ int main(int argc, char** argv) {
   if (argc != 2) return 2;
   pairs_cases();
   triplet_cases();
+  refbugs_cases();
   io_cases(argv[1]);
   if (fails) return 1;
   std::printf("sanitize_check ok\n");
