@@ -172,10 +172,10 @@ __device__ __forceinline__ unsigned exact_row(const float4 pi, const float4 pj, 
   return (unsigned)(bz * bins + bo);
 }
 
-#ifndef PCF_GRAM_H
-#define PCF_GRAM_H 2
+#ifndef PCF_BATCH_CLAIM
+#define PCF_BATCH_CLAIM 4
 #endif
-constexpr int kGramH = PCF_GRAM_H;  // 2iso Gram path: i blocks per matrix-product group
+constexpr int kBatchClaim = PCF_BATCH_CLAIM;  // items a wave takes per claim in a batch chunk
 constexpr int kBlockMixed = kMixedBlock;
 constexpr int kXcds = 8;  // MI355X: 8 XCDs, each with its own L2
 constexpr int kMaxTable = kMixedTableMax;
@@ -280,7 +280,24 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
     const int code = __builtin_amdgcn_readfirstlane(items[it].code);
     const int img = code & 31;
     const bool masked = (code & kCodeMask) != 0;
-    evals += (unsigned long long)ni * (unsigned long long)nj;
+    // The j range any of the tile's blocks can reach (the union of the step
+    // windows, see PairItem): the rest of the run is out of range.
+    int jlo = j0, jhi = j0 + nj;
+    if (!(code & kCodeNoWin)) {
+      const unsigned ws = (unsigned)__builtin_amdgcn_readfirstlane((int)items[it].ws);
+      const unsigned we = (unsigned)__builtin_amdgcn_readfirstlane((int)items[it].we);
+      int t_first = 0x7fffffff, t_last = 0;
+      for (int b = 0; b < 4 && 16 * b < ni; ++b) {
+        const int s0 = (int)((ws >> (8 * b)) & 0xffu), e0 = (int)((we >> (8 * b)) & 0xffu);
+        if (e0 > s0) {
+          t_first = min(t_first, s0);
+          t_last = max(t_last, e0);
+        }
+      }
+      jlo = t_first < t_last ? j0 + 16 * t_first : j0;
+      jhi = t_first < t_last ? min(j0 + nj, j0 + 16 * t_last) : j0;
+    }
+    evals += (unsigned long long)ni * (unsigned long long)(jhi - jlo);
 
     float4 ps, pi;
     if (lane < ni) {
@@ -347,8 +364,8 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
       deposit(r3, q3.w);
     };
 
-    int j = j0;
-    const int jend = j0 + nj;
+    int j = jlo;
+    const int jend = jhi;
     for (; j + 8 <= jend; j += 8) {
       const float4 q0 = yp[j], q1 = yp[j + 1], q2 = yp[j + 2], q3 = yp[j + 3];
       const float4 q4 = yp[j + 4], q5 = yp[j + 5], q6 = yp[j + 6], q7 = yp[j + 7];
@@ -364,20 +381,24 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
     }
   };
 
-  // Gram-form item on the matrix cores (MF = 1).  Returns false, having done
-  // nothing, when the tile's bounding box is larger than the error bound of
-  // the table covers; the caller then runs the direct path.
-  auto run_item_mf = [&](const int it, const float4* __restrict__ xp,
-                         const float4* __restrict__ yp, const float4* __restrict__ xp_orig,
-                         const float4* __restrict__ yp_orig, const int zero) -> bool {
-    typedef float f4v __attribute__((ext_vector_type(4)));
-    const int i0 = __builtin_amdgcn_readfirstlane(items[it].i0);
-    const int ni = __builtin_amdgcn_readfirstlane(items[it].ni);
-    const int j0 = __builtin_amdgcn_readfirstlane(items[it].j0);
-    const int nj = __builtin_amdgcn_readfirstlane(items[it].nj);
-    const int code = __builtin_amdgcn_readfirstlane(items[it].code);
-    const int img = code & 31;
-    const bool masked = (code & kCodeMask) != 0;
+  // Gram path: the i side of the wave's current tile -- its origin, whether
+  // its box fits the table's error bound, and the B operands and C columns of
+  // its four blocks -- kept from one item to the next.  The work list emits a
+  // tile's neighbour columns one after another and a wave takes runs of
+  // consecutive items, so most items find their tile here and skip the tile
+  // box, the i loads and the operand set-up.  Keyed by (i0, ni, image);
+  // t_i0 = -1 when the point arrays change (a new chunk of a batch).
+  int t_i0 = -1, t_ni = 0, t_img = -1;
+  bool t_ok = false;
+  float t_ox = 0.f, t_oy = 0.f, t_oz = 0.f;
+  float t_b[4], t_c[4];  // 2iso: B, C; 2ani: perpendicular B, line-of-sight B
+#pragma unroll
+  for (int b = 0; b < 4; ++b) t_b[b] = t_c[b] = 0.f;
+
+  // The i side of a tile (see t_i0 above): false when its box is larger than
+  // the table's error bound allows.
+  auto tile_setup = [&](const int i0, const int ni, const int img,
+                        const float4* __restrict__ xp) -> bool {
     // Image offset on the i side, the same FP32 operation as the direct path.
     const float shx = (float)(img / 9 - 1) * k.L;
     const float shy = (float)((img / 3) % 3 - 1) * k.L;
@@ -413,7 +434,73 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
       oy = 0.5f * (loy + hiy);
       oz = 0.5f * (loz + hiz);
     }
-    evals += (unsigned long long)ni * (unsigned long long)nj;
+    t_ox = ox;
+    t_oy = oy;
+    t_oz = oz;
+    if constexpr (!ANI) {
+      // i side: B operand (component grp of point i = 16b + col, times 2;
+      // grp 3: -1) and the C column Tf - |x_i|^2, so the product is
+      // Tf - d2, the flipped table index (beyond the table: negative, cell
+      // 0).  Lanes past the tile get a hugely negative C (always out).
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int ir = 16 * b + col;
+        const float4 p = gld(xp, i0 + min(ir, ni - 1));
+        const float x = (p.x - shx) - ox, y = (p.y - shy) - oy, z = (p.z - shz) - oz;
+        // Exact select by 0/1 coefficients (products by 0 and 1 are exact).
+        t_b[b] = __builtin_fmaf(sel_x, 2.0f * x,
+                                __builtin_fmaf(sel_y, 2.0f * y, __builtin_fmaf(sel_z, 2.0f * z, -sel_w)));
+        t_c[b] = ir < ni ? k.tflip - __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x)) : -1e30f;
+      }
+    } else {
+      // i side, 2ani: two products per block.  Perpendicular: B operand
+      // (-2x, -2y, 1, |x_perp|^2) of i against A (x, y, |x_perp|^2, 1) of j;
+      // line of sight: B (-2z, 1, z^2, 0) against A (z, z^2, 1, 0).  C = 0 for
+      // both (the i side's squared norm rides in the K dimension).
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int ir = 16 * b + col;
+        const float4 p = gld(xp, i0 + min(ir, ni - 1));
+        const float x = (p.x - shx) - ox, y = (p.y - shy) - oy, z = (p.z - shz) - oz;
+        // A lane past the tile gets huge squared norms in both products, so
+        // neither component can read as ambiguous (its pair does not exist).
+        const bool iv = ir < ni;
+        const float np2 = iv ? __builtin_fmaf(y, y, x * x) : 1e30f;
+        const float nz2 = iv ? z * z : 1e30f;
+        t_b[b] = __builtin_fmaf(sel_x, -2.0f * x,
+                                 __builtin_fmaf(sel_y, -2.0f * y, __builtin_fmaf(sel_w, np2, sel_z)));
+        t_c[b] = __builtin_fmaf(sel_x, -2.0f * z, __builtin_fmaf(sel_z, nz2, sel_y));
+      }
+    }
+    return true;
+  };
+
+  // Gram-form item on the matrix cores (MF = 1).  Returns false, having done
+  // nothing, when the tile's bounding box is larger than the error bound of
+  // the table covers; the caller then runs the direct path.
+  auto run_item_mf = [&](const int it, const float4* __restrict__ xp,
+                         const float4* __restrict__ yp, const float4* __restrict__ xp_orig,
+                         const float4* __restrict__ yp_orig, const int zero) -> bool {
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    const int i0 = __builtin_amdgcn_readfirstlane(items[it].i0);
+    const int ni = __builtin_amdgcn_readfirstlane(items[it].ni);
+    const int j0 = __builtin_amdgcn_readfirstlane(items[it].j0);
+    const int nj = __builtin_amdgcn_readfirstlane(items[it].nj);
+    const int code = __builtin_amdgcn_readfirstlane(items[it].code);
+    const int img = code & 31;
+    const bool masked = (code & kCodeMask) != 0;
+    if (i0 != t_i0 || ni != t_ni || img != t_img) {
+      t_i0 = i0;
+      t_ni = ni;
+      t_img = img;
+      t_ok = tile_setup(i0, ni, img, xp);
+    }
+    if (!t_ok) return false;
+    const float ox = t_ox, oy = t_oy, oz = t_oz;
+    // Step windows of the four blocks (see PairItem).
+    const bool nowin = (code & kCodeNoWin) != 0;
+    const unsigned wsb = nowin ? 0u : (unsigned)__builtin_amdgcn_readfirstlane((int)items[it].ws);
+    const unsigned web = nowin ? 0xffffffffu : (unsigned)__builtin_amdgcn_readfirstlane((int)items[it].we);
 
     const int nb = (ni + 15) >> 4;
     const int jend = j0 + nj;
@@ -421,6 +508,49 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
     // start before the tile's end hold pairs with j <= i.  A uniform integer,
     // so the test per step is a scalar compare.
     const int mask_end = masked ? i0 + ni : j0;
+    // Block skip: block b takes only the steps of its window (the work list's
+    // z-cell reach rule applied to the block's own cells), so the loop runs
+    // from the first window's start to the last one's end, and a block
+    // outside its window is not evaluated (all its pairs there are out of
+    // range).  Windows are in steps of 16 j from j0; without windows every
+    // block takes every step.
+    auto windows = [&](unsigned* wlo, unsigned* wlen, int* jstart, int* jstop) {
+      const int nbk = nb;
+      int t_first = 0x7fffffff, t_last = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        if (b >= nbk) break;
+        const unsigned s0 = (wsb >> (8 * b)) & 0xffu;
+        const unsigned e0 = nowin ? 0xffffffu : ((web >> (8 * b)) & 0xffu);
+        wlo[b] = s0;
+        wlen[b] = e0 > s0 ? e0 - s0 : 0u;
+        if (e0 > s0) {
+          t_first = min(t_first, (int)s0);
+          t_last = max(t_last, (int)e0);
+        }
+      }
+      *jstart = t_first < t_last ? j0 + 16 * t_first : j0;
+      *jstop = t_first < t_last ? min(jend, j0 + 16 * t_last) : j0;
+    };
+    // Blocks run at step jb (bit b), and their pairs counted as evaluations
+    // (the last block may hold fewer than 16 points).
+    auto step_blocks = [&](int jb, const unsigned* wlo, const unsigned* wlen) -> unsigned {
+      const unsigned t = (unsigned)((jb - j0) >> 4);
+      unsigned act = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        if (b >= nb) break;
+        if (t - wlo[b] < wlen[b]) act |= 1u << b;
+      }
+      if (act) {
+        const unsigned njs = (unsigned)min(16, jend - jb);
+        const unsigned last = 1u << (nb - 1);
+        unsigned nip = 16u * (unsigned)__builtin_popcount(act & (last - 1u));
+        if (act & last) nip += (unsigned)(ni - 16 * (nb - 1));
+        evals += (unsigned long long)(njs * nip);
+      }
+      return act;
+    };
     auto fix2 = [&](unsigned& r, bool is_amb, int ii, int jj) {
       const unsigned long long amb = __ballot(is_amb);
       if (amb) {
@@ -437,85 +567,50 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
       }
     };
     if constexpr (!ANI) {
-      // i side: B operand (component grp of point i = 16b + col, times 2;
-      // grp 3: -1) and the C column Tf - |x_i|^2, so the product is
-      // Tf - d2, the flipped table index (beyond the table: negative, cell
-      // 0).  Lanes past the tile get a hugely negative C (always out).
-      float bop[4], cin[4];
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const int ir = 16 * b + col;
-        const float4 p = gld(xp, i0 + min(ir, ni - 1));
-        const float x = (p.x - shx) - ox, y = (p.y - shy) - oy, z = (p.z - shz) - oz;
-        // Exact select by 0/1 coefficients (products by 0 and 1 are exact).
-        bop[b] = __builtin_fmaf(sel_x, 2.0f * x,
-                                __builtin_fmaf(sel_y, 2.0f * y, __builtin_fmaf(sel_z, 2.0f * z, -sel_w)));
-        cin[b] = ir < ni ? k.tflip - __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x)) : -1e30f;
-      }
       // The j loop for a tile of NB i-blocks (NB*16 >= ni), 16 j per step.
       auto jloop = [&](auto nbc) {
         constexpr int NB = decltype(nbc)::value;
+        unsigned wlo[NB], wlen[NB];
+        int jstart, jstop;
+        windows(wlo, wlen, &jstart, &jstop);
         // The next step's j point is loaded one step ahead (its latency hides
         // behind this step's matrix, LDS and histogram work).
-        float4 qn4 = gld(yp, min(j0 + col, jend - 1));
-        for (int jb = j0; jb < jend; jb += 16) {
+        float4 qn4 = gld(yp, min(jstart + col, jend - 1));
+        for (int jb = jstart; jb < jstop; jb += 16) {
           // j side: A operand (component grp of point j = jb + col; grp 3:
           // |x_j|^2, huge past the run's end).
           const int jj = jb + col;
           const float4 q = qn4;
           qn4 = gld(yp, min(jj + 16, jend - 1));
-          const float x = q.x - ox, y = q.y - oy, z = q.z - oz;
+          const unsigned act = step_blocks(jb, wlo, wlen);
+          if (act == 0) continue;
+          const float z = q.z - oz;
+          const float x = q.x - ox, y = q.y - oy;
           const float bj = jj < jend ? __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x)) : 1e30f;
           const float aop =
               __builtin_fmaf(sel_x, x, __builtin_fmaf(sel_y, y, __builtin_fmaf(sel_z, z, sel_w * bj)));
-          // kGramH blocks at a time.
 #pragma unroll
-          for (int h = 0; h < NB; h += kGramH) {
-            constexpr int H = kGramH;
-            const int nh = (NB - h) < H ? (NB - h) : H;
-            f4v d[H];
+          for (int b = 0; b < NB; ++b) {
+            if (!(act & (1u << b))) continue;
+            const f4v cv = {t_c[b], t_c[b], t_c[b], t_c[b]};
+            const f4v d = __builtin_amdgcn_mfma_f32_16x16x4f32(aop, t_b[b], cv, 0, 0, 0);
+            unsigned r[4];
 #pragma unroll
-            for (int u = 0; u < H; ++u) {
-              if (u < nh) {
-                const int b = h + u;
-                const f4v cv = {cin[b], cin[b], cin[b], cin[b]};
-                d[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop, bop[b], cv, 0, 0, 0);
-              }
-            }
-            unsigned r[4 * H];
-#pragma unroll
-            for (int u = 0; u < H; ++u)
-#pragma unroll
-              for (int v = 0; v < 4; ++v)
-                r[4 * u + v] = u < nh ? tab[__float2uint_rz(d[u][v])] : 0u;
-            // Element (b, v) of this lane: i = i0 + 16b + col, j = jb + 4*grp + v.
+            for (int v = 0; v < 4; ++v) r[v] = tab[__float2uint_rz(d[v])];
+            // Element v of this lane: i = i0 + 16b + col, j = jb + 4*grp + v.
             if (jb < mask_end) {
 #pragma unroll
-              for (int u = 0; u < H; ++u)
-#pragma unroll
-                for (int v = 0; v < 4; ++v)
-                  r[4 * u + v] =
-                      (jb + 4 * grp + v <= i0 + 16 * (h + u) + col) ? out_row : r[4 * u + v];
+              for (int v = 0; v < 4; ++v)
+                r[v] = (jb + 4 * grp + v <= i0 + 16 * b + col) ? out_row : r[v];
             }
-            unsigned m = r[0];
-#pragma unroll
-            for (int e = 1; e < 4 * H; ++e) m = max(m, r[e]);
+            const unsigned m = max(max(r[0], r[1]), max(r[2], r[3]));
             if (__builtin_expect(__ballot(m == amb_row) != 0, 0)) {
 #pragma unroll
-              for (int u = 0; u < H; ++u)
-                if (u < nh) {
-#pragma unroll
-                  for (int v = 0; v < 4; ++v)
-                    fix2(r[4 * u + v], r[4 * u + v] == amb_row, i0 + 16 * (h + u) + col,
-                         jb + 4 * grp + v);
-                }
+              for (int v = 0; v < 4; ++v)
+                fix2(r[v], r[v] == amb_row, i0 + 16 * b + col, jb + 4 * grp + v);
             }
 #pragma unroll
-            for (int u = 0; u < H; ++u)
-              if (u < nh) {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) atomicAdd(hc_lane + (r[4 * u + v] << CS), 1u);
-              }
+            for (int v = 0; v < 4; ++v) atomicAdd(hc_lane + (r[v] << CS), 1u);
           }
         }
       };
@@ -526,36 +621,22 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
         default: jloop(std::integral_constant<int, 1>{}); break;
       }
     } else {
-      // i side, 2ani: two products per block.  Perpendicular: B operand
-      // (-2x, -2y, 1, |x_perp|^2) of i against A (x, y, |x_perp|^2, 1) of j;
-      // line of sight: B (-2z, 1, z^2, 0) against A (z, z^2, 1, 0).  C = 0 for
-      // both (the i side's squared norm rides in the K dimension).
-      float bopp[4], bopz[4];
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const int ir = 16 * b + col;
-        const float4 p = gld(xp, i0 + min(ir, ni - 1));
-        const float x = (p.x - shx) - ox, y = (p.y - shy) - oy, z = (p.z - shz) - oz;
-        // A lane past the tile gets huge squared norms in both products, so
-        // neither component can read as ambiguous (its pair does not exist).
-        const bool iv = ir < ni;
-        const float np2 = iv ? __builtin_fmaf(y, y, x * x) : 1e30f;
-        const float nz2 = iv ? z * z : 1e30f;
-        bopp[b] = __builtin_fmaf(sel_x, -2.0f * x,
-                                 __builtin_fmaf(sel_y, -2.0f * y, __builtin_fmaf(sel_w, np2, sel_z)));
-        bopz[b] = __builtin_fmaf(sel_x, -2.0f * z, __builtin_fmaf(sel_z, nz2, sel_y));
-      }
       // The j loop for a tile of NB i-blocks (NB*16 >= ni), 16 j per step.
       auto jloop = [&](auto nbc) {
         constexpr int NB = decltype(nbc)::value;
         const f4v zero4 = {0.f, 0.f, 0.f, 0.f};
-        float4 qn4 = gld(yp, min(j0 + col, jend - 1));
-        for (int jb = j0; jb < jend; jb += 16) {
+        unsigned wlo[NB], wlen[NB];
+        int jstart, jstop;
+        windows(wlo, wlen, &jstart, &jstop);
+        float4 qn4 = gld(yp, min(jstart + col, jend - 1));
+        for (int jb = jstart; jb < jstop; jb += 16) {
           // j side: A operands (component grp of point j = jb + col; a j past
           // the run's end gets huge squared norms in both).
           const int jj = jb + col;
           const float4 q = qn4;
           qn4 = gld(yp, min(jj + 16, jend - 1));
+          const unsigned act = step_blocks(jb, wlo, wlen);
+          if (act == 0) continue;
           const float x = q.x - ox, y = q.y - oy, z = q.z - oz;
           const bool jv = jj < jend;
           const float np2 = jv ? __builtin_fmaf(y, y, x * x) : 1e30f;
@@ -564,67 +645,42 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
               __builtin_fmaf(sel_x, x, __builtin_fmaf(sel_y, y, __builtin_fmaf(sel_z, np2, sel_w)));
           const float aopz = __builtin_fmaf(sel_x, z, __builtin_fmaf(sel_y, nz2, sel_z));
 #pragma unroll
-          for (int h = 0; h < NB; h += 2) {
-            constexpr int H = 2;
-            const int nh = (NB - h) < H ? (NB - h) : H;
-            f4v dp[H], dz[H];
-#pragma unroll
-            for (int u = 0; u < H; ++u) {
-              if (u < nh) {
-                dp[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(aopp, bopp[h + u], zero4, 0, 0, 0);
-                dz[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(aopz, bopz[h + u], zero4, 0, 0, 0);
-              }
-            }
+          for (int b = 0; b < NB; ++b) {
+            if (!(act & (1u << b))) continue;
+            const f4v dp = __builtin_amdgcn_mfma_f32_16x16x4f32(aopp, t_b[b], zero4, 0, 0, 0);
+            const f4v dz = __builtin_amdgcn_mfma_f32_16x16x4f32(aopz, t_c[b], zero4, 0, 0, 0);
             // Per element: line-of-sight and perpendicular codes (bin,
             // ambiguous = 254 or out = 255); the row is cz * bins + co when
             // both are bins, and their max is out when either is out.
-            unsigned r[4 * H], mc[4 * H];
+            unsigned r[4], mc[4];
 #pragma unroll
-            for (int u = 0; u < H; ++u)
-#pragma unroll
-              for (int v = 0; v < 4; ++v) {
-                if (u < nh) {
-                  const unsigned cz = tab[min(__float2uint_rz(dz[u][v]), ntm1)];
-                  const unsigned co = tab[min(__float2uint_rz(dp[u][v]), ntm1)];
-                  // The row; it counts only when both codes are bins (mc < bins).
-                  mc[4 * u + v] = max(cz, co);
-                  r[4 * u + v] = __umul24(cz, bins) + co;
-                } else {
-                  mc[4 * u + v] = kAniOut8;
-                  r[4 * u + v] = out_row;
-                }
-              }
+            for (int v = 0; v < 4; ++v) {
+              const unsigned cz = tab[min(__float2uint_rz(dz[v]), ntm1)];
+              const unsigned co = tab[min(__float2uint_rz(dp[v]), ntm1)];
+              // The row; it counts only when both codes are bins (mc < bins).
+              mc[v] = max(cz, co);
+              r[v] = __umul24(cz, bins) + co;
+            }
             if (jb < mask_end) {
 #pragma unroll
-              for (int u = 0; u < H; ++u)
-#pragma unroll
-                for (int v = 0; v < 4; ++v)
-                  if (jb + 4 * grp + v <= i0 + 16 * (h + u) + col) mc[4 * u + v] = kAniOut8;
+              for (int v = 0; v < 4; ++v)
+                if (jb + 4 * grp + v <= i0 + 16 * b + col) mc[v] = kAniOut8;
             }
             bool any_amb = false;
 #pragma unroll
-            for (int e = 0; e < 4 * H; ++e) any_amb = any_amb || mc[e] == kAniAmb8;
+            for (int v = 0; v < 4; ++v) any_amb = any_amb || mc[v] == kAniAmb8;
             if (__builtin_expect(__ballot(any_amb) != 0, 0)) {
 #pragma unroll
-              for (int u = 0; u < H; ++u)
-                if (u < nh) {
-#pragma unroll
-                  for (int v = 0; v < 4; ++v)
-                    fix2(r[4 * u + v], mc[4 * u + v] == kAniAmb8, i0 + 16 * (h + u) + col,
-                         jb + 4 * grp + v);
-                }
+              for (int v = 0; v < 4; ++v)
+                fix2(r[v], mc[v] == kAniAmb8, i0 + 16 * b + col, jb + 4 * grp + v);
             }
             // Most 2ani candidates are out of range, and their deposits would
             // all land on one row per LDS copy (8 lanes per address): only
             // pairs with both codes bins deposit (ambiguous ones go through
             // the queue; excluded self pairs read as out).
 #pragma unroll
-            for (int u = 0; u < H; ++u)
-              if (u < nh) {
-#pragma unroll
-                for (int v = 0; v < 4; ++v)
-                  if (mc[4 * u + v] < bins) atomicAdd(hc_lane + (r[4 * u + v] << CS), 1u);
-              }
+            for (int v = 0; v < 4; ++v)
+              if (mc[v] < bins) atomicAdd(hc_lane + (r[v] << CS), 1u);
           }
         }
       };
@@ -730,16 +786,22 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
       out_row = (unsigned)k.nflat;
       amb_row = (unsigned)k.nflat + 1;
       bins = (unsigned)k.bins;
+      t_i0 = -1;  // another job's points (and table) may follow
       for (;;) {
-        int it = 0;
-        if (lane == 0) it = cb + atomicAdd(&s_next, 1);
-        it = __builtin_amdgcn_readfirstlane(it);
-        if (it >= ce) break;
-        if constexpr (MF) {
-          if (run_item_mf(it, jd.xs, jd.ys, jd.xo, jd.yo, jd.zero)) continue;
-          ++ndirect;
+        // Runs of kBatchClaim consecutive items per wave: consecutive items
+        // mostly share their tile (the Gram path then keeps its i side).
+        int it0 = 0;
+        if (lane == 0) it0 = cb + atomicAdd(&s_next, kBatchClaim);
+        it0 = __builtin_amdgcn_readfirstlane(it0);
+        if (it0 >= ce) break;
+        const int it1 = min(it0 + kBatchClaim, ce);
+        for (int it = it0; it < it1; ++it) {
+          if constexpr (MF) {
+            if (run_item_mf(it, jd.xs, jd.ys, jd.xo, jd.yo, jd.zero)) continue;
+            ++ndirect;
+          }
+          run_item(it, jd.xs, jd.ys, jd.xo, jd.yo, jd.zero);
         }
-        run_item(it, jd.xs, jd.ys, jd.xo, jd.yo, jd.zero);
       }
       if (qn > 0) drain(qn, jd.xo, jd.yo, jd.zero);
       __syncthreads();

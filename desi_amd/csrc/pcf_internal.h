@@ -51,10 +51,16 @@ struct GridGeom {
 // a batched launch.
 struct PairItem {
   int i0, ni, j0, nj, code;
+  // Step windows of the tile's four 16-point blocks (byte b of each word):
+  // block b can pair in range only with the j of steps [ws_b, we_b), a step
+  // being 16 consecutive j from j0.  kCodeNoWin in code: no windows (more
+  // than 255 steps; every block takes every step).
+  unsigned ws, we;
 };
 
 constexpr int kImgZero = 13;  // kx = ky = kz = 0
 constexpr int kCodeMask = 32;
+constexpr int kCodeNoWin = 64;
 
 // Exact re-evaluation rules, uploaded once per job (device copy in Job).
 struct BinRules {

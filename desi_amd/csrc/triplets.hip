@@ -414,8 +414,10 @@ struct ListCap {
 // form is part of the third-point table's ambiguity margin (every coordinate
 // is within reach of point 1).  A lane reads one list component, the rows'
 // b31 and w3 four at a time, per 16 entries, instead of three broadcast reads
-// per entry.
-template <int ANI, int WEIGHTED, int PLAIN, int MF>
+// per entry.  GLOB = 1 (3iso): the V rows go to global memory (one b12 row
+// exceeds the LDS budget), a separate instantiation so that the LDS kernels
+// carry no global-atomic code (its registers would spill them).
+template <int ANI, int WEIGHTED, int PLAIN, int MF, int GLOB>
 __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
     const float4* __restrict__ xs, const float4* __restrict__ xo,
     const float4* __restrict__ ks_all, const float4* __restrict__ ko_all,
@@ -430,7 +432,7 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
   __shared__ double sT[kMaxTripBins + 2];
   const int bins = a.bins;
   const int bins2 = bins * bins;
-  const int nrows = (ANI || a.glob) ? 0 : (a.hi - a.lo) * bins2;
+  const int nrows = (ANI || GLOB) ? 0 : (a.hi - a.lo) * bins2;
   uint32_t* hc = reinterpret_cast<uint32_t*>(smem);
   const size_t hc_bytes = ((size_t)nrows * 4 + 15) & ~(size_t)15;
   double* hw = reinterpret_cast<double*>(smem + hc_bytes);
@@ -461,9 +463,9 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
   unsigned long long evals = 0, nre = 0;
   // 3iso deposit of V row vrow = (b12 * bins + b23) * bins + b31: into the
   // LDS slice [lo, hi) of b12 rows, or (glob) straight into global memory.
-  const unsigned vrow0 = a.glob ? 0u : (unsigned)a.lo * (unsigned)bins2;
+  const unsigned vrow0 = GLOB ? 0u : (unsigned)a.lo * (unsigned)bins2;
   auto deposit_v = [&](unsigned vrow, double w) {
-    if (a.glob) {
+    if constexpr (GLOB) {
       atomicAdd(&hist[vrow], 1ull);
       if (WEIGHTED) atomicAdd(&whist[vrow], w);
     } else {
@@ -1539,19 +1541,25 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
             ((((size_t)nrows * 4 + 15) / 16 * 16 + (weighted ? (size_t)nrows * 8 : 0) + 15) &
              ~(size_t)15) + lists;
         HIPCHK(c, launch_zero(d_counter, sizeof(int), c->stream));
-#define PCF_TRIP_ISO(W, P, M)                                                                  \
+#define PCF_TRIP_ISO1(W, P, M, G)                                                              \
   do {                                                                                         \
-    HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_triplets<0, W, P, M>),       \
+    HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_triplets<0, W, P, M, G>),    \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));      \
-    hipLaunchKernelGGL((k_triplets<0, W, P, M>), dim3(c->ncu), dim3(kTripBlock), lds,          \
+    hipLaunchKernelGGL((k_triplets<0, W, P, M, G>), dim3(c->ncu), dim3(kTripBlock), lds,       \
                        c->stream, xs, gx->pts, ksc, gk->pts, gk->cell_start, d_pairs, d_off,   \
                        d_counter, d_ktab, d_T, d_V, d_VW, d_stats, t);                         \
+  } while (0)
+#define PCF_TRIP_ISO(W, P, M)               \
+  do {                                      \
+    if (glob) PCF_TRIP_ISO1(W, P, M, 1);    \
+    else PCF_TRIP_ISO1(W, P, M, 0);         \
   } while (0)
         if (weighted && plain) PCF_TRIP_ISO(1, 1, 0);
         else if (weighted) PCF_TRIP_ISO(1, 0, 1);
         else if (plain) PCF_TRIP_ISO(0, 1, 0);
         else PCF_TRIP_ISO(0, 0, 1);
 #undef PCF_TRIP_ISO
+#undef PCF_TRIP_ISO1
         HIPCHK(c, hipGetLastError());
       }
     } else if (ani_lds) {
@@ -1671,7 +1679,7 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
       const int blocks = c->ncu * 4;
       const size_t lds = (size_t)(kAniTripBlock / kWave) * ListCap<1>::value * 24;
 #define PCF_TRIP_ANI(W, P)                                                                     \
-  hipLaunchKernelGGL((k_triplets<1, W, P, 0>), dim3(blocks), dim3(kAniTripBlock), lds, c->stream, \
+  hipLaunchKernelGGL((k_triplets<1, W, P, 0, 0>), dim3(blocks), dim3(kAniTripBlock), lds, c->stream, \
                      xs, gx->pts, ksc, gk->pts, gk->cell_start, d_pairs, d_off, d_counter,     \
                      d_ktab, d_T, d_V, d_VW, d_stats, t)
       if (weighted && plain) PCF_TRIP_ANI(1, 1);

@@ -49,6 +49,25 @@ __device__ inline int enumerate_cell(const WorkListArgs& a, int cell, int s, int
     // z-cell of the tile's last point.
     int zb = cz;
     while (zb + 1 < Pz && a.x_cell_start[colbase + zb + 1] <= i0 + ni - 1) ++zb;
+    // z cells of the first and last point of each 16-point block (-1: no
+    // such block), for the items' step windows.
+    int bcf[4], bcl[4];
+    if (FILL) {
+      int c = cz;
+      for (int b = 0; b < 4; ++b) {
+        const int f = i0 + 16 * b;
+        if (f >= i0 + ni) {
+          bcf[b] = bcl[b] = -1;
+          continue;
+        }
+        const int l = min(f + 15, i0 + ni - 1);
+        while (c + 1 < Pz && a.x_cell_start[colbase + c + 1] <= f) ++c;
+        int e = c;
+        while (e + 1 < Pz && a.x_cell_start[colbase + e + 1] <= l) ++e;
+        bcf[b] = c;
+        bcl[b] = e;
+      }
+    }
     for (int dx = dxl; dx <= dxh; ++dx) {
       int ccx = cx + dx, kx = 0;
       if (ccx < 0) {
@@ -117,6 +136,33 @@ __device__ inline int enumerate_cell(const WorkListArgs& a, int cell, int s, int
             it.j0 = j0;
             it.nj = j1 - j0;
             it.code = ((kx + 1) * 9 + (ky + 1) * 3 + (seg_k[s] + 1)) | flag;
+            // Step windows: the z cells within Kz of block b's cells (the
+            // tile's own rule, per block), in this segment's frame.
+            const int nsteps = (it.nj + 15) >> 4;
+            it.ws = it.we = 0u;
+            if (nsteps > 255) {
+              it.code |= kCodeNoWin;
+            } else {
+              for (int b = 0; b < 4; ++b) {
+                if (bcf[b] < 0) continue;
+                const int lo = max(seg_lo[s], bcf[b] - Kz - seg_k[s] * Pz);
+                const int hi = min(seg_hi[s], bcl[b] + Kz - seg_k[s] * Pz);
+                if (lo > hi) continue;
+                int w0 = a.y_cell_start[col + lo];
+                int w1 = a.y_cell_start[col + hi + 1];
+                if (a.auto_) w0 = max(w0, i0 + 1);
+                if (w1 <= w0) continue;
+                if (a.map) {
+                  w0 = a.map[w0];
+                  w1 = a.map[w1];
+                }
+                const int sb = min(max((w0 - j0) >> 4, 0), nsteps);
+                const int eb = min(max((w1 - j0 + 15) >> 4, 0), nsteps);
+                if (eb <= sb) continue;
+                it.ws |= (unsigned)sb << (8 * b);
+                it.we |= (unsigned)eb << (8 * b);
+              }
+            }
             out[count] = it;
           }
           ++count;
