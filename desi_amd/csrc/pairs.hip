@@ -109,6 +109,20 @@ __device__ __forceinline__ float4 gld(const float4* p, int i) {
   const gv4f v = ((GV4)p)[i];
   return make_float4(v.x, v.y, v.z, v.w);
 }
+// The Gram loops' j points: x, y, z only (a 12-byte load), so that no
+// register of the prefetch in flight is free for the compiler to reuse (an
+// unused w would be, forcing a wait on the prefetch within the step that
+// issued it), through a buffer resource over the run [0, n): the lane's
+// offset is fixed and the step's is scalar (no per-lane address arithmetic),
+// and points at or past n read as zeros (the loops mask those lanes).
+typedef float gv3f __attribute__((ext_vector_type(3)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t point_rsrc(const float4* p, int n) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)((unsigned)n * 16u),
+                                           0x00020000);
+}
+__device__ __forceinline__ gv3f bld3(__amdgpu_buffer_rsrc_t r, int lane_off, int j) {
+  return __builtin_amdgcn_raw_buffer_load_b96(r, lane_off, j * 16, 0);
+}
 
 // FP32 bulk + FP64 edge re-evaluation.
 //
@@ -522,8 +536,9 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
         if (b >= nbk) break;
         const unsigned s0 = (wsb >> (8 * b)) & 0xffu;
         const unsigned e0 = nowin ? 0xffffffu : ((web >> (8 * b)) & 0xffu);
-        wlo[b] = s0;
-        wlen[b] = e0 > s0 ? e0 - s0 : 0u;
+        // Uniform: kept in scalar registers, so the per-step test is scalar.
+        wlo[b] = (unsigned)__builtin_amdgcn_readfirstlane((int)s0);
+        wlen[b] = (unsigned)__builtin_amdgcn_readfirstlane((int)(e0 > s0 ? e0 - s0 : 0u));
         if (e0 > s0) {
           t_first = min(t_first, (int)s0);
           t_last = max(t_last, (int)e0);
@@ -575,13 +590,14 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
         windows(wlo, wlen, &jstart, &jstop);
         // The next step's j point is loaded one step ahead (its latency hides
         // behind this step's matrix, LDS and histogram work).
-        float4 qn4 = gld(yp, min(jstart + col, jend - 1));
+        const __amdgpu_buffer_rsrc_t jrs = point_rsrc(yp, jend);
+        gv3f qn4 = bld3(jrs, 16 * col, jstart);
         for (int jb = jstart; jb < jstop; jb += 16) {
           // j side: A operand (component grp of point j = jb + col; grp 3:
           // |x_j|^2, huge past the run's end).
           const int jj = jb + col;
-          const float4 q = qn4;
-          qn4 = gld(yp, min(jj + 16, jend - 1));
+          const gv3f q = qn4;
+          qn4 = bld3(jrs, 16 * col, jb + 16);
           const unsigned act = step_blocks(jb, wlo, wlen);
           if (act == 0) continue;
           const float z = q.z - oz;
@@ -628,13 +644,14 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
         unsigned wlo[NB], wlen[NB];
         int jstart, jstop;
         windows(wlo, wlen, &jstart, &jstop);
-        float4 qn4 = gld(yp, min(jstart + col, jend - 1));
+        const __amdgpu_buffer_rsrc_t jrs = point_rsrc(yp, jend);
+        gv3f qn4 = bld3(jrs, 16 * col, jstart);
         for (int jb = jstart; jb < jstop; jb += 16) {
           // j side: A operands (component grp of point j = jb + col; a j past
           // the run's end gets huge squared norms in both).
           const int jj = jb + col;
-          const float4 q = qn4;
-          qn4 = gld(yp, min(jj + 16, jend - 1));
+          const gv3f q = qn4;
+          qn4 = bld3(jrs, 16 * col, jb + 16);
           const unsigned act = step_blocks(jb, wlo, wlen);
           if (act == 0) continue;
           const float x = q.x - ox, y = q.y - oy, z = q.z - oz;
