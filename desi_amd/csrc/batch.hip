@@ -124,7 +124,8 @@ hipError_t enqueue_plan(pcf_ctx* c, BatchPlan* p, size_t njobs, bool prepare, bo
       if ((e = launch_scale_points(p->scale_src[k], p->scale_n[k], p->scale_sigma[k],
                                    sc + p->scale_off[k], c->stream)) != hipSuccess)
         return e;
-    if ((e = launch_zero(p->hist_d.p, hist_bytes + w_bytes + 64, c->stream)) != hipSuccess)
+    if ((e = launch_zero(p->hist_d.p, hist_bytes + w_bytes + kBatchTailBytes, c->stream)) !=
+        hipSuccess)
       return e;
   }
   unsigned long long* d_stats =
@@ -135,7 +136,8 @@ hipError_t enqueue_plan(pcf_ctx* c, BatchPlan* p, size_t njobs, bool prepare, bo
   PairLaunch pl{};
   pl.items = p->items.as<PairItem>();
   pl.nitems = p->total_items;
-  pl.counter = reinterpret_cast<int*>(d_stats + 4);
+  // Chunk cursors: one per XCD range, each on a 64-byte line of its own.
+  pl.counter = reinterpret_cast<int*>(d_stats + 8);
   pl.table = p->table_d.as<uint8_t>() + g0.table_off;
   pl.nt = g0.nt;
   pl.inv_h = g0.inv_h;
@@ -387,7 +389,7 @@ int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p, GridPins
   HIPCHK(c, p->jobs_d.ensure(sizeof(BatchJob) * njobs));
   HIPCHK(c, p->chunks_d.ensure(sizeof(BatchChunk) * std::max<size_t>(1, p->chunks.size())));
   HIPCHK(c, p->hist_d.ensure((sizeof(unsigned long long) + sizeof(double)) *
-                                 (size_t)p->total_rows + 64));
+                                 (size_t)p->total_rows + kBatchTailBytes));
   HIPCHK(c, hipMemcpyAsync(p->jobs_d.p, jd.data(), sizeof(BatchJob) * njobs,
                            hipMemcpyHostToDevice, c->stream));
   if (!p->chunks.empty())

@@ -768,14 +768,34 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
   } else {
     __shared__ int s_chunk, s_next;
     const uint8_t* loaded = table;  // the table (and rules) now in LDS
+    // The chunk list (jobs in order, each in cell order) is cut into kXcds
+    // contiguous ranges, one per XCD: workgroups are dealt to the XCDs round
+    // robin, so workgroup b first takes the chunks of range b % kXcds
+    // (neighbouring cells of one job, whose j columns the XCD's L2 then
+    // shares), then helps the other ranges.  counter[r * 16] is range r's
+    // cursor; thread 0 keeps the ranges it has found empty.
+    const int home = blockIdx.x % kXcds;
+    int nspent = 0;
     for (;;) {
       if (threadIdx.x == 0) {
-        s_chunk = atomicAdd(counter, 1);
+        int ch = -1;
+        while (nspent < kXcds) {
+          const int r = (home + nspent) % kXcds;
+          const int rb = (int)(((long long)k.nitems * r) / kXcds);
+          const int re = (int)(((long long)k.nitems * (r + 1)) / kXcds);
+          const int v = rb + atomicAdd(counter + r * 16, 1);
+          if (v < re) {
+            ch = v;
+            break;
+          }
+          ++nspent;
+        }
+        s_chunk = ch;
         s_next = 0;
       }
       __syncthreads();
       const int c = s_chunk;
-      if (c >= k.nitems) break;  // nitems = number of chunks in batch mode
+      if (c < 0) break;  // every range done (nitems = number of chunks in batch mode)
       const int cb = chunks[c].begin, ce = chunks[c].end;
       const BatchJob jd = jobs[chunks[c].job];
       // The chunk's job parameters (the same for every thread of the

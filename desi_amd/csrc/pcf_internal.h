@@ -191,6 +191,9 @@ struct BatchJob {
   float rho2;        // Gram path bound (scaled units)
   float glo[3], gcs[3], ginv[3];  // grid geometry in this job's scaled units
 };
+// Behind a batch's histograms: 4 statistics words, 4 spare, then one chunk
+// cursor per XCD range (8 ranges, 64 bytes apart).
+constexpr size_t kBatchTailBytes = 64 + 8 * 64;
 // A run of work items [begin, end) that all belong to job `job`.
 struct BatchChunk {
   int begin, end, job;
