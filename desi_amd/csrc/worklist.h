@@ -46,26 +46,37 @@ __device__ inline int enumerate_cell(const WorkListArgs& a, int cell, int s, int
   int i0 = col_start + ((ib - col_start + kTile - 1) / kTile) * kTile;
   for (; i0 < ie; i0 += kTile) {
     const int ni = min(kTile, col_end - i0);
+    // The ends of cells cz .. cz + kCellWin - 1 of the column, read together
+    // (independent loads, not a chain of dependent ones): the cell of a
+    // tile point p >= i0 is cz plus the number of those ends <= p, while
+    // that count stays below kCellWin (beyond it the walk reads on).
+    constexpr int kCellWin = 8;
+    int cend[kCellWin];
+#pragma unroll
+    for (int t = 0; t < kCellWin; ++t)
+      cend[t] = cz + t + 1 < Pz ? a.x_cell_start[colbase + cz + t + 1] : 0x7fffffff;
+    auto cell_of_point = [&](int p) {
+      int c = cz;
+#pragma unroll
+      for (int t = 0; t < kCellWin; ++t) c += cend[t] <= p ? 1 : 0;
+      while (c + 1 < Pz && c - cz >= kCellWin && a.x_cell_start[colbase + c + 1] <= p) ++c;
+      return c;
+    };
     // z-cell of the tile's last point.
-    int zb = cz;
-    while (zb + 1 < Pz && a.x_cell_start[colbase + zb + 1] <= i0 + ni - 1) ++zb;
+    const int zb = cell_of_point(i0 + ni - 1);
     // z cells of the first and last point of each 16-point block (-1: no
     // such block), for the items' step windows.
     int bcf[4], bcl[4];
     if (FILL) {
-      int c = cz;
+#pragma unroll
       for (int b = 0; b < 4; ++b) {
         const int f = i0 + 16 * b;
         if (f >= i0 + ni) {
           bcf[b] = bcl[b] = -1;
           continue;
         }
-        const int l = min(f + 15, i0 + ni - 1);
-        while (c + 1 < Pz && a.x_cell_start[colbase + c + 1] <= f) ++c;
-        int e = c;
-        while (e + 1 < Pz && a.x_cell_start[colbase + e + 1] <= l) ++e;
-        bcf[b] = c;
-        bcl[b] = e;
+        bcf[b] = cell_of_point(f);
+        bcl[b] = cell_of_point(min(f + 15, i0 + ni - 1));
       }
     }
     for (int dx = dxl; dx <= dxh; ++dx) {
