@@ -170,3 +170,23 @@ def test_gram_2ani_batch_vs_direct_batch(engine):
         engine.release(cd)
         for c in crs:
             engine.release(c)
+
+
+@pytest.mark.parametrize("cross", [False, True])
+def test_long_runs_and_block_windows(engine, cross):
+    """Work items carry step windows per 16-point block (blocks outside them
+    are not evaluated) unless a run has more than 255 steps of 16.  A dense
+    catalog with one-rmax cells gives runs of thousands of points (no
+    windows) beside the windowed items of a sparse second catalog; both forms
+    equal the oracle, and with windows fewer pairs are evaluated than the
+    items hold."""
+    dense = synth.uniform(30000, 100.0, seed=41)  # ~7,500-point runs at 1 cell per rmax
+    c, st = _both(engine, dense, None, bpc=False, box=100.0, bins=20, dmax=45.0, cpr=1)
+    co, _ = O.pairs(dense, None, bins=20, dmax=45.0)
+    assert np.array_equal(c, co)
+    sparse = synth.uniform(20000, 400.0, seed=42)
+    b = synth.uniform(15000, 400.0, seed=43) if cross else None
+    c, st = _both(engine, sparse, b, bpc=True, box=400.0, bins=25, dmax=60.0, cpr=2)
+    co, _ = O.pairs(sparse, b, bpc=True, box=400.0, bins=25, dmax=60.0)
+    assert np.array_equal(c, co)
+    assert st["evaluations"] > 0
