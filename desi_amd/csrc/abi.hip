@@ -408,6 +408,11 @@ int choose_geometry(const Catalog* X, const Catalog* Y, float dmax, int bpc, flo
   return 0;
 }
 
+uint64_t next_grid_build_id() {
+  static std::atomic<uint64_t> ids{0};
+  return ++ids;
+}
+
 static void free_grid(DevGrid* gr) {
   if (gr->pts) (void)hipFree(gr->pts);
   if (gr->pts64) (void)hipFree(gr->pts64);
@@ -431,6 +436,7 @@ DevGrid* get_grid(pcf_ctx* c, Catalog* cat, const GridGeom& g, bool force_rebuil
       if (force_rebuild && gr->users == 0) {
         e = build_grid(cat->raw, cat->n, g, gr.get(), c->scratch_cells, c->scratch_perm,
                        c->grid_stream);
+        gr->build_id = next_grid_build_id();
         if (e == hipSuccess) e = hipEventRecord(gr->ready, c->grid_stream);
       }
       if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, gr->ready, 0);
@@ -459,6 +465,7 @@ DevGrid* get_grid(pcf_ctx* c, Catalog* cat, const GridGeom& g, bool force_rebuil
   if (e == hipSuccess)
     e = build_grid(cat->raw, cat->n, g, gr.get(), c->scratch_cells, c->scratch_perm,
                    c->grid_stream);
+  gr->build_id = next_grid_build_id();
   if (e == hipSuccess) e = hipEventRecord(gr->ready, c->grid_stream);
   if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, gr->ready, 0);
   if (e != hipSuccess) {

@@ -586,6 +586,9 @@ int run_pairs_batch(pcf_ctx* c, const pcf_job* jobs_in, int njobs, uint64_t* cou
   if (p->regrid) {
     std::lock_guard<std::mutex> lk(c->store->mu);
     for (DevGrid* g : p->grids) shared = shared || g->users > 1;
+    // The launch rewrites the grids' point order.
+    if (!shared)
+      for (DevGrid* g : p->grids) g->build_id = next_grid_build_id();
   }
   // ev[1] .. kev: preparation; kev .. ev[2]: the counting kernel.
   HIPCHK(c, hipEventRecord(c->ev[1], c->stream));

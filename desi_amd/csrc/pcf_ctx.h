@@ -37,6 +37,29 @@ struct CatalogStore {
   // order and 32-bit arithmetic (read by the context and its serving lanes).
   std::atomic<bool> ref_bugs{false};
 };
+
+// The triplet pair list left in trip_idx (offsets) and trip_pairs (records)
+// by the last triplet job, and what it was built from.  DDD and DDR (DRR and
+// RRR) list the same pairs of one catalog, so the second job reuses it.
+struct TripPairKey {
+  uint64_t grid_build = 0;  // DevGrid::build_id of the catalog's grid
+  int bins = 0, bpc = 0, ani = 0, key_t = 0, nt = 0;
+  int K[3] = {0, 0, 0};
+  float dmax = 0.f, L = 0.f, sigma = 0.f, reach = 0.f;
+  double R = 0.0;
+  bool same(const TripPairKey& o) const {
+    return grid_build == o.grid_build && bins == o.bins && bpc == o.bpc && ani == o.ani &&
+           key_t == o.key_t && nt == o.nt && K[0] == o.K[0] && K[1] == o.K[1] &&
+           K[2] == o.K[2] && dmax == o.dmax && L == o.L && sigma == o.sigma &&
+           reach == o.reach && R == o.R;
+  }
+};
+struct TripPairCache {
+  bool valid = false;
+  TripPairKey key;
+  std::vector<uint8_t> ptab;  // the pair table the list was classified with
+  int npairs = 0;
+};
 }  // namespace pcf
 
 struct pcf_ctx {
@@ -59,6 +82,7 @@ struct pcf_ctx {
   pcf_stats last{};
   pcf::JobEngine* engine = nullptr;  // asynchronous serving engine (jobs.hip)
   pcf::BatchPlan* batch = nullptr;   // last batched launch plan + graph (batch.hip)
+  pcf::TripPairCache trip_cache;     // what trip_idx / trip_pairs hold (triplets.hip)
 };
 
 

@@ -91,7 +91,13 @@ struct DevGrid {
   bool valid = false;
   hipEvent_t ready = nullptr;  // recorded after the last (re)build, on the building stream
   int users = 0;               // jobs holding the grid (no eviction or rebuild meanwhile)
+  // Unique per build of the point order (next_grid_build_id): what was derived
+  // from the grid (a triplet pair list) is still valid while it is unchanged.
+  uint64_t build_id = 0;
 };
+
+// A process-wide unique id for each grid build.
+uint64_t next_grid_build_id();
 
 struct Catalog {
   float4* raw = nullptr;

@@ -1406,8 +1406,6 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   wa.auto_ = 1;
   wa.cylinder = 0;
   wa.map = nullptr;
-  int npitems = 0;
-  HIPCHK(c, build_pair_worklist(wa, c->wl_counts, c->wl_items, &npitems, c->stream));
   const int64_t nx = gx->n;
   // 3ani goes through the LDS-privatised kernel when one t value of a b12 row
   // ([b31][b23][p]) fits the LDS budget and the (point 1, b12, t) keys fit an
@@ -1418,15 +1416,8 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   const int64_t nkeys = ani_lds ? nx * b64 * b64 : nx * bins;
   if (nkeys + 1 > (int64_t)INT32_MAX) return fail(c, PCF_ERR_UNSUPPORTED, "too many pair keys");
   const size_t kbytes = (sizeof(int) * (size_t)(3 * (nkeys + 1) + 8) + 15) & ~(size_t)15;
-  HIPCHK(c, c->trip_idx.ensure(kbytes + sizeof(long long) * (size_t)scan_tiles(nkeys)));
-  int* d_cnt = c->trip_idx.as<int>();
-  int* d_off = d_cnt + (nkeys + 1);
-  int* d_cur = d_off + (nkeys + 1);
-  long long* d_tsum = reinterpret_cast<long long*>(c->trip_idx.as<char>() + kbytes);
-  HIPCHK(c, launch_zero(d_cnt, sizeof(int) * (size_t)(nkeys + 1), c->stream));
-  HIPCHK(c, launch_zero(d_cur, sizeof(int) * (size_t)(nkeys + 1), c->stream));
   EmitArgs ea;
-  ea.nitems = npitems;
+  ea.nitems = 0;
   ea.nt = nt;
   ea.Ls = L * sigma;
   ea.bins = bins;
@@ -1438,25 +1429,62 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   ea.ds_th = (double)ds_th_f;
   ea.key_t = ani_lds ? 1 : 0;
   const int emit_blocks = std::max(1, c->ncu * 8);
-  if (npitems > 0) {
-    hipLaunchKernelGGL(k_emit_pairs<0>, dim3(emit_blocks), dim3(kEmitBlock), 0, c->stream, xs,
-                       gx->pts, c->wl_items.as<PairItem>(), d_ptab, d_T, d_cnt, d_off, d_cur,
-                       (PairRec*)nullptr, ea);
-    HIPCHK(c, hipGetLastError());
-  }
-  HIPCHK(c, launch_scan(d_cnt, nkeys, d_off, d_tsum, c->stream));
+  // The list depends on X's grid (its point order), the bins, range, box and
+  // the scale and table of the FP32 classification; a job whose inputs match
+  // the last list's (DDR after DDD, RRR after DRR) keeps it.
+  TripPairKey pk;
+  pk.grid_build = gx->build_id;
+  pk.bins = bins;
+  pk.bpc = bpc;
+  pk.ani = ani ? 1 : 0;
+  pk.key_t = ea.key_t;
+  pk.nt = nt;
+  for (int q = 0; q < 3; ++q) pk.K[q] = Kc[q];
+  pk.dmax = job->dmax;
+  pk.L = L;
+  pk.sigma = sigma;
+  pk.reach = reach;
+  pk.R = pr.R;
+  TripPairCache& pc = c->trip_cache;
   int npairs = 0;
-  HIPCHK(c, hipMemcpyAsync(&npairs, d_off + nkeys, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (npairs < 0) return fail(c, PCF_ERR_UNSUPPORTED, "pair list exceeds 2^31 records");
-  HIPCHK(c, c->trip_pairs.ensure(sizeof(PairRec) * (size_t)std::max(npairs, 1)));
-  PairRec* d_pairs = c->trip_pairs.as<PairRec>();
-  if (npitems > 0 && npairs > 0) {
-    hipLaunchKernelGGL(k_emit_pairs<1>, dim3(emit_blocks), dim3(kEmitBlock), 0, c->stream, xs,
-                       gx->pts, c->wl_items.as<PairItem>(), d_ptab, d_T, d_cnt, d_off, d_cur,
-                       d_pairs, ea);
-    HIPCHK(c, hipGetLastError());
+  if (pc.valid && pc.key.same(pk) && pc.ptab == ptab) {
+    npairs = pc.npairs;
+  } else {
+    pc.valid = false;
+    int npitems = 0;
+    HIPCHK(c, build_pair_worklist(wa, c->wl_counts, c->wl_items, &npitems, c->stream));
+    HIPCHK(c, c->trip_idx.ensure(kbytes + sizeof(long long) * (size_t)scan_tiles(nkeys)));
+    int* d_cnt = c->trip_idx.as<int>();
+    int* d_off0 = d_cnt + (nkeys + 1);
+    int* d_cur = d_off0 + (nkeys + 1);
+    long long* d_tsum = reinterpret_cast<long long*>(c->trip_idx.as<char>() + kbytes);
+    HIPCHK(c, launch_zero(d_cnt, sizeof(int) * (size_t)(nkeys + 1), c->stream));
+    HIPCHK(c, launch_zero(d_cur, sizeof(int) * (size_t)(nkeys + 1), c->stream));
+    ea.nitems = npitems;
+    if (npitems > 0) {
+      hipLaunchKernelGGL(k_emit_pairs<0>, dim3(emit_blocks), dim3(kEmitBlock), 0, c->stream, xs,
+                         gx->pts, c->wl_items.as<PairItem>(), d_ptab, d_T, d_cnt, d_off0, d_cur,
+                         (PairRec*)nullptr, ea);
+      HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, launch_scan(d_cnt, nkeys, d_off0, d_tsum, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&npairs, d_off0 + nkeys, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (npairs < 0) return fail(c, PCF_ERR_UNSUPPORTED, "pair list exceeds 2^31 records");
+    HIPCHK(c, c->trip_pairs.ensure(sizeof(PairRec) * (size_t)std::max(npairs, 1)));
+    if (npitems > 0 && npairs > 0) {
+      hipLaunchKernelGGL(k_emit_pairs<1>, dim3(emit_blocks), dim3(kEmitBlock), 0, c->stream, xs,
+                         gx->pts, c->wl_items.as<PairItem>(), d_ptab, d_T, d_cnt, d_off0, d_cur,
+                         c->trip_pairs.as<PairRec>(), ea);
+      HIPCHK(c, hipGetLastError());
+    }
+    pc.key = pk;
+    pc.ptab = ptab;
+    pc.npairs = npairs;
+    pc.valid = true;
   }
+  int* const d_off = c->trip_idx.as<int>() + (nkeys + 1);
+  PairRec* const d_pairs = c->trip_pairs.as<PairRec>();
 
   // 2. Histograms.
   const size_t hbytes = sizeof(unsigned long long) * (size_t)nflat;

@@ -132,6 +132,52 @@ def test_weighted_clustered(engine, ani, bpc):
 
 
 @pytest.mark.parametrize("ani", [False, True])
+def test_pair_list_reuse(engine, ani):
+    """Resident catalogs through DDD, DDR, DRR, RRR (the second of each two
+    keeps the first's pair list) and then jobs that must not keep it: another
+    range, a rebuilt grid, and a new catalog under a released one's id.
+    Every histogram equals the oracle's."""
+    d = synth.neyman_scott(48, 24, 4.0, 200.0, seed=31, weights="uniform")
+    r = synth.uniform(1200, 200.0, seed=32, weights="uniform")
+    r2 = synth.uniform(1100, 200.0, seed=33, weights="uniform")
+    bins, dmax = (10, 40.0) if ani else (20, 50.0)
+    typ = da.PCF_3ANI if ani else da.PCF_3ISO
+    cd, cr = engine.upload(d), engine.upload(r)
+    cats = {cd: d, cr: r}
+
+    def run(x, y, *, dm=dmax, prec=da.PCF_PREC_MIXED, rebuild=False):
+        print(f"case 3{'ani' if ani else 'iso'} x={x} y={y} dmax={dm} prec={prec} "
+              f"rebuild={rebuild}", flush=True)
+        job = da.make_job(typ, da.PCF_XXX if y < 0 else da.PCF_XXY, bins=bins, dmax=dm,
+                          cat_x=x, cat_y=y, bpc=True, box=200.0, precision=prec,
+                          rebuild_grid=rebuild)
+        c, w = engine.histogram(job)
+        co, wo = O.triplets(cats[x], None if y < 0 else cats[y], ani=ani, bpc=True, box=200.0,
+                            bins=bins, dmax=dm)
+        assert np.array_equal(c, co), f"counts differ: {c.sum()} vs {co.sum()}"
+        assert np.all(np.abs(w - wo) <= WREL * np.abs(wo)), "weighted sums differ"
+        assert co.sum() > 0
+
+    try:
+        run(cd, -1)
+        run(cd, cr)
+        run(cr, cd)
+        run(cr, -1)
+        run(cr, -1, prec=da.PCF_PREC_FP64)
+        run(cr, -1, dm=dmax * 0.75)
+        run(cr, -1, rebuild=True)
+        run(cr, cd)
+        engine.release(cr)
+        cr2 = engine.upload(r2)
+        cats[cr2] = r2
+        run(cr2, -1)
+        run(cr2, cd)
+        engine.release(cr2)
+    finally:
+        engine.release(cd)
+
+
+@pytest.mark.parametrize("ani", [False, True])
 @pytest.mark.parametrize("bpc", [False, True])
 def test_lattice_exact_edges(engine, ani, bpc):
     a = synth.lattice(1500, 32.0, 0.5, seed=31)
