@@ -417,10 +417,12 @@ static void free_grid(DevGrid* gr) {
   if (gr->pts) (void)hipFree(gr->pts);
   if (gr->pts64) (void)hipFree(gr->pts64);
   if (gr->cell_start) (void)hipFree(gr->cell_start);
+  if (gr->fine_start) (void)hipFree(gr->fine_start);
   if (gr->ready) (void)hipEventDestroy(gr->ready);
   gr->pts = nullptr;
   gr->pts64 = nullptr;
   gr->cell_start = nullptr;
+  gr->fine_start = nullptr;
   gr->ready = nullptr;
 }
 
@@ -512,8 +514,9 @@ namespace pcf {
 
 // Largest tile box half-diagonal the Gram path's error bound is built for:
 // a 64-point tile spans about 64 / (points per cell) cells of its column
-// (points inside a cell are in no particular z order, so the end cells count
-// whole), plus one cell either way; tiles beyond it take the direct path.
+// (the end cells count whole: the bound does not rely on the z sub-slab order
+// inside a cell), plus one cell either way; tiles beyond it take the direct
+// path.
 double gram_rho(const GridGeom& g, int64_t n) {
   const double pc = (double)std::max<int64_t>(n, 1) / (double)g.ncell();
   const double zsub = std::max(1.0, std::round((double)g.cs[0] / (double)g.cs[2]));
@@ -631,6 +634,9 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights,
   wa.auto_ = auto_ ? 1 : 0;
   wa.cylinder = ani ? 1 : 0;
   wa.map = nullptr;
+  wa.y_fine_start = nullptr;
+  wa.x_pts = nullptr;
+  wa.zq = 1;
   const int64_t cap = worklist_bound(wa, X->n);
 
   const bool rebuild = job->rebuild_grid != 0;
@@ -679,6 +685,11 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights,
     // host round trip), unless the bound is too large to allocate casually.
     wa.x_cell_start = gx->cell_start;
     wa.y_cell_start = gy->cell_start;
+    if (gx->zq > 1 && gy->zq == gx->zq) {
+      wa.y_fine_start = gy->fine_start;
+      wa.x_pts = gx->pts;
+      wa.zq = gx->zq;
+    }
     if (cap <= ((int64_t)1 << 25)) {
       HIPCHK(c, build_pair_worklist_async(wa, cap, c->wl_counts, c->wl_items, &nitems_dev,
                                           c->stream));
@@ -735,16 +746,16 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights,
     void* wl_counts = tmp.second.second;
     ax.g = g;
     ax.n = X->n;
-    HIPCHK(c, build_grid_into(X->raw, X->n, g, ax.pts, ax.cell_start, cells_tmp, perm_tmp,
-                              c->stream));
+    HIPCHK(c, build_grid_into(X->raw, X->n, g, ax.pts, ax.cell_start, nullptr, 1, cells_tmp,
+                              perm_tmp, c->stream));
     if (plain) HIPCHK(c, convert_grid64(ax.pts, ax.n, ax.pts64, c->stream));
     gx = &ax;
     gy = gx;
     if (!auto_) {
       ay.g = g;
       ay.n = Y->n;
-      HIPCHK(c, build_grid_into(Y->raw, Y->n, g, ay.pts, ay.cell_start, cells_tmp, perm_tmp,
-                                c->stream));
+      HIPCHK(c, build_grid_into(Y->raw, Y->n, g, ay.pts, ay.cell_start, nullptr, 1, cells_tmp,
+                                perm_tmp, c->stream));
       if (plain) HIPCHK(c, convert_grid64(ay.pts, ay.n, ay.pts64, c->stream));
       gy = &ay;
     }

@@ -299,6 +299,9 @@ int run_analytic_3iso(pcf_ctx* c, pcf_catalog data, int bins, float dmax, float 
   wa.auto_ = 1;
   wa.cylinder = 0;
   wa.map = nullptr;
+  wa.y_fine_start = nullptr;  // windows from whole z cells
+  wa.x_pts = nullptr;
+  wa.zq = 1;
   int nitems = 0;
   HIPCHK(c, build_pair_worklist(wa, c->wl_counts, c->wl_items, &nitems, c->stream));
 

@@ -115,8 +115,9 @@ hipError_t enqueue_plan(pcf_ctx* c, BatchPlan* p, size_t njobs, bool prepare, bo
   if (part == 0 && !prepare && regrid) {
     for (size_t k = 0; k < p->grids.size(); ++k)
       if ((e = build_grid_into(p->regrid_raw[k], p->regrid_n[k], p->g, p->grids[k]->pts,
-                               p->grids[k]->cell_start, p->regrid_cells.p,
-                               p->regrid_perm.as<int>(), c->stream)) != hipSuccess)
+                               p->grids[k]->cell_start, p->grids[k]->fine_start,
+                               p->grids[k]->zq, p->regrid_cells.p, p->regrid_perm.as<int>(),
+                               c->stream)) != hipSuccess)
         return e;
   }
   if (part == 0 && !prepare) {
@@ -339,6 +340,10 @@ int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p, GridPins
     wa[k].auto_ = auto_ ? 1 : 0;
     wa[k].cylinder = ani ? 1 : 0;
     wa[k].map = nullptr;
+    const bool fine = gx->zq > 1 && gy->zq == gx->zq;
+    wa[k].y_fine_start = fine ? gy->fine_start : nullptr;
+    wa[k].x_pts = fine ? gx->pts : nullptr;
+    wa[k].zq = fine ? gx->zq : 1;
   }
   HIPCHK(c, build_batch_worklists(wa, p->counts, p->items, &p->job_offsets, c->stream));
   p->total_items = p->job_offsets[njobs];
@@ -423,7 +428,11 @@ int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p, GridPins
   p->regrid = false;
   for (int k = 0; k < njobs; ++k) p->regrid = p->regrid || jobs[k].rebuild_grid != 0;
   if (p->regrid) {
-    HIPCHK(c, p->regrid_cells.ensure(grid_cells_tmp_bytes(p->g.ncell())));
+    // Each grid is rebuilt in its own point order (z sub-slabs): the plan's
+    // step windows assume it.
+    int zq = 1;
+    for (DevGrid* g : p->grids) zq = std::max(zq, g->zq);
+    HIPCHK(c, p->regrid_cells.ensure(grid_cells_tmp_bytes(p->g.ncell() * zq)));
     HIPCHK(c, p->regrid_perm.ensure(2 * sizeof(int) * (size_t)nmax));
   }
 

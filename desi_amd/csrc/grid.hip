@@ -4,12 +4,15 @@
 // re-allocating the node's array on every insert (cuda/create_grid.cuh:81-138),
 // then linearises the non-empty nodes and copies them to the device
 // (cuda/main.cu:215-329).  Here the grid is a counting sort on the device:
-//   1. k_cell_index  : cell of every point, per-cell counts (global atomics)
-//                      and the point's rank in its cell (the atomic's return)
-//   2. launch_scan   : cell_start[] = exclusive prefix sum of the counts
-//   3. k_scatter     : point i copied to cell_start[cell] + rank, float4
+//   1. k_cell_index  : sort key of every point (its cell's z sub-slab), per-key
+//                      counts (global atomics) and the point's rank among its
+//                      key's points (the atomic's return)
+//   2. launch_scan   : key starts = exclusive prefix sum of the counts; the
+//                      cell starts are every zq-th of them
+//   3. k_scatter     : point i copied to start[key] + rank, float4
 // The resulting layout is cell-ordered (x-major, z fastest) so that every z-range
-// of a (cx,cy) column is one contiguous, coalesced point range.
+// of a (cx,cy) column is one contiguous, coalesced point range, and within a
+// cell the points are in z order up to one sub-slab (cs_z / zq).
 #include <algorithm>
 
 #include "pcf_internal.h"
@@ -73,9 +76,10 @@ __device__ __forceinline__ int cell_axis(float v, float lo, float inv_cs, int P)
   return c >= P ? P - 1 : c;
 }
 
-// Cell of every point and its rank among its cell's points (the counting
-// atomic's return value), so the scatter needs no second atomic pass.
-__global__ void k_cell_index(const float4* __restrict__ raw, int64_t n, GridGeom g,
+// Sort key of every point (its cell, or with zq > 1 its cell's z sub-slab:
+// column * Pz * zq + zkey) and its rank among the points of that key (the
+// counting atomic's return value), so the scatter needs no second atomic pass.
+__global__ void k_cell_index(const float4* __restrict__ raw, int64_t n, GridGeom g, int zq,
                              int* __restrict__ cell_of, int* __restrict__ rank,
                              int* __restrict__ counts) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -83,10 +87,16 @@ __global__ void k_cell_index(const float4* __restrict__ raw, int64_t n, GridGeom
   float4 p = raw[i];
   int cx = cell_axis(p.x, g.lo[0], g.inv_cs[0], g.P[0]);
   int cy = cell_axis(p.y, g.lo[1], g.inv_cs[1], g.P[1]);
-  int cz = cell_axis(p.z, g.lo[2], g.inv_cs[2], g.P[2]);
-  int c = (cx * g.P[1] + cy) * g.P[2] + cz;
+  int c = (cx * g.P[1] + cy) * g.P[2] * zq + zkey(p.z, g, zq);
   cell_of[i] = c;
   rank[i] = atomicAdd(&counts[c], 1);
+}
+
+// cell_start[c] = fine_start[c * zq] for c = 0 .. ncell (the last is the total).
+__global__ void k_coarse_starts(const int* __restrict__ fine_start, int64_t ncell, int zq,
+                                int* __restrict__ cell_start) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c <= ncell) cell_start[c] = fine_start[c * zq];
 }
 
 
@@ -135,10 +145,15 @@ hipError_t build_grid(const float4* raw, int64_t n, const GridGeom& g, DevGrid* 
     out->pts = nullptr;
     if ((e = hipMalloc(&out->pts, sizeof(float4) * (n > 0 ? n : 1))) != hipSuccess) return e;
   }
+  const int zq = grid_zq(g);
   if (!out->cell_start || !out->g.same(g)) {
     if (out->cell_start) (void)hipFree(out->cell_start);
+    if (out->fine_start) (void)hipFree(out->fine_start);
     out->cell_start = nullptr;
+    out->fine_start = nullptr;
     if ((e = hipMalloc(&out->cell_start, sizeof(int) * (nc + 1))) != hipSuccess) return e;
+    if (zq > 1 && (e = hipMalloc(&out->fine_start, sizeof(int) * (nc * zq + 1))) != hipSuccess)
+      return e;
   }
   if (out->pts64) {
     (void)hipFree(out->pts64);
@@ -146,10 +161,11 @@ hipError_t build_grid(const float4* raw, int64_t n, const GridGeom& g, DevGrid* 
   }
   out->g = g;
   out->n = n;
-  if ((e = scratch_cells.ensure(grid_cells_tmp_bytes(nc))) != hipSuccess) return e;
+  out->zq = zq;
+  if ((e = scratch_cells.ensure(grid_cells_tmp_bytes(nc * zq))) != hipSuccess) return e;
   if ((e = scratch_perm.ensure(2 * sizeof(int) * (n > 0 ? n : 1))) != hipSuccess) return e;
-  if ((e = build_grid_into(raw, n, g, out->pts, out->cell_start, scratch_cells.p,
-                           scratch_perm.as<int>(), s)) != hipSuccess)
+  if ((e = build_grid_into(raw, n, g, out->pts, out->cell_start, out->fine_start, zq,
+                           scratch_cells.p, scratch_perm.as<int>(), s)) != hipSuccess)
     return e;
   out->valid = true;
   return hipSuccess;
@@ -161,27 +177,36 @@ size_t grid_cells_tmp_bytes(int64_t ncell) {
 }
 
 hipError_t build_grid_into(const float4* raw, int64_t n, const GridGeom& g, float4* pts,
-                           int* cell_start, void* cells_tmp, int* perm_tmp, hipStream_t s) {
+                           int* cell_start, int* fine_start, int zq, void* cells_tmp,
+                           int* perm_tmp, hipStream_t s) {
   hipError_t e;
+  if (zq < 1 || (zq > 1 && !fine_start)) return hipErrorInvalidValue;
   const int64_t nc = g.ncell();
-  const size_t cbytes = (sizeof(int) * (size_t)(nc + 1) + 15) & ~(size_t)15;
+  const int64_t nk = nc * zq;  // sort keys
+  int* starts = zq > 1 ? fine_start : cell_start;
+  const size_t cbytes = (sizeof(int) * (size_t)(nk + 1) + 15) & ~(size_t)15;
   int* counts = reinterpret_cast<int*>(cells_tmp);
   int* cell_of = perm_tmp;
   int* rank = cell_of + (n > 0 ? n : 1);
-  if ((e = launch_zero(counts, sizeof(int) * (nc + 1), s)) != hipSuccess) return e;
+  if ((e = launch_zero(counts, sizeof(int) * (nk + 1), s)) != hipSuccess) return e;
   const int blocks = (int)((n + 255) / 256);
   if (n > 0) {
-    hipLaunchKernelGGL(k_cell_index, dim3(blocks), dim3(256), 0, s, raw, n, g, cell_of, rank,
+    hipLaunchKernelGGL(k_cell_index, dim3(blocks), dim3(256), 0, s, raw, n, g, zq, cell_of, rank,
                        counts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  if ((e = launch_scan(counts, nc, cell_start,
+  if ((e = launch_scan(counts, nk, starts,
                        reinterpret_cast<long long*>(reinterpret_cast<char*>(cells_tmp) + cbytes),
                        s)) != hipSuccess)
     return e;
+  if (zq > 1) {
+    hipLaunchKernelGGL(k_coarse_starts, dim3((unsigned)((nc + 1 + 255) / 256)), dim3(256), 0, s,
+                       fine_start, nc, zq, cell_start);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
   if (n > 0) {
-    hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, raw, n, cell_of, rank,
-                       cell_start, pts);
+    hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, raw, n, cell_of, rank, starts,
+                       pts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   return hipSuccess;

@@ -43,6 +43,30 @@ struct GridGeom {
   }
 };
 
+// Resident grids order the points of a cell by z sub-slab: a cell is cut into
+// zq slabs along z and the counting sort runs over (cell, slab), so a column's
+// points are in z order up to one slab and a 16-point block spans a thin range
+// of z.  zq is a power of two, 16 unless the grid has more than 2^18 cells.
+inline int grid_zq(const GridGeom& g) {
+  int q = 16;
+  while (q > 1 && g.ncell() * q > ((int64_t)1 << 22)) q >>= 1;
+  return q;
+}
+
+// Index of the z sub-slab of coordinate z within its column: z cell * zq +
+// slab (the z cell clamped as the grid clamps it).  Monotonic in z, and the
+// same function places the points (grid build) and bounds the step windows
+// (work list).
+__host__ __device__ inline int zkey(float z, const GridGeom& g, int zq) {
+  const float t = (z - g.lo[2]) * g.inv_cs[2];
+  int cz = (int)floorf(t);
+  cz = cz < 0 ? 0 : (cz >= g.P[2] ? g.P[2] - 1 : cz);
+  if (zq == 1) return cz;
+  int sub = (int)floorf((t - (float)cz) * (float)zq);
+  sub = sub < 0 ? 0 : (sub >= zq ? zq - 1 : sub);
+  return cz * zq + sub;
+}
+
 // One work item of the flattened pair work list: the points [i0, i0+ni) of
 // one i-tile (ni <= 64, one per lane) against the contiguous j-range
 // [j0, j0+nj) of one neighbour column segment, seen through periodic image
@@ -87,6 +111,8 @@ struct DevGrid {
   float4* pts = nullptr;     // cell-ordered points
   double4* pts64 = nullptr;  // FP64 copy for the plain path (lazily built)
   int* cell_start = nullptr; // ncell+1
+  int* fine_start = nullptr; // ncell*zq+1: start of each (cell, z sub-slab), when zq > 1
+  int zq = 1;                // z sub-slabs per cell of the point order (grid_zq)
   int64_t n = 0;
   bool valid = false;
   hipEvent_t ready = nullptr;  // recorded after the last (re)build, on the building stream
@@ -172,10 +198,13 @@ hipError_t launch_catalog_stats(const float4* raw, int64_t n, float* partials, i
 hipError_t build_grid(const float4* raw, int64_t n, const GridGeom& g, DevGrid* out,
                       DevBuf& scratch_cells, DevBuf& scratch_perm, hipStream_t s);
 // The same counting sort into caller-provided memory: pts[n], cell_start[ncell+1],
-// and temporaries cells_tmp (grid_cells_tmp_bytes) and perm_tmp (2n ints).
+// with zq > 1 fine_start[ncell*zq+1] (the sort then runs over the z sub-slabs),
+// and temporaries cells_tmp (grid_cells_tmp_bytes(ncell * zq)) and perm_tmp
+// (2n ints).
 size_t grid_cells_tmp_bytes(int64_t ncell);
 hipError_t build_grid_into(const float4* raw, int64_t n, const GridGeom& g, float4* pts,
-                           int* cell_start, void* cells_tmp, int* perm_tmp, hipStream_t s);
+                           int* cell_start, int* fine_start, int zq, void* cells_tmp,
+                           int* perm_tmp, hipStream_t s);
 hipError_t convert_grid64(const float4* pts, int64_t n, double4* out, hipStream_t s);
 hipError_t build_grid64(DevGrid* grid, hipStream_t s);
 
@@ -328,6 +357,12 @@ struct WorkListArgs {
   int auto_;
   int cylinder;             // 2ani: |dz| < rmax and d_perp < rmax (not a sphere)
   const int* map;           // optional: j index -> position (pair-list offsets)
+  // Optional (null: windows from whole z cells): the j grid's sub-slab starts
+  // and the i grid's points, both ordered with zq sub-slabs per cell; a
+  // block's window then covers only the sub-slabs within reach of its own.
+  const int* y_fine_start;
+  const float4* x_pts;
+  int zq;
 };
 // Two-pass (count, scan, fill) work list on the device.  Returns the item
 // count through *nitems (a synchronous read of one integer).

@@ -1406,6 +1406,9 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   wa.auto_ = 1;
   wa.cylinder = 0;
   wa.map = nullptr;
+  wa.y_fine_start = nullptr;  // windows from whole z cells
+  wa.x_pts = nullptr;
+  wa.zq = 1;
   const int64_t nx = gx->n;
   // 3ani goes through the LDS-privatised kernel when one t value of a b12 row
   // ([b31][b23][p]) fits the LDS budget and the (point 1, b12, t) keys fit an

@@ -65,18 +65,26 @@ __device__ inline int enumerate_cell(const WorkListArgs& a, int cell, int s, int
     // z-cell of the tile's last point.
     const int zb = cell_of_point(i0 + ni - 1);
     // z cells of the first and last point of each 16-point block (-1: no
-    // such block), for the items' step windows.
-    int bcf[4], bcl[4];
+    // such block), for the items' step windows; with sub-slab grids also
+    // their sub-slab keys (the column's points are sorted by key, so the
+    // block's points all lie between the two).
+    int bcf[4], bcl[4], bkf[4], bkl[4];
     if (FILL) {
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         const int f = i0 + 16 * b;
+        bkf[b] = bkl[b] = 0;
         if (f >= i0 + ni) {
           bcf[b] = bcl[b] = -1;
           continue;
         }
+        const int l = min(f + 15, i0 + ni - 1);
         bcf[b] = cell_of_point(f);
-        bcl[b] = cell_of_point(min(f + 15, i0 + ni - 1));
+        bcl[b] = cell_of_point(l);
+        if (a.y_fine_start) {
+          bkf[b] = zkey(a.x_pts[f].z, g, a.zq);
+          bkl[b] = zkey(a.x_pts[l].z, g, a.zq);
+        }
       }
     }
     for (int dx = dxl; dx <= dxh; ++dx) {
@@ -109,6 +117,8 @@ __device__ inline int enumerate_cell(const WorkListArgs& a, int cell, int s, int
         // the z and perpendicular ranges are independent.
         const float hz = a.cylinder ? a.reach : sqrtf(reach2 - rho2);
         const int Kz = min(a.K[2], (int)ceilf(hz * g.inv_cs[2]));
+        // The same reach in sub-slabs, plus one for the rounding of the keys.
+        const int Kf = (int)ceilf(hz * 1.00001f * g.inv_cs[2] * (float)a.zq) + 1;
         const int col = (ccx * Py + ccy) * Pz;
         const int zlo = cz - Kz, zhi = zb + Kz;
         // Up to three z segments: wrapped below (kz=-1), inside, wrapped above.
@@ -148,7 +158,8 @@ __device__ inline int enumerate_cell(const WorkListArgs& a, int cell, int s, int
             it.nj = j1 - j0;
             it.code = ((kx + 1) * 9 + (ky + 1) * 3 + (seg_k[s] + 1)) | flag;
             // Step windows: the z cells within Kz of block b's cells (the
-            // tile's own rule, per block), in this segment's frame.
+            // tile's own rule, per block), in this segment's frame; with
+            // sub-slab grids the sub-slabs within Kf of the block's.
             const int nsteps = (it.nj + 15) >> 4;
             it.ws = it.we = 0u;
             if (nsteps > 255) {
@@ -156,11 +167,21 @@ __device__ inline int enumerate_cell(const WorkListArgs& a, int cell, int s, int
             } else {
               for (int b = 0; b < 4; ++b) {
                 if (bcf[b] < 0) continue;
-                const int lo = max(seg_lo[s], bcf[b] - Kz - seg_k[s] * Pz);
-                const int hi = min(seg_hi[s], bcl[b] + Kz - seg_k[s] * Pz);
-                if (lo > hi) continue;
-                int w0 = a.y_cell_start[col + lo];
-                int w1 = a.y_cell_start[col + hi + 1];
+                int w0, w1;
+                if (a.y_fine_start) {
+                  const int zq = a.zq;
+                  const int lo = max(seg_lo[s] * zq, bkf[b] - Kf - seg_k[s] * Pz * zq);
+                  const int hi = min(seg_hi[s] * zq + zq - 1, bkl[b] + Kf - seg_k[s] * Pz * zq);
+                  if (lo > hi) continue;
+                  w0 = a.y_fine_start[col * zq + lo];
+                  w1 = a.y_fine_start[col * zq + hi + 1];
+                } else {
+                  const int lo = max(seg_lo[s], bcf[b] - Kz - seg_k[s] * Pz);
+                  const int hi = min(seg_hi[s], bcl[b] + Kz - seg_k[s] * Pz);
+                  if (lo > hi) continue;
+                  w0 = a.y_cell_start[col + lo];
+                  w1 = a.y_cell_start[col + hi + 1];
+                }
                 if (a.auto_) w0 = max(w0, i0 + 1);
                 if (w1 <= w0) continue;
                 if (a.map) {

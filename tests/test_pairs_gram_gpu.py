@@ -190,3 +190,50 @@ def test_long_runs_and_block_windows(engine, cross):
     co, _ = O.pairs(sparse, b, bpc=True, box=400.0, bins=25, dmax=60.0)
     assert np.array_equal(c, co)
     assert st["evaluations"] > 0
+
+
+@pytest.mark.parametrize("ani", [False, True])
+@pytest.mark.parametrize("bpc", [False, True])
+def test_slab_windows_at_the_reach(engine, ani, bpc):
+    """Resident grids order each cell's points by z sub-slab, and a block's
+    step window keeps only the j slabs within the column's z reach of its own.
+    Pairs separated almost entirely along z, just inside and just outside
+    the range, at several lateral offsets, in a catalog dense enough for
+    windows that cut through cells: both forms equal the oracle, so no window
+    drops an in-range pair."""
+    rng = np.random.default_rng(7)
+    box, dmax, bins = 200.0, 30.0, 15
+    base = rng.uniform(5.0, box - 5.0, (4000, 3))
+    parts = [base]
+    for eps in (1e-3, 3e-5, -3e-5):
+        for gx in (0.0, 7.0, 19.0):
+            dz = np.sqrt(dmax * dmax - gx * gx) - eps
+            q = base[:250].copy()
+            q[:, 0] += gx
+            q[:, 2] += dz
+            parts.append(q)
+    xyz = np.concatenate(parts)
+    if bpc:
+        xyz = np.mod(xyz, box)
+        xyz[xyz >= box] = 0.0
+    a = np.zeros((len(xyz), 4), np.float32)
+    a[:, :3] = xyz
+    a[:, 3] = 1.0
+    b = a[::-1].copy()[:3000]
+    for y in (None, b):
+        print(f"case ani={ani} bpc={bpc} cross={y is not None}", flush=True)
+        for kernel in (da.PCF_PAIR_GRAM, da.PCF_PAIR_DIRECT):
+            ca = engine.upload(a)
+            cb = engine.upload(y) if y is not None else -1
+            try:
+                job = da.make_job(da.PCF_2ANI if ani else da.PCF_2ISO,
+                                  da.PCF_XX if y is None else da.PCF_XY, bins=bins, dmax=dmax,
+                                  cat_x=ca, cat_y=cb, bpc=bpc, box=box, cells_per_rmax=2,
+                                  pair_kernel=kernel)
+                c, _ = engine.histogram(job)
+            finally:
+                engine.release(ca)
+                if cb >= 0:
+                    engine.release(cb)
+            co, _ = O.pairs(a, y, ani=ani, bpc=bpc, box=box, bins=bins, dmax=dmax)
+            assert np.array_equal(c, co), f"kernel {kernel}: {c.sum()} vs {co.sum()}"
