@@ -47,14 +47,17 @@ BINS = 30
 
 # rocprofv3 summaries of the NS pair kernel: the Gram form (matrix cores) and
 # the direct form (vector units); the tuned choice's summary gives `traffic`.
-PAIR_PROFILE = {2: os.path.join(ROOT, "profiles", "k_pairs_gram_2iso_N1M_box1000_rmax80.md"),
-                1: os.path.join(ROOT, "profiles", "k_pairs_mixed_2iso_N1M_box1000_rmax80.md")}
+# Counter summary of the tuned pair kernel by form (the Gram form is the one the autotuner
+# picks on this workload; another pick reports traffic null).
+PAIR_PROFILE = {2: os.path.join(ROOT, "profiles", "k_pairs_gram_2iso_N1M_box1000_rmax80.md")}
 
 
 def profiled_traffic(path, kernel):
     """HBM bytes per dispatch of `kernel` from a committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, each scaled by the reading of a plain
     copy of known size), or None when the summary is absent."""
+    if path is None:
+        return None
     try:
         for line in open(path):
             if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
@@ -605,9 +608,12 @@ def main():
     # The dominant kernel: the Gram form forms d2 with four chained FMAs on the
     # matrix cores (FP32 MFMA peak 157.3 TFLOP/s, the same figure as the packed
     # FP32 vector peak); the direct form runs on the vector units.
-    kern = None if tuning is None else tuning.get("pair_kernel")
+    # Untuned, the library's automatic choice is the Gram form for this unweighted 2iso step.
+    kern = da.PCF_PAIR_GRAM if tuning is None else tuning.get("pair_kernel", da.PCF_PAIR_GRAM)
+    if kern == da.PCF_PAIR_AUTO:
+        kern = da.PCF_PAIR_GRAM
     bound = "mfma" if kern == da.PCF_PAIR_GRAM else "valu"
-    prof = PAIR_PROFILE.get(kern if kern in PAIR_PROFILE else da.PCF_PAIR_GRAM)
+    prof = PAIR_PROFILE.get(kern)
     out = {
         "metric": "pairs/s (2iso DD+RR+DR, useful pairs)",
         "value": value,
