@@ -616,8 +616,8 @@ int run_pairs(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weights,
   static thread_local std::vector<uint8_t> table;
   float inv_h = 1.f;
   int nt = 2;
-  float tflip = 0.f;  // 2iso: flipped table (index cvt_rz(tflip - d2))
-  if (!plain && build_table(rules, A, nt_max, &table, &inv_h, &nt, rho, !ani, &tflip) != 0)
+  float tflip = 0.f;  // flipped table: index cvt_rz(tflip - d2) (per component for 2ani)
+  if (!plain && build_table(rules, A, nt_max, &table, &inv_h, &nt, rho, true, &tflip) != 0)
     return fail(c, PCF_ERR_UNSUPPORTED, "bin table");
   const float sigma = std::sqrt(inv_h);  // exact power of two
   const size_t rt_bytes = sizeof(BinRules) + (plain ? 0 : (size_t)nt);

@@ -256,7 +256,7 @@ int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p, GridPins
       gr.dmax = jobs[k].dmax;
       make_rules(j0.type, bpc, 1, gr.bins, gr.dmax, L, &gr.rules);
       std::vector<uint8_t> table;
-      if (build_table(gr.rules, A, kMixedTableMax, &table, &gr.inv_h, &gr.nt, rho, !ani,
+      if (build_table(gr.rules, A, kMixedTableMax, &table, &gr.inv_h, &gr.nt, rho, true,
                       &gr.tflip) != 0)
         return fail(c, PCF_ERR_UNSUPPORTED, "bin table");
       gr.rho2 = p->gram ? (float)(rho * rho / (1.02 * 1.02) * (double)gr.inv_h) : 0.f;

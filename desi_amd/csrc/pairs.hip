@@ -76,7 +76,7 @@ struct KArgs {
   int cap;                // items allocated (the count is clamped to it)
   float rho2;             // Gram path: largest squared half-diagonal of a tile's
                           // bounding box (scaled units) its error bound covers
-  float tflip;            // 2iso: the table is indexed by cvt_rz(tflip - d2)
+  float tflip;            // the table is indexed by cvt_rz(tflip - d2) (2ani: per component)
   // Grid geometry in the bulk path's (scaled) units: the Gram path bounds a
   // tile by its cells.
   float glo[3], gcs[3], ginv[3];
@@ -143,14 +143,14 @@ __device__ __forceinline__ unsigned pair_row(const float4 q, float xs, float ys,
     const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
     return tab[__float2uint_rz(tflip - d2)];
   } else {
-    // 2ani reads two table cells: lanes with a component beyond the table's
-    // last cell (out of range) skip both reads.
+    // 2ani reads two cells of the flipped table: lanes with a component
+    // beyond the table (cell 0, out of range) skip both reads.
     const float dz2 = dz * dz;
     const float do2 = __builtin_fmaf(dy, dy, dx * dx);
-    const unsigned tz = __float2uint_rz(dz2);
-    const unsigned to = __float2uint_rz(do2);
+    const unsigned tz = __float2uint_rz(tflip - dz2);
+    const unsigned to = __float2uint_rz(tflip - do2);
     unsigned r = out_row;
-    if (tz < ntm1 && to < ntm1) {
+    if (tz != 0u && to != 0u) {
       const unsigned cz = tab[tz];
       const unsigned co = tab[to];
       const unsigned m = max(cz, co);
@@ -467,23 +467,25 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
         t_c[b] = ir < ni ? k.tflip - __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x)) : -1e30f;
       }
     } else {
-      // i side, 2ani: two products per block.  Perpendicular: B operand
-      // (-2x, -2y, 1, |x_perp|^2) of i against A (x, y, |x_perp|^2, 1) of j;
-      // line of sight: B (-2z, 1, z^2, 0) against A (z, z^2, 1, 0).  C = 0 for
-      // both (the i side's squared norm rides in the K dimension).
+      // i side, 2ani: two products per block, both giving Tf - d2 (the
+      // flipped table's index).  Perpendicular: B operand (2x, 2y, -1,
+      // Tf - |x_perp|^2) of i against A (x, y, |x_perp|^2, 1) of j; line of
+      // sight: B (2z, -1, Tf - z^2, 0) against A (z, z^2, 1, 0).  C = 0 for
+      // both (the i side's terms ride in the K dimension).
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         const int ir = 16 * b + col;
         const float4 p = gld(xp, i0 + min(ir, ni - 1));
         const float x = (p.x - shx) - ox, y = (p.y - shy) - oy, z = (p.z - shz) - oz;
-        // A lane past the tile gets huge squared norms in both products, so
-        // neither component can read as ambiguous (its pair does not exist).
+        // A lane past the tile gets hugely negative products (cell 0, out)
+        // in both, so neither component can read as ambiguous (its pair
+        // does not exist).
         const bool iv = ir < ni;
-        const float np2 = iv ? __builtin_fmaf(y, y, x * x) : 1e30f;
-        const float nz2 = iv ? z * z : 1e30f;
-        t_b[b] = __builtin_fmaf(sel_x, -2.0f * x,
-                                 __builtin_fmaf(sel_y, -2.0f * y, __builtin_fmaf(sel_w, np2, sel_z)));
-        t_c[b] = __builtin_fmaf(sel_x, -2.0f * z, __builtin_fmaf(sel_z, nz2, sel_y));
+        const float tp = iv ? k.tflip - __builtin_fmaf(y, y, x * x) : -1e30f;
+        const float tz = iv ? k.tflip - z * z : -1e30f;
+        t_b[b] = __builtin_fmaf(sel_x, 2.0f * x,
+                                 __builtin_fmaf(sel_y, 2.0f * y, __builtin_fmaf(sel_w, tp, -sel_z)));
+        t_c[b] = __builtin_fmaf(sel_x, 2.0f * z, __builtin_fmaf(sel_z, tz, -sel_y));
       }
     }
     return true;
@@ -672,8 +674,8 @@ __global__ __launch_bounds__(kBlockMixed) __attribute__((amdgpu_waves_per_eu(6, 
             unsigned r[4], mc[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-              const unsigned cz = tab[min(__float2uint_rz(dz[v]), ntm1)];
-              const unsigned co = tab[min(__float2uint_rz(dp[v]), ntm1)];
+              const unsigned cz = tab[__float2uint_rz(dz[v])];
+              const unsigned co = tab[__float2uint_rz(dp[v])];
               // The row; it counts only when both codes are bins (mc < bins).
               mc[v] = max(cz, co);
               r[v] = __umul24(cz, bins) + co;

@@ -140,7 +140,8 @@ void bin_thresholds(int bins, double ds, double* T);
 // rho >= 0: the table also covers the Gram-form pair path for tiles of box
 // half-diagonal <= rho (unscaled units).
 // flip: the table is indexed by cvt_rz(*top_out - d2) (d2 in table units)
-// instead of min(cvt_rz(d2), nt - 1); used by the 2iso pair kernels.
+// instead of min(cvt_rz(d2), nt - 1); used by the pair kernels (2iso, and each
+// component of 2ani).
 int build_table(const BinRules& r, double A, int nt_max, std::vector<uint8_t>* table, float* inv_h,
                 int* nt_out, double rho = -1.0, bool flip = false, float* top_out = nullptr);
 // zsub > 1 cuts every cell into zsub thinner cells along z (the line of the
