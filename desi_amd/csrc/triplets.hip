@@ -63,6 +63,7 @@ static_assert(sizeof(PairRec) == 24, "pair record layout");
 struct EmitArgs {
   int nitems;
   int nt;
+  float tf;       // the flipped tables' top: cell cvt_rz(tf - d2) holds d2
   float Ls;       // box side times sigma
   int bins;
   int bpc;
@@ -118,7 +119,6 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_pairs(
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = (int)((blockIdx.x * (unsigned)kEmitBlock + threadIdx.x) >> 6);
   const int nwaves = (int)((gridDim.x * (unsigned)kEmitBlock) >> 6);
-  const unsigned ntm1 = (unsigned)(e.nt - 1);
   const unsigned bins = (unsigned)e.bins;
   const unsigned amb_row = bins + 1;
   for (int it = wave; it < e.nitems; it += nwaves) {
@@ -143,7 +143,7 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_pairs(
       const float4 q = xs[j];
       const float dx = q.x - sx, dy = q.y - sy, dz = q.z - sz;
       const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-      unsigned row = tab[min(__float2uint_rz(d2), ntm1)];
+      unsigned row = tab[__float2uint_rz(e.tf - d2)];
       if (masked && j <= my_i) row = bins;
       if (!active) row = bins;
       // Everything emitted gets the exact bin: the pairs are few compared
@@ -241,7 +241,6 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_cross(
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = (int)((blockIdx.x * (unsigned)kEmitBlock + threadIdx.x) >> 6);
   const int nwaves = (int)((gridDim.x * (unsigned)kEmitBlock) >> 6);
-  const unsigned ntm1 = (unsigned)(e.nt - 1);
   const unsigned bins = (unsigned)e.bins;
   const unsigned amb_row = bins + 1;
   for (int it = wave; it < e.nitems; it += nwaves) {
@@ -264,7 +263,7 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_cross(
       const float4 q = ys[j];
       const float dx = q.x - sx, dy = q.y - sy, dz = q.z - sz;
       const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-      unsigned row = tab[min(__float2uint_rz(d2), ntm1)];
+      unsigned row = tab[__float2uint_rz(e.tf - d2)];
       if (!active) row = bins;
       if (row < bins || row == amb_row) {
         const float4 qo = yo[j];
@@ -293,6 +292,7 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_cross(
 struct TripArgs {
   int nx;         // points of X (point-1 tasks)
   int nt;
+  float tf;       // the flipped tables' top: cell cvt_rz(tf - d2) holds d2
   float Ls;
   int bins;
   int auto_;      // XXX: all sides > 0, point 3 is neither point 1 nor point 2
@@ -458,7 +458,6 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
   __syncthreads();
 
   const int lane = threadIdx.x & (kWave - 1);
-  const unsigned ntm1 = (unsigned)(a.nt - 1);
   const unsigned ubins = (unsigned)bins;
   unsigned long long evals = 0, nre = 0;
   // 3iso deposit of V row vrow = (b12 * bins + b23) * bins + b31: into the
@@ -526,7 +525,7 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
           const int simg = meta >> 16;
           const float cx = kq.x - r.xj, cy = kq.y - r.yj, cz = kq.z - r.zj;
           const float d23 = __builtin_fmaf(cz, cz, __builtin_fmaf(cy, cy, cx * cx));
-          unsigned c23 = PLAIN ? ubins + 1 : tab[min(__float2uint_rz(d23), ntm1)];
+          unsigned c23 = PLAIN ? ubins + 1 : tab[__float2uint_rz(a.tf - d23)];
           const bool amb = active && c23 > ubins;
           if (__ballot(amb)) {
             if (!PLAIN) nre += __popcll(__ballot(amb));
@@ -602,9 +601,12 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
           r.meta = 0;
           if (act) r = pairs[pbeg + pi];
           const float x = r.xj - is.x, y = r.yj - is.y, z = r.zj - is.z;
-          bop[b] = __builtin_fmaf(sel_x, -2.0f * x, __builtin_fmaf(sel_y, -2.0f * y,
-                                                                   __builtin_fmaf(sel_z, -2.0f * z, sel_w)));
-          cin[b] = act ? __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x)) : 1e30f;
+          // B (2x, 2y, 2z, -1) of point 2 and C = tf - |x2|^2: the product is
+          // tf - d23, the flipped table's index (an inactive column: hugely
+          // negative, cell 0, out).
+          bop[b] = __builtin_fmaf(sel_x, 2.0f * x, __builtin_fmaf(sel_y, 2.0f * y,
+                                                                  __builtin_fmaf(sel_z, 2.0f * z, -sel_w)));
+          cin[b] = act ? a.tf - __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x)) : -1e30f;
           rb[b] = (unsigned)((r.meta >> 8) & 0xfff) * (unsigned)bins2;
           wj[b] = r.wj;
           npact += (unsigned)__popcll(__ballot(act && grp == 0));
@@ -633,7 +635,7 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
 #pragma unroll
             for (int u = 0; u < HB; ++u)
 #pragma unroll
-              for (int v = 0; v < 4; ++v) c23[4 * u + v] = tab[min(__float2uint_rz(d[u][v]), ntm1)];
+              for (int v = 0; v < 4; ++v) c23[4 * u + v] = tab[__float2uint_rz(d[u][v])];
             unsigned mx = c23[0];
 #pragma unroll
             for (int q = 1; q < 4 * HB; ++q) mx = max(mx, c23[q]);
@@ -722,7 +724,7 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
         const float kx = kq.x + sxL, ky = kq.y + syL, kz = kq.z + szL;
         const float ax = kx - is.x, ay = ky - is.y, az = kz - is.z;
         const float d31 = __builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax));
-        unsigned c31 = PLAIN ? ubins + 1 : tab[min(__float2uint_rz(d31), ntm1)];
+        unsigned c31 = PLAIN ? ubins + 1 : tab[__float2uint_rz(a.tf - d31)];
         if (c31 == ubins) v = false;  // certainly out of range of point 1
         int b31 = (int)c31, pb = 0;
         if (v && (ANI || c31 > ubins)) {
@@ -864,7 +866,6 @@ __global__ __launch_bounds__(kTripBlock) void k_trip_ani_lds(
   }
   __syncthreads();
   const int lane = threadIdx.x & (kWave - 1);
-  const unsigned ntm1 = (unsigned)(a.nt - 1);
   const unsigned ubins = (unsigned)B;
   const int B2 = B * B;
   unsigned long long evals = 0, nre = 0;
@@ -1031,7 +1032,7 @@ __global__ __launch_bounds__(kTripBlock) void k_trip_ani_lds(
         auto lookup = [&](const PairRec& r) -> unsigned {
           const float cx = kr.xj - r.xj, cy = kr.yj - r.yj, cz = kr.zj - r.zj;
           const float d23 = __builtin_fmaf(cz, cz, __builtin_fmaf(cy, cy, cx * cx));
-          return PLAIN ? ubins + 1 : tab[min(__float2uint_rz(d23), ntm1)];
+          return PLAIN ? ubins + 1 : tab[__float2uint_rz(a.tf - d23)];
         };
         auto deposit = [&](const PairRec& r, unsigned c23) {
           if (active && c23 < ubins) {
@@ -1365,8 +1366,12 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   const double rho = gram ? (double)reach * 1.01 : -1.0;
   // 3ani's LDS kernel keeps its recheck queues behind a smaller table.
   const int tab_max = ani ? kAniTableMax : kMixedTableMax;
-  if (build_table(pr, A, tab_max, &ptab, &inv_h, &nt, rho) != 0 ||
-      build_table(kr, A, tab_max, &ktab, &inv_h2, &nt2, rho) != 0 || inv_h != inv_h2 ||
+  // Both tables are flipped (cell cvt_rz(tf - d2) holds d2: beyond the top
+  // reads cell 0, out, with no clamp) and share one scale and top.
+  float tf = 0.f, tf2 = 0.f;
+  if (build_table(pr, A, tab_max, &ptab, &inv_h, &nt, rho, true, &tf) != 0 ||
+      build_table(kr, A, tab_max, &ktab, &inv_h2, &nt2, rho, true, &tf2) != 0 || tf != tf2 ||
+      inv_h != inv_h2 ||
       nt != nt2 || nt > tab_max)
     return fail(c, PCF_ERR_UNSUPPORTED, "bin table");
   const float sigma = std::sqrt(inv_h);
@@ -1422,6 +1427,7 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   EmitArgs ea;
   ea.nitems = 0;
   ea.nt = nt;
+  ea.tf = tf;
   ea.Ls = L * sigma;
   ea.bins = bins;
   ea.bpc = bpc;
@@ -1506,6 +1512,7 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   TripArgs t;
   t.nx = (int)nx;
   t.nt = nt;
+  t.tf = tf;
   t.Ls = L * sigma;
   t.bins = bins;
   t.auto_ = auto_ ? 1 : 0;
