@@ -708,6 +708,43 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
       __builtin_amdgcn_wave_barrier();
     };
 
+    // ---- XXX 3iso on the matrix cores: point 1's third points (k > i with
+    // b31 >= lo) are its own pairs of b12 >= lo (i < j, in range), already
+    // listed in bin order with exact bins and images and in point 1's frame:
+    // copied from the pair list instead of walking the neighbour cells again
+    // in every row launch.
+    if constexpr (MF && !ANI) {
+      if (a.auto_) {
+        const int qb = __builtin_amdgcn_readfirstlane(poff[i * bins + a.lo]);
+        const int qe = __builtin_amdgcn_readfirstlane(poff[i * bins + bins]);
+        for (int kb = qb; kb < qe; kb += kWave) {
+          const int q = kb + lane;
+          const bool v = q < qe;
+          PairRec r;
+          r.xj = r.yj = r.zj = 0.f;
+          r.wj = 0.f;
+          r.j = -1;
+          r.meta = 0;
+          if (v) r = pairs[q];
+          const unsigned long long m = __ballot(v);
+          const int cnt = __popcll(m);
+          if (n + cnt > CAP) run_list_mf();
+          const int pos = n + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+          if (v) {
+            const float ax = r.xj - is.x, ay = r.yj - is.y, az = r.zj - is.z;
+            Lq[pos] = make_float4(ax, ay, az, __builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax)));
+            Lw[pos] = r.wj;
+            Lk[pos] = r.j;
+            Lm[pos] = ((r.meta >> 8) & 0xfff) | ((r.meta & 31) << 16);
+          }
+          n += cnt;
+        }
+        if (n > 0) run_list_mf();
+        continue;
+      }
+    }
+
     // ---- phase A: point 1's third points, one per lane
     for_each_segment(io, a, k_cell_start, [&](int k0, int k1, int sx, int sy, int sz) {
       const float sxL = (float)sx * a.Ls, syL = (float)sy * a.Ls, szL = (float)sz * a.Ls;
