@@ -300,6 +300,7 @@ struct TripArgs {
   double R, L, ds_th;
   int lo, hi;     // b12 slice
   int glob;       // 3iso: V rows in global memory (one b12 row exceeds LDS)
+  int cs;         // 3iso LDS slice: 2^cs interleaved copies of each V row (lane & (2^cs - 1))
   float reach;    // rmax plus slack (unscaled)
   int K[3];       // neighbour reach in cells
   GridGeom g;
@@ -433,10 +434,13 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
   const int bins = a.bins;
   const int bins2 = bins * bins;
   const int nrows = (ANI || GLOB) ? 0 : (a.hi - a.lo) * bins2;
+  // Each V row has 2^cs copies side by side; a lane deposits into copy
+  // lane & (2^cs - 1), so lanes meeting on one row hit distinct words.
+  const int ncopy = nrows << a.cs;
   uint32_t* hc = reinterpret_cast<uint32_t*>(smem);
-  const size_t hc_bytes = ((size_t)nrows * 4 + 15) & ~(size_t)15;
+  const size_t hc_bytes = ((size_t)ncopy * 4 + 15) & ~(size_t)15;
   double* hw = reinterpret_cast<double*>(smem + hc_bytes);
-  const size_t hist_bytes = (hc_bytes + (WEIGHTED ? (size_t)nrows * 8 : 0) + 15) & ~(size_t)15;
+  const size_t hist_bytes = (hc_bytes + (WEIGHTED ? (size_t)ncopy * 8 : 0) + 15) & ~(size_t)15;
   const int wave = threadIdx.x / kWave;
   float4* Lq = reinterpret_cast<float4*>(smem + hist_bytes) + wave * CAP;
   int* Lk = reinterpret_cast<int*>(smem + hist_bytes + (size_t)(BLOCK / kWave) * CAP * 16) +
@@ -450,7 +454,7 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
     for (int t = threadIdx.x; t < a.nt; t += BLOCK) tab[t] = table[t];
   for (int t = threadIdx.x; t <= bins; t += BLOCK) sT[t] = T[t];
   if (!ANI) {
-    for (int t = threadIdx.x; t < nrows; t += BLOCK) {
+    for (int t = threadIdx.x; t < ncopy; t += BLOCK) {
       hc[t] = 0u;
       if (WEIGHTED) hw[t] = 0.0;
     }
@@ -468,8 +472,9 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
       atomicAdd(&hist[vrow], 1ull);
       if (WEIGHTED) atomicAdd(&whist[vrow], w);
     } else {
-      atomicAdd(&hc[vrow - vrow0], 1u);
-      if (WEIGHTED) atomicAdd(&hw[vrow - vrow0], w);
+      const unsigned slot = ((vrow - vrow0) << a.cs) | (threadIdx.x & ((1u << a.cs) - 1u));
+      atomicAdd(&hc[slot], 1u);
+      if (WEIGHTED) atomicAdd(&hw[slot], w);
     }
   };
 
@@ -825,10 +830,17 @@ __global__ __launch_bounds__(ANI ? kAniTripBlock : kTripBlock) void k_triplets(
   if (!ANI) {
     __syncthreads();
     const int base = a.lo * bins2;
+    const int C = 1 << a.cs;
     for (int t = threadIdx.x; t < nrows; t += BLOCK) {
-      if (hc[t]) {
-        atomicAdd(&hist[base + t], (unsigned long long)hc[t]);
-        if (WEIGHTED) atomicAdd(&whist[base + t], hw[t]);
+      unsigned long long cnt = 0;
+      double w = 0.0;
+      for (int q = 0; q < C; ++q) {
+        cnt += hc[t * C + q];
+        if (WEIGHTED) w += hw[t * C + q];
+      }
+      if (cnt) {
+        atomicAdd(&hist[base + t], cnt);
+        if (WEIGHTED) atomicAdd(&whist[base + t], w);
       }
     }
   }
@@ -1559,6 +1571,7 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
   t.ds_th = (double)ds_th_f;
   t.reach = reach;
   t.glob = 0;
+  t.cs = 0;
   for (int q = 0; q < 3; ++q) t.K[q] = Kc[q];
   t.g = g;
   // The reference's node order of the third-point catalog (refchain only).
@@ -1604,14 +1617,29 @@ int run_triplets(pcf_ctx* c, const pcf_job* job, uint64_t* counts, double* weigh
       t.glob = glob ? 1 : 0;
       // XXX takes one b12 row per launch: its third points are then cut to
       // b31 >= b12 (each unordered triplet deposited once, k_sym_cyc).
+      // Copies of each V row in LDS: the FP64 weight atomics of lanes that
+      // meet on one row are what conflict.  XXX's one row leaves LDS to spare
+      // for 8; weighted XXY takes 4 at the price of more, narrower slices
+      // (measured at 30 bins, rmax 80: weighted DDD 166 -> 129 ms, DDR
+      // 362 -> 350 ms; unweighted XXY slower with copies, XXX unchanged).
+      // The plain FP64 variant keeps one copy: copies made its weighted step
+      // slower (4.5 -> 5.1 s).
+      int cs = 0;
+      if (!glob && !plain) {
+        int cmax = 0;
+        while (cmax < 3 && (per_b12 << (cmax + 1)) + 16 <= budget) ++cmax;
+        cs = auto_ ? cmax : (weighted ? std::min(cmax, 2) : 0);
+      }
+      t.cs = cs;
       const int S =
           auto_ ? 1
                 : (glob ? bins
-                        : (int)std::max<size_t>(1, std::min<size_t>(bins, (budget - 16) / per_b12)));
+                        : (int)std::max<size_t>(
+                              1, std::min<size_t>(bins, (budget - 16) / (per_b12 << cs))));
       for (int lo = 0; lo < bins; lo += S) {
         t.lo = lo;
         t.hi = std::min(bins, lo + S);
-        const int nrows = glob ? 0 : (t.hi - t.lo) * bins * bins;
+        const int nrows = glob ? 0 : ((t.hi - t.lo) * bins * bins) << cs;
         const size_t lds =
             ((((size_t)nrows * 4 + 15) / 16 * 16 + (weighted ? (size_t)nrows * 8 : 0) + 15) &
              ~(size_t)15) + lists;
