@@ -63,6 +63,10 @@ struct BatchPlan {
   std::vector<const float4*> regrid_raw;
   std::vector<int64_t> regrid_n;
   DevBuf regrid_cells, regrid_perm;
+  // Regrid of up to kMultiGridMax catalogs of one z sub-slab count in one
+  // counting sort (fewer dependent launches in the step).
+  bool regrid_joint = false;
+  MultiGridArgs regrid_mg{};
   // scaled copies: one per distinct (catalog, group scale)
   std::vector<const float4*> scale_src;
   std::vector<int64_t> scale_n;
@@ -112,7 +116,11 @@ hipError_t enqueue_plan(pcf_ctx* c, BatchPlan* p, size_t njobs, bool prepare, bo
   float4* sc = p->scaled.as<float4>();
   const size_t hist_bytes = sizeof(unsigned long long) * (size_t)p->total_rows;
   const size_t w_bytes = sizeof(double) * (size_t)p->total_rows;
-  if (part == 0 && !prepare && regrid) {
+  if (part == 0 && !prepare && regrid && p->regrid_joint) {
+    if ((e = build_grids_into(p->regrid_mg, p->g, p->grids[0]->zq, p->regrid_cells.p,
+                              p->regrid_perm.as<int>(), c->stream)) != hipSuccess)
+      return e;
+  } else if (part == 0 && !prepare && regrid) {
     for (size_t k = 0; k < p->grids.size(); ++k)
       if ((e = build_grid_into(p->regrid_raw[k], p->regrid_n[k], p->g, p->grids[k]->pts,
                                p->grids[k]->cell_start, p->grids[k]->fine_start,
@@ -431,9 +439,35 @@ int make_plan(pcf_ctx* c, const pcf_job* jobs, int njobs, BatchPlan* p, GridPins
     // Each grid is rebuilt in its own point order (z sub-slabs): the plan's
     // step windows assume it.
     int zq = 1;
-    for (DevGrid* g : p->grids) zq = std::max(zq, g->zq);
-    HIPCHK(c, p->regrid_cells.ensure(grid_cells_tmp_bytes(p->g.ncell() * zq)));
-    HIPCHK(c, p->regrid_perm.ensure(2 * sizeof(int) * (size_t)nmax));
+    bool same_zq = true;
+    int64_t ntot = 0;
+    for (DevGrid* g : p->grids) {
+      zq = std::max(zq, g->zq);
+      same_zq = same_zq && g->zq == p->grids[0]->zq;
+    }
+    for (int64_t nk : p->regrid_n) ntot += nk;
+    const int m = (int)p->grids.size();
+    p->regrid_joint = m >= 2 && m <= kMultiGridMax && same_zq &&
+                      (int64_t)m * p->g.ncell() * zq + 1 <= (int64_t)INT32_MAX &&
+                      ntot <= (int64_t)INT32_MAX;
+    if (p->regrid_joint) {
+      MultiGridArgs& mg = p->regrid_mg;
+      mg = MultiGridArgs{};
+      mg.m = m;
+      mg.off[0] = 0;
+      for (int k = 0; k < m; ++k) {
+        mg.raw[k] = p->regrid_raw[k];
+        mg.pts[k] = p->grids[k]->pts;
+        mg.cell_start[k] = p->grids[k]->cell_start;
+        mg.fine_start[k] = p->grids[k]->fine_start;
+        mg.off[k + 1] = mg.off[k] + p->regrid_n[k];
+      }
+      HIPCHK(c, p->regrid_cells.ensure(grid_cells_tmp_bytes((int64_t)m * p->g.ncell() * zq)));
+      HIPCHK(c, p->regrid_perm.ensure(2 * sizeof(int) * (size_t)std::max<int64_t>(ntot, 1)));
+    } else {
+      HIPCHK(c, p->regrid_cells.ensure(grid_cells_tmp_bytes(p->g.ncell() * zq)));
+      HIPCHK(c, p->regrid_perm.ensure(2 * sizeof(int) * (size_t)nmax));
+    }
   }
 
   // Capture the run into a graph (serving lanes run their batches directly:

@@ -108,6 +108,50 @@ __global__ void k_scatter(const float4* __restrict__ raw, int64_t n, const int* 
   out[cell_start[cell_of[i]] + rank[i]] = raw[i];
 }
 
+// Several catalogs on one geometry in one counting sort: catalog k's keys
+// are offset by k * nk, so one scan gives the starts of every catalog, each
+// offset by the points of the catalogs before it (off[k]).
+__device__ __forceinline__ int multi_cat(const MultiGridArgs& mg, int64_t t) {
+  int k = 0;
+  while (k + 1 < mg.m && t >= mg.off[k + 1]) ++k;
+  return k;
+}
+
+__global__ void k_cell_index_multi(MultiGridArgs mg, GridGeom g, int zq, int64_t nk,
+                                   int* __restrict__ cell_of, int* __restrict__ rank,
+                                   int* __restrict__ counts) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= mg.off[mg.m]) return;
+  const int k = multi_cat(mg, t);
+  const float4 p = mg.raw[k][t - mg.off[k]];
+  const int cx = cell_axis(p.x, g.lo[0], g.inv_cs[0], g.P[0]);
+  const int cy = cell_axis(p.y, g.lo[1], g.inv_cs[1], g.P[1]);
+  const int c = (int)((int64_t)k * nk) + (cx * g.P[1] + cy) * g.P[2] * zq + zkey(p.z, g, zq);
+  cell_of[t] = c;
+  rank[t] = atomicAdd(&counts[c], 1);
+}
+
+// Catalog k's fine starts (zq > 1) and cell starts from the joint scan.
+__global__ void k_split_starts(MultiGridArgs mg, const int* __restrict__ starts, int64_t nc,
+                               int zq) {
+  const int64_t nk = nc * zq;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)mg.m * (nk + 1)) return;
+  const int k = (int)(t / (nk + 1));
+  const int64_t q = t - (int64_t)k * (nk + 1);
+  const int v = starts[(int64_t)k * nk + q] - (int)mg.off[k];
+  if (zq > 1) mg.fine_start[k][q] = v;
+  if (q % zq == 0) mg.cell_start[k][q / zq] = v;
+}
+
+__global__ void k_scatter_multi(MultiGridArgs mg, const int* __restrict__ cell_of,
+                                const int* __restrict__ rank, const int* __restrict__ starts) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= mg.off[mg.m]) return;
+  const int k = multi_cat(mg, t);
+  mg.pts[k][starts[cell_of[t]] + rank[t] - (int)mg.off[k]] = mg.raw[k][t - mg.off[k]];
+}
+
 __global__ void k_to_double(const float4* __restrict__ in, int64_t n, double4* __restrict__ out) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -207,6 +251,44 @@ hipError_t build_grid_into(const float4* raw, int64_t n, const GridGeom& g, floa
   if (n > 0) {
     hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, raw, n, cell_of, rank, starts,
                        pts);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t build_grids_into(const MultiGridArgs& mg, const GridGeom& g, int zq, void* cells_tmp,
+                            int* perm_tmp, hipStream_t s) {
+  hipError_t e;
+  if (zq < 1 || mg.m < 1 || mg.m > kMultiGridMax) return hipErrorInvalidValue;
+  const int64_t nc = g.ncell();
+  const int64_t nk = nc * zq;
+  const int64_t keys = (int64_t)mg.m * nk;
+  const int64_t n = mg.off[mg.m];
+  if (keys + 1 > (int64_t)INT32_MAX || n > (int64_t)INT32_MAX) return hipErrorInvalidValue;
+  for (int k = 0; k < mg.m; ++k)
+    if (!mg.cell_start[k] || (zq > 1 && !mg.fine_start[k])) return hipErrorInvalidValue;
+  int* counts = reinterpret_cast<int*>(cells_tmp);
+  const size_t cbytes = (sizeof(int) * (size_t)(keys + 1) + 15) & ~(size_t)15;
+  int* cell_of = perm_tmp;
+  int* rank = cell_of + (n > 0 ? n : 1);
+  if ((e = launch_zero(counts, sizeof(int) * (size_t)(keys + 1), s)) != hipSuccess) return e;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_cell_index_multi, dim3(blocks), dim3(256), 0, s, mg, g, zq, nk, cell_of,
+                       rank, counts);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  // In place: counts become the joint starts.
+  if ((e = launch_scan(counts, keys, counts,
+                       reinterpret_cast<long long*>(reinterpret_cast<char*>(cells_tmp) + cbytes),
+                       s)) != hipSuccess)
+    return e;
+  const int64_t nsplit = (int64_t)mg.m * (nk + 1);
+  hipLaunchKernelGGL(k_split_starts, dim3((unsigned)((nsplit + 255) / 256)), dim3(256), 0, s, mg,
+                     counts, nc, zq);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (n > 0) {
+    hipLaunchKernelGGL(k_scatter_multi, dim3(blocks), dim3(256), 0, s, mg, cell_of, rank, counts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   return hipSuccess;

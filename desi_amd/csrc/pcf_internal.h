@@ -205,6 +205,22 @@ size_t grid_cells_tmp_bytes(int64_t ncell);
 hipError_t build_grid_into(const float4* raw, int64_t n, const GridGeom& g, float4* pts,
                            int* cell_start, int* fine_start, int zq, void* cells_tmp,
                            int* perm_tmp, hipStream_t s);
+// Up to kMultiGridMax catalogs gridded on one geometry by one counting sort
+// (build_grids_into): catalog k is raw[k][0 .. off[k+1] - off[k]), off[0] = 0,
+// and its grid goes to pts[k], cell_start[k] and, with zq > 1, fine_start[k],
+// as build_grid_into would write them.  Temporaries: cells_tmp of
+// grid_cells_tmp_bytes(m * ncell * zq), perm_tmp of 2 * off[m] ints.
+constexpr int kMultiGridMax = 8;
+struct MultiGridArgs {
+  const float4* raw[kMultiGridMax];
+  float4* pts[kMultiGridMax];
+  int* cell_start[kMultiGridMax];
+  int* fine_start[kMultiGridMax];
+  int64_t off[kMultiGridMax + 1];
+  int m;
+};
+hipError_t build_grids_into(const MultiGridArgs& mg, const GridGeom& g, int zq, void* cells_tmp,
+                            int* perm_tmp, hipStream_t s);
 hipError_t convert_grid64(const float4* pts, int64_t n, double4* out, hipStream_t s);
 hipError_t build_grid64(DevGrid* grid, hipStream_t s);
 

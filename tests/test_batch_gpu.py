@@ -205,3 +205,39 @@ def test_heterogeneous_batch(engine, type_, bpc):
     finally:
         for c in (cd, c1, c2):
             engine.release(c)
+
+
+@pytest.mark.parametrize("type_", [da.PCF_2ISO, da.PCF_2ANI])
+@pytest.mark.parametrize("ncat", [5, 10])
+def test_regrid_batch_joint_sort(engine, type_, ncat):
+    """Batches that re-grid their catalogs: up to eight catalogs are sorted in
+    one joint counting sort, more one by one.  Catalogs of 0, 1 and thousands
+    of points, periodic box: every histogram equals its single run (grids
+    built one catalog at a time) and the oracle, on the first run and on the
+    graph replays."""
+    box, dmax = 120.0, 17.0
+    sizes = [2500, 0, 1, 1700, 4000, 900, 3, 2200, 1200, 60][:ncat]
+    cats = [synth.neyman_scott(max(1, n // 20), 20, 2.0, box, seed=300 + k, weights="uniform")[:n]
+            if k % 2 == 0 else synth.uniform(n, box, seed=300 + k, weights="uniform")
+            for k, n in enumerate(sizes)]
+    ids = [engine.upload(x) for x in cats]
+    try:
+        jobs = [da.make_job(type_, da.PCF_XX, bins=12, dmax=dmax, cat_x=ids[0], bpc=True, box=box,
+                            rebuild_grid=True)]
+        for k in range(1, ncat):
+            jobs.append(da.make_job(type_, da.PCF_XX, bins=12, dmax=dmax, cat_x=ids[k], bpc=True,
+                                    box=box, rebuild_grid=True))
+            jobs.append(da.make_job(type_, da.PCF_XY, bins=12, dmax=dmax, cat_x=ids[0],
+                                    cat_y=ids[k], bpc=True, box=box, rebuild_grid=True))
+        single = _singles(engine, jobs)
+        for rep in range(3):
+            _same(engine.histogram_batch(jobs), single)
+        for k in (0, 3, 4):
+            c, w = single[0 if k == 0 else 2 * k - 1]
+            co, wo = O.pairs(cats[k], None, ani=type_ == da.PCF_2ANI, bpc=True, box=box, bins=12,
+                             dmax=dmax)
+            assert np.array_equal(c, co), f"catalog {k}"
+            assert np.all(np.abs(w - wo) <= 1e-12 * np.maximum(np.abs(wo), 1e-300))
+    finally:
+        for i in ids:
+            engine.release(i)
