@@ -50,6 +50,11 @@ BINS = 30
 # Counter summary of the tuned pair kernel by form (the Gram form is the one the autotuner
 # picks on this workload; another pick reports traffic null).
 PAIR_PROFILE = {2: os.path.join(ROOT, "profiles", "k_pairs_gram_2iso_N1M_box1000_rmax80.md")}
+# HBM traffic of the triplet and batch legs' counting kernels (average bytes per
+# dispatch of the timed step), from their committed counter summaries.
+ISO_PROFILE = os.path.join(ROOT, "profiles", "k_triplets_3iso_bpc_N32K_box250_rmax80.md")
+ANI_PROFILE = os.path.join(ROOT, "profiles", "k_trip_ani_lds_3ani_N64K_box500_rmax60.md")
+BATCH_PROFILE = os.path.join(ROOT, "profiles", "k_pairs_mixed_batch_2ani_64randoms.md")
 
 
 def profiled_traffic(path, kernel):
@@ -212,7 +217,8 @@ def triplet_ani_leg(e, steps):
             "speedup_vs_fp64_plain": out["fp64_plain"]["ms"] / m["ms"],
             "parity_vs_fp64": parity,
             "roofline": {"bound": "valu", "achieved": ach / 1e12, "peak": PEAK_FP32 / 1e12,
-                         "unit": "TFLOP/s", "frac": ach / PEAK_FP32, "traffic": None}}
+                         "unit": "TFLOP/s", "frac": ach / PEAK_FP32,
+                         "traffic": profiled_traffic(ANI_PROFILE, "k_trip_ani_lds 3ani")}}
 
 
 def triplet_leg(e, steps):
@@ -251,7 +257,8 @@ def triplet_leg(e, steps):
         "parity_vs_fp64": parity,
         # the 2-3 side of every triplet is formed on the matrix cores
         "roofline": {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK_FP32 / 1e12,
-                     "unit": "TFLOP/s", "frac": ach / PEAK_FP32, "traffic": None},
+                     "unit": "TFLOP/s", "frac": ach / PEAK_FP32,
+                     "traffic": profiled_traffic(ISO_PROFILE, "k_triplets 3iso")},
     }
 
 
@@ -318,7 +325,8 @@ def batch_leg(e, steps, no_tune=False):
             "speedup_vs_unbatched": (t4 - t3) * 1e3 / ms,
             "parity_vs_fp64": parity,
             "roofline": {"bound": "valu", "achieved": ach / 1e12, "peak": PEAK_FP32 / 1e12,
-                         "unit": "TFLOP/s", "frac": ach / PEAK_FP32, "traffic": None}}
+                         "unit": "TFLOP/s", "frac": ach / PEAK_FP32,
+                         "traffic": profiled_traffic(BATCH_PROFILE, "k_pairs_mixed batch")}}
 
 
 # Expected ordered triplets above which a 3iso request of the serving mix is
